@@ -1,0 +1,170 @@
+/*
+ * mvoro.h -- C ABI of the MI355X-native per-cell Voronoi construction
+ * (libmvoro.so, gfx950 HIP kernels).
+ *
+ * Drop-in boundary for the construction path of the meshless_voronoi crate
+ * (v0.7.2).  Every entry point names the reference item it replaces.  The
+ * reference has no FFI of its own (it is a Rust crate); the binding stub a
+ * Rust caller would add is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - positions are f64 AoS [x, y, z] with a 24-byte stride, which is the
+ *    memory layout of glam's DVec3, so a &[DVec3] can be passed as-is;
+ *  - masks are one byte per generator (0 = inactive), as Rust's &[bool];
+ *  - every call blocks until its results are complete;
+ *  - errors are returned as mv_status, one code per panic site of the
+ *    reference (it panics where this library returns an error);
+ *  - a handle is immutable after the build returns and may be read from any
+ *    thread; builds on different handles may run concurrently.
+ */
+#ifndef MVORO_H
+#define MVORO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mv_status {
+  MV_OK = 0,
+  MV_E_ARG = 1,               /* bad argument (also: mask shorter than N,
+                                 reference src/voronoi.rs:286) */
+  MV_E_DEGENERATE_POINTS = 2, /* zero or non-finite generator distance,
+                                 src/voronoi/convex_cell.rs:342 */
+  MV_E_SELF_NOT_FIRST = 3,    /* duplicate generators,
+                                 src/voronoi/convex_cell.rs:325-329 */
+  MV_E_PLANE_DET_ZERO = 4,    /* src/geometry.rs:70 */
+  MV_E_NAN = 5,               /* src/voronoi/convex_cell.rs:448 */
+  MV_E_BOUNDARY_WALK = 6,     /* src/voronoi/convex_cell.rs:428 */
+  MV_E_DEVICE = 7,            /* HIP runtime error or no GPU */
+  MV_E_OOM = 8,               /* device or host allocation failed */
+  MV_E_CAPACITY = 9           /* a cell exceeded the largest per-cell
+                                 capacity (planes / vertices / neighbours) */
+} mv_status;
+
+/* Construction modes. */
+enum {
+  MV_MODE_PLAIN = 0,  /* fixed k neighbours per pass, k doubled for the cells
+                         whose security radius was not reached */
+  MV_MODE_LEARNED = 1 /* per-cell k proposed by the learned model, cells
+                         bucketed by k and run in one persistent kernel;
+                         the security test still decides every cell */
+};
+
+typedef struct mv_options {
+  uint32_t mode;     /* MV_MODE_*; default MV_MODE_PLAIN */
+  uint32_t k0;       /* neighbours in the first pass, self included
+                        (plain mode); default 27 */
+  uint32_t k_max;    /* largest neighbour list tried; default 4096 */
+  uint32_t lanes;    /* cells in flight (thread-per-cell lanes); 0 = auto */
+  uint32_t reserved[8];
+} mv_options;
+
+/* Fill *opt with the defaults. */
+void mv_default_options(mv_options *opt);
+
+typedef struct mv_tess mv_tess;
+
+/* Voronoi::build (src/voronoi.rs:151-159) when mask == NULL and
+ * Voronoi::build_partial (src/voronoi.rs:181-190) otherwise.
+ * xyz, mask: host pointers.  dimensionality: 1, 2 or 3.  Results are
+ * downloaded to host memory owned by *out. */
+mv_status mv_build(const double *xyz, size_t n, const uint8_t *mask,
+                   const double anchor[3], const double width[3],
+                   uint32_t dimensionality, uint32_t periodic,
+                   const mv_options *opt, mv_tess **out);
+
+/* Same as mv_build with device-resident inputs (d_xyz, d_mask are device
+ * pointers on the current HIP device); the results stay in device memory
+ * (see mv_device_view) and are downloaded on the first host accessor
+ * call. */
+mv_status mv_build_device(const double *d_xyz, size_t n,
+                          const uint8_t *d_mask, const double anchor[3],
+                          const double width[3], uint32_t dimensionality,
+                          uint32_t periodic, const mv_options *opt,
+                          mv_tess **out);
+
+void mv_free(mv_tess *t);
+
+/* Voronoi::anchor / width / dimensionality / periodic
+ * (src/voronoi.rs:317-359); anchor and width after the 1D/2D normalisation
+ * of src/voronoi.rs:200-210. */
+void mv_anchor(const mv_tess *t, double out[3]);
+void mv_width(const mv_tess *t, double out[3]);
+uint32_t mv_dimensionality(const mv_tess *t);
+uint32_t mv_periodic(const mv_tess *t);
+
+/* sizes: Voronoi::cells().len(), faces().len(),
+ * cell_face_connections().len() */
+size_t mv_n_cells(const mv_tess *t);
+size_t mv_n_faces(const mv_tess *t);
+size_t mv_n_links(const mv_tess *t);
+
+/* VoronoiCell accessors (src/voronoi/voronoi_cell.rs:107-165), SoA, host
+ * memory valid until mv_free.  Inactive cells of a partial build are all
+ * zero (VoronoiCell::default), including loc and idx. */
+const double *mv_cell_loc(mv_tess *t);           /* N x 3 */
+const double *mv_cell_centroid(mv_tess *t);      /* N x 3 */
+const double *mv_cell_volume(mv_tess *t);        /* N */
+const double *mv_cell_safety_radius(mv_tess *t); /* N */
+const uint64_t *mv_cell_face_offset(mv_tess *t); /* face_connections_offset */
+const uint64_t *mv_cell_face_count(mv_tess *t);  /* face_count */
+const uint64_t *mv_cell_idx(mv_tess *t);         /* N (0 when inactive) */
+
+/* VoronoiFace accessors (src/voronoi/voronoi_face.rs:241-300). */
+const uint64_t *mv_face_left(mv_tess *t);
+const int64_t *mv_face_right(mv_tess *t);   /* -1 = None (boundary face) */
+const double *mv_face_area(mv_tess *t);
+const double *mv_face_centroid(mv_tess *t); /* F x 3 */
+const double *mv_face_normal(mv_tess *t);   /* F x 3 (plane normal of the
+                                               left cell's half-space) */
+const uint8_t *mv_face_has_shift(mv_tess *t);
+const double *mv_face_shift(mv_tess *t);    /* F x 3 (0 when no shift) */
+
+/* Voronoi::cell_face_connections (src/voronoi.rs:346-349). */
+const uint64_t *mv_cell_face_connections(mv_tess *t);
+
+/* Device-resident results (same layouts as the host accessors). */
+typedef struct mv_device_view {
+  const double *cell_loc, *cell_centroid, *cell_volume, *cell_safety_radius;
+  const uint64_t *cell_face_offset, *cell_face_count, *cell_idx;
+  const uint64_t *face_left;
+  const int64_t *face_right;
+  const double *face_area, *face_centroid, *face_normal;
+  const uint8_t *face_has_shift;
+  const double *face_shift;
+  const uint64_t *cell_face_connections;
+} mv_device_view;
+void mv_get_device_view(const mv_tess *t, mv_device_view *v);
+
+/* Construction statistics (no counterpart in the reference). */
+typedef struct mv_stats {
+  uint32_t passes;          /* neighbour-list passes run */
+  uint32_t k_last;          /* list length of the last pass */
+  uint64_t cells_redone;    /* cells rebuilt after an insufficient k */
+  uint64_t exact_tests;     /* exact in-sphere evaluations */
+  uint64_t big_cells;       /* cells run with the large-capacity kernel */
+  double ms_grid, ms_knn, ms_clip, ms_finalize, ms_total;
+} mv_stats;
+void mv_get_stats(const mv_tess *t, mv_stats *s);
+
+/* Per-cell construction record (no counterpart in the reference):
+ * number of neighbour-stream entries consumed, self and the entry that
+ * triggered the security test included (0 for inactive cells). */
+const uint32_t *mv_cell_n_popped(mv_tess *t);
+
+const char *mv_status_string(mv_status s);
+
+/* Exact in-sphere sign on snapped integer coordinates, evaluated on the
+ * device for `count` 5-tuples (a, b, c, d, v; 15 int64 per tuple, host
+ * memory).  Replaces in_sphere_test_exact (src/geometry.rs:271-314);
+ * writes -1, 0 or +1 per tuple. */
+mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,
+                             int8_t *out_sign);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,115 @@
+"""Loading and building libmvoro.so (the gfx950 HIP library behind the C ABI
+of include/mvoro.h).
+
+There is no CPU fallback: when the library is missing or no GPU is present,
+the entry points raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_PKG)
+LIB_PATH = os.path.join(_PKG, "libmvoro.so")
+SOURCES = [os.path.join(_PKG, "csrc", f) for f in (
+    "mvoro.hip", "mv_common.hpp", "mv_geom.hpp", "mv_grid.hpp", "mv_clip.hpp",
+    "mv_finalize.hpp")]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # the reference never fuses multiply-add; clip decisions must see the
+    # same bits
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libmvoro.so in-tree with hipcc for gfx950."""
+    cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(_ROOT, "include"),
+           "-o", LIB_PATH + ".tmp", os.path.join(_PKG, "csrc", "mvoro.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    if verbose:
+        print(" ".join(cmd))
+    return LIB_PATH
+
+
+_lib = None
+
+
+class mv_options(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("k0", C.c_uint32), ("k_max", C.c_uint32),
+                ("lanes", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+
+
+class mv_stats(C.Structure):
+    _fields_ = [("passes", C.c_uint32), ("k_last", C.c_uint32),
+                ("cells_redone", C.c_uint64), ("exact_tests", C.c_uint64),
+                ("big_cells", C.c_uint64), ("ms_grid", C.c_double),
+                ("ms_knn", C.c_double), ("ms_clip", C.c_double),
+                ("ms_finalize", C.c_double), ("ms_total", C.c_double)]
+
+
+class mv_device_view(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "cell_loc", "cell_centroid", "cell_volume", "cell_safety_radius",
+        "cell_face_offset", "cell_face_count", "cell_idx", "face_left",
+        "face_right", "face_area", "face_centroid", "face_normal",
+        "face_has_shift", "face_shift", "cell_face_connections")]
+
+
+# every symbol declared in include/mvoro.h
+EXPORTS = [
+    "mv_default_options", "mv_build", "mv_build_device", "mv_free", "mv_anchor",
+    "mv_width", "mv_dimensionality", "mv_periodic", "mv_n_cells", "mv_n_faces",
+    "mv_n_links", "mv_cell_loc", "mv_cell_centroid", "mv_cell_volume",
+    "mv_cell_safety_radius", "mv_cell_face_offset", "mv_cell_face_count",
+    "mv_cell_idx", "mv_face_left", "mv_face_right", "mv_face_area",
+    "mv_face_centroid", "mv_face_normal", "mv_face_has_shift", "mv_face_shift",
+    "mv_cell_face_connections", "mv_get_device_view", "mv_get_stats",
+    "mv_cell_n_popped", "mv_status_string", "mv_in_sphere_exact",
+]
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(hipcc, gfx950); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u32 = C.c_void_p, C.c_size_t, C.c_uint32
+    L.mv_default_options.argtypes = [C.POINTER(mv_options)]
+    L.mv_build.restype = C.c_int
+    L.mv_build.argtypes = [vp, sz, vp, vp, vp, u32, u32, C.POINTER(mv_options), C.POINTER(vp)]
+    L.mv_build_device.restype = C.c_int
+    L.mv_build_device.argtypes = [vp, sz, vp, vp, vp, u32, u32, C.POINTER(mv_options), C.POINTER(vp)]
+    L.mv_free.argtypes = [vp]
+    L.mv_anchor.argtypes = [vp, vp]
+    L.mv_width.argtypes = [vp, vp]
+    for f in ("mv_dimensionality", "mv_periodic"):
+        getattr(L, f).restype = u32
+        getattr(L, f).argtypes = [vp]
+    for f in ("mv_n_cells", "mv_n_faces", "mv_n_links"):
+        getattr(L, f).restype = sz
+        getattr(L, f).argtypes = [vp]
+    for f in EXPORTS:
+        if f.startswith(("mv_cell_", "mv_face_")):
+            getattr(L, f).restype = vp
+            getattr(L, f).argtypes = [vp]
+    L.mv_get_device_view.argtypes = [vp, C.POINTER(mv_device_view)]
+    L.mv_get_stats.argtypes = [vp, C.POINTER(mv_stats)]
+    L.mv_status_string.restype = C.c_char_p
+    L.mv_status_string.argtypes = [C.c_int]
+    L.mv_in_sphere_exact.restype = C.c_int
+    L.mv_in_sphere_exact.argtypes = [vp, sz, vp]
+    _lib = L
+    return L

@@ -1,0 +1,501 @@
+// mv_clip.hpp -- per-cell construction: convex-cell clipping by the
+// neighbour half-spaces with the reference's security-radius stop, then the
+// decomposition into oriented tets with the volume/centroid and per-face
+// area/centroid integrals.
+//
+// One cell per lane.  A lane runs the reference's sequential algorithm
+// (src/voronoi/convex_cell.rs:294-453, 131-187; voronoi_cell.rs:40-100)
+// step for step, so its vertex order, face order and every float
+// operation match the reference's; the cell's plane, vertex and cycle
+// arrays live in device memory interleaved across lanes (element j of lane
+// l at [j * lanes + l]), which keeps the accesses of a wavefront coalesced
+// while its lanes walk their arrays in step.
+#pragma once
+#include "mv_common.hpp"
+
+namespace mv {
+
+struct ClipArgs {
+  const double4 *spos;   // sorted generators, w = generator index bits
+  const uint8_t *mask;   // by generator index, nullptr = all active
+  BoxParams box;
+  uint64_t total_stream;
+  const uint32_t *work;  // slots to build
+  uint32_t nwork;
+  const uint64_t *kpk;   // neighbour lists, [j * nwork + w]
+  const uint32_t *kcnt;
+  uint32_t L;
+  // lane state
+  double4 *plA;   // plane: n.xyz, d
+  double4 *plB;   // plane: p.xyz, reference word (packed entry, ~0 = box)
+  double4 *vtx;   // vertex: loc.xyz, radius2
+  uint32_t *vdual;
+  uint8_t *cyc;   // SimpleCycle pointers, reused as face flags
+  double4 *facc;  // face accumulators: area, centroid sums
+  uint32_t lanes;
+  // outputs
+  uint8_t *wstate;       // CellState per work item
+  double *cell_vol;      // by generator index
+  double *cell_cen;
+  double *cell_sr;
+  uint32_t *cell_nf;
+  uint64_t *cell_fbase;
+  uint32_t *cell_npop;
+  // face staging
+  uint32_t *st_cell;
+  int64_t *st_right;
+  uint8_t *st_img;
+  double *st_area;
+  double *st_cen;
+  double *st_norm;
+  uint64_t st_cap;
+  unsigned long long *st_counter;
+  unsigned long long *err_word;  // min over (idx << 8 | code)
+  unsigned long long *exact_counter;
+  unsigned long long *overflow;  // set when the staging area is too small
+};
+
+constexpr uint64_t BOX_REF = ~0ull;
+
+enum ErrCode : uint32_t {
+  E_DEGENERATE = 2,
+  E_SELF_NOT_FIRST = 3,
+  E_DET_ZERO = 4,
+  E_NAN = 5,
+  E_WALK = 6
+};
+
+struct Lane {
+  double4 *plA, *plB, *vtx, *facc;
+  uint32_t *vdual;
+  uint8_t *cyc;
+  size_t lane, lanes;
+  __device__ __forceinline__ size_t at(int j) const {
+    return (size_t)j * lanes + lane;
+  }
+};
+
+__device__ __forceinline__ uint32_t pack_dual(int a, int b, int c) {
+  return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16);
+}
+__device__ __forceinline__ int dual_at(uint32_t d, int k) {
+  return (int)((d >> (8 * k)) & 0xFFu);
+}
+
+struct CellRegs {
+  d3 q;
+  uint64_t idx;
+  int np, nv;
+  int cstart, clen;  // SimpleCycle start and len
+  double sr;         // safety radius
+};
+
+// Vertex::from_dual (src/voronoi/convex_cell.rs:28-48)
+__device__ __forceinline__ bool make_vertex(const Lane &S, const CellRegs &c,
+                                            int dim, int i, int j, int k,
+                                            double4 *out) {
+  double4 a = S.plA[S.at(i)], b = S.plA[S.at(j)], e = S.plA[S.at(k)];
+  d3 loc;
+  if (!intersect_planes(xyz(a), a.w, xyz(b), b.w, xyz(e), e.w, &loc))
+    return false;
+  d3 dl = loc;
+  if (dim == 1) {
+    dl.y = 0.;
+    dl.z = 0.;
+  } else if (dim == 2) {
+    dl.z = 0.;
+  }
+  d3 df = sub3(c.q, dl);
+  *out = make_double4(loc.x, loc.y, loc.z, dot3(df, df));
+  return true;
+}
+
+// update_safety_radius (src/voronoi/convex_cell.rs:443-453)
+__device__ __forceinline__ bool update_safety_radius(const Lane &S,
+                                                     CellRegs &c) {
+  double m = -INFINITY;
+  for (int v = 0; v < c.nv; v++) {
+    double r = S.vtx[S.at(v)].w;
+    if (isnan(r)) return false;
+    m = fmax(m, r);
+  }
+  c.sr = 2. * sqrt(m);
+  return true;
+}
+
+// HalfSpace::right_loc (src/voronoi/half_space.rs:53-65) of stored plane j
+__device__ __forceinline__ d3 right_loc(const Lane &S, const CellRegs &c,
+                                        const ClipArgs &A, int j) {
+  double4 b = S.plB[S.at(j)];
+  uint64_t ref = (uint64_t)__double_as_longlong(b.w);
+  if (ref == BOX_REF) {
+    double4 a = S.plA[S.at(j)];
+    d3 pr = plane_project_onto(xyz(a), xyz(b), c.q);
+    return sub3(scl3(2., pr), c.q);
+  }
+  double4 g = A.spos[pk_slot(ref)];
+  d3 l = xyz(g);
+  d3 s;
+  if (image_shift(pk_img(ref), A.box.width, &s)) l = add3(l, s);
+  return l;
+}
+
+// SimpleCycle (src/simple_cycle.rs:1-101) on the lane's pointer array
+__device__ __forceinline__ void cyc_init(const Lane &S, CellRegs &c, int a,
+                                         int b, int e) {
+  int cur = c.cstart;
+  for (int i = 0; i < c.clen; i++) {
+    int nx = S.cyc[S.at(cur)];
+    S.cyc[S.at(cur)] = (uint8_t)cur;
+    cur = nx;
+  }
+  c.clen = 3;
+  c.cstart = a;
+  S.cyc[S.at(a)] = (uint8_t)b;
+  S.cyc[S.at(b)] = (uint8_t)e;
+  S.cyc[S.at(e)] = (uint8_t)a;
+}
+
+__device__ __forceinline__ bool cyc_try_extend(const Lane &S, CellRegs &c,
+                                               int a, int b, int e) {
+  int tri[3] = {a, b, e};
+  int pa = S.cyc[S.at(a)], pb = S.cyc[S.at(b)], pe = S.cyc[S.at(e)];
+  int ptr[3] = {pa, pb, pe};
+  bool con[3] = {pa != a, pb != b, pe != e};
+  for (int i = 0; i < 3; i++) {
+    int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
+    if (!con[i] && con[j] && con[k] && ptr[k] == tri[j]) {
+      S.cyc[S.at(tri[k])] = (uint8_t)tri[i];
+      S.cyc[S.at(tri[i])] = (uint8_t)tri[j];
+      c.clen += 1;
+      return true;
+    } else if (con[i] && con[j] && con[k] && ptr[k] == tri[j] &&
+               ptr[j] == tri[i]) {
+      S.cyc[S.at(tri[k])] = (uint8_t)tri[i];
+      S.cyc[S.at(tri[j])] = (uint8_t)tri[j];
+      if (c.cstart == tri[j]) c.cstart = tri[i];
+      c.clen -= 1;
+      return true;
+    }
+  }
+  return false;
+}
+
+// ConvexCell::clip_by_plane (src/voronoi/convex_cell.rs:358-419).
+// Returns 0, E_* for a reference panic, or 255 for capacity overflow.
+template <int MAXP, int MAXV>
+__device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
+                                  const ClipArgs &A, d3 n, d3 p, double d,
+                                  double errb, uint64_t ref, d3 ngb) {
+  int i = 0, num_v = c.nv;
+  while (i < num_v) {
+    double4 v = S.vtx[S.at(i)];
+    double cv = dot3(n, xyz(v)) - d;
+    int clip;
+    if (fabs(cv) < errb) {
+      // in-sphere test on snapped coordinates (convex_cell.rs:370-382)
+      uint32_t du = S.vdual[S.at(i)];
+      int64_t ia[3], ib[3], ic[3], id[3], iv[3];
+      iloc(A.box, c.q, ia);
+      iloc(A.box, right_loc(S, c, A, dual_at(du, 0)), ib);
+      iloc(A.box, right_loc(S, c, A, dual_at(du, 1)), ic);
+      iloc(A.box, right_loc(S, c, A, dual_at(du, 2)), id);
+      iloc(A.box, ngb, iv);
+      clip = in_sphere_exact(ia, ib, ic, id, iv);
+      atomicAdd(A.exact_counter, 1ull);
+    } else {
+      clip = cv < 0. ? -1 : (cv > 0. ? 1 : 0);  // NaN keeps the vertex
+    }
+    if (clip < 0) {
+      num_v -= 1;
+      double4 t = S.vtx[S.at(num_v)];
+      uint32_t td = S.vdual[S.at(num_v)];
+      S.vtx[S.at(num_v)] = v;
+      S.vdual[S.at(num_v)] = S.vdual[S.at(i)];
+      S.vtx[S.at(i)] = t;
+      S.vdual[S.at(i)] = td;
+    } else {
+      i += 1;
+    }
+  }
+  if (num_v == c.nv) return 0;
+  // push the plane and grow the cycle
+  if (c.np >= MAXP) return 255;
+  const int p_idx = c.np;
+  S.plA[S.at(p_idx)] = make_double4(n.x, n.y, n.z, d);
+  S.plB[S.at(p_idx)] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)ref));
+  S.cyc[S.at(p_idx)] = (uint8_t)p_idx;
+  c.np += 1;
+  // compute_boundary (convex_cell.rs:421-441) over the removed vertices
+  {
+    uint32_t d0 = S.vdual[S.at(num_v)];
+    cyc_init(S, c, dual_at(d0, 0), dual_at(d0, 1), dual_at(d0, 2));
+    for (int r = num_v + 1; r < c.nv; r++) {
+      int idx = r;
+      for (;;) {
+        if (idx >= c.nv) return E_WALK;
+        uint32_t du = S.vdual[S.at(idx)];
+        if (cyc_try_extend(S, c, dual_at(du, 0), dual_at(du, 1),
+                           dual_at(du, 2))) {
+          if (idx > r) {
+            S.vdual[S.at(idx)] = S.vdual[S.at(r)];
+            S.vdual[S.at(r)] = du;
+          }
+          break;
+        }
+        idx++;
+      }
+    }
+  }
+  c.nv = num_v;
+  if (c.nv + c.clen > MAXV) return 255;
+  int cur = c.cstart;
+  for (int t = 0; t < c.clen; t++) {
+    int next = S.cyc[S.at(cur)];
+    double4 nvtx;
+    if (!make_vertex(S, c, A.box.dim, cur, next, p_idx, &nvtx))
+      return E_DET_ZERO;
+    S.vtx[S.at(c.nv)] = nvtx;
+    S.vdual[S.at(c.nv)] = pack_dual(cur, next, p_idx);
+    c.nv++;
+    cur = next;
+  }
+  if (!update_safety_radius(S, c)) return E_NAN;
+  return 0;
+}
+
+// should a face be built for plane j (voronoi_cell.rs:53-73)
+__device__ __forceinline__ bool should_construct(const Lane &S,
+                                                 const CellRegs &c,
+                                                 const ClipArgs &A, int j) {
+  double4 a = S.plA[S.at(j)];
+  int dim = A.box.dim;
+  if (dim == 1 && !(a.y == 0. && a.z == 0.)) return false;
+  if (dim == 2 && !(a.z == 0.)) return false;
+  uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(j)].w);
+  if (ref != BOX_REF && pk_img(ref) == 13) {
+    uint32_t right = pk_idx(ref);
+    return (uint64_t)right > c.idx || (A.mask != nullptr && !A.mask[right]);
+  }
+  return true;
+}
+
+// Runs one cell.  Returns its CellState; on CS_DONE, *nf = faces to emit.
+template <int MAXP, int MAXV>
+__device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
+                              uint32_t w, uint32_t *nf, uint32_t *npop,
+                              uint32_t *err) {
+  const BoxParams &B = A.box;
+  *nf = 0;
+  *npop = 0;
+  // ConvexCell::init (convex_cell.rs:294-312)
+  for (int k = 0; k < 6; k++) {
+    S.plA[S.at(k)] = make_double4(B.pn[k][0], B.pn[k][1], B.pn[k][2], B.pd[k]);
+    S.plB[S.at(k)] = make_double4(B.pp[k][0], B.pp[k][1], B.pp[k][2],
+                                  __longlong_as_double((long long)BOX_REF));
+    S.cyc[S.at(k)] = (uint8_t)k;
+  }
+  c.np = 6;
+  c.cstart = 0;
+  c.clen = 0;
+  const uint8_t D[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
+                           {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
+  c.nv = 8;
+  for (int v = 0; v < 8; v++) {
+    double4 nv;
+    if (!make_vertex(S, c, B.dim, D[v][0], D[v][1], D[v][2], &nv)) {
+      *err = E_DET_ZERO;
+      return CS_ERROR;
+    }
+    S.vtx[S.at(v)] = nv;
+    S.vdual[S.at(v)] = pack_dual(D[v][0], D[v][1], D[v][2]);
+  }
+  if (!update_safety_radius(S, c)) {
+    *err = E_NAN;
+    return CS_ERROR;
+  }
+  // ConvexCell::build (convex_cell.rs:316-356)
+  const uint32_t cnt = A.kcnt[w];
+  const size_t NW = A.nwork;
+  uint64_t first = A.kpk[w];
+  if (cnt == 0 || (uint64_t)pk_idx(first) != c.idx) {
+    *err = E_SELF_NOT_FIRST;
+    return CS_ERROR;
+  }
+  bool stopped = false;
+  uint32_t j = 1;
+  for (; j < cnt; j++) {
+    uint64_t pk = A.kpk[(size_t)j * NW + w];
+    double4 g = A.spos[pk_slot(pk)];
+    d3 ngb = xyz(g);
+    d3 s;
+    if (image_shift(pk_img(pk), B.width, &s)) ngb = add3(ngb, s);
+    d3 dx = sub3(c.q, ngb);
+    double dist = sqrt(dot3(dx, dx));
+    if (!(isfinite(dist) && dist > 0.0)) {
+      *err = E_DEGENERATE;
+      return CS_ERROR;
+    }
+    if (c.sr < dist) {
+      stopped = true;
+      break;
+    }
+    d3 n = div3(dx, dist);
+    d3 p = scl3(0.5, add3(c.q, ngb));
+    // HalfSpace::new (half_space.rs:24-34)
+    double errb = 1e-13 * (1. + dot3(abs3(n), abs3(p)));
+    double d = dot3(n, p);
+    uint32_t r = clip_by_plane<MAXP, MAXV>(S, c, A, n, p, d, errb, pk, ngb);
+    if (r == 255) return CS_CAPACITY;
+    if (r) {
+      *err = r;
+      return CS_ERROR;
+    }
+  }
+  if (!stopped && (uint64_t)cnt < A.total_stream) return CS_NEED_MORE;
+  *npop = stopped ? j + 1 : cnt;
+
+  // decomposition and integrals (convex_cell.rs:131-187,
+  // voronoi_cell.rs:77-99, integrals.rs:91-105, voronoi_face.rs:30-43)
+  for (int k = 0; k < c.np; k++) {
+    S.cyc[S.at(k)] = should_construct(S, c, A, k) ? 1 : 0;
+    S.facc[S.at(k)] = make_double4(0., 0., 0., 0.);
+  }
+  double vol = 0.;
+  d3 cen = mk3(0, 0, 0);
+  const d3 q = c.q;
+  for (int vi = 0; vi < c.nv; vi++) {
+    double4 v4 = S.vtx[S.at(vi)];
+    uint32_t du = S.vdual[S.at(vi)];
+    int pl[3] = {dual_at(du, 0), dual_at(du, 1), dual_at(du, 2)};
+    double4 pa[3], pb[3];
+    for (int k = 0; k < 3; k++) {
+      pa[k] = S.plA[S.at(pl[k])];
+      pb[k] = S.plB[S.at(pl[k])];
+    }
+    d3 proj[6];
+    for (int k = 0; k < 3; k++) {
+      int k1 = k == 2 ? 0 : k + 1;
+      proj[2 * k] = plane_project_onto(xyz(pa[k]), xyz(pb[k]), q);
+      if (!project_onto_intersection(xyz(pa[k1]), pa[k1].w, xyz(pa[k]),
+                                     pa[k].w, q, &proj[2 * k + 1])) {
+        *err = E_DET_ZERO;
+        return CS_ERROR;
+      }
+    }
+    const d3 vl = xyz(v4);
+    for (int k = 0; k < 3; k++) {
+      const int pi = pl[k];
+      const bool face = S.cyc[S.at(pi)] & 1;
+      double4 fa;
+      if (face) fa = S.facc[S.at(pi)];
+      for (int h = 0; h < 2; h++) {
+        const int t = 2 * k + h;
+        const d3 v0 = proj[t], v1 = proj[t == 0 ? 5 : t - 1];
+        double tv = signed_volume_tet(v0, v1, vl, q);
+        vol += tv;
+        cen = add3(cen, scl3(tv, add3(add3(add3(v0, v1), vl), q)));
+        if (face) {
+          double ar = signed_area_tri(v0, v1, vl, q);
+          d3 sc = scl3(ar, add3(add3(v0, v1), vl));
+          fa.x += ar;
+          fa.y += sc.x;
+          fa.z += sc.y;
+          fa.w += sc.z;
+        }
+      }
+      if (face) {
+        S.facc[S.at(pi)] = fa;
+        S.cyc[S.at(pi)] = 3;
+      }
+    }
+  }
+  const size_t gi = c.idx;
+  const double norm = vol > 0. ? 0.25 / vol : 0.;
+  A.cell_vol[gi] = vol;
+  A.cell_cen[3 * gi] = cen.x * norm;
+  A.cell_cen[3 * gi + 1] = cen.y * norm;
+  A.cell_cen[3 * gi + 2] = cen.z * norm;
+  A.cell_sr[gi] = c.sr;
+  uint32_t count = 0;
+  for (int k = 0; k < c.np; k++) count += S.cyc[S.at(k)] == 3;
+  *nf = count;
+  return CS_DONE;
+}
+
+template <int MAXP, int MAXV>
+__global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
+  const size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  Lane S;
+  S.plA = A.plA;
+  S.plB = A.plB;
+  S.vtx = A.vtx;
+  S.facc = A.facc;
+  S.vdual = A.vdual;
+  S.cyc = A.cyc;
+  S.lane = gl;
+  S.lanes = A.lanes;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // wave-uniform trip count: every lane of a wavefront runs the same
+  // iterations, so the face-offset scan below sees a converged wavefront
+  for (size_t w0 = gl - lane; w0 < A.nwork; w0 += stride) {
+    const size_t w = w0 + lane;
+    uint32_t nf = 0;
+    bool emit = false;
+    CellRegs c;
+    if (w < A.nwork) {
+      const uint32_t slot = A.work[w];
+      const double4 qq = A.spos[slot];
+      c.q = xyz(qq);
+      c.idx = (uint64_t)__double_as_longlong(qq.w);
+      uint32_t err = 0, npop = 0;
+      uint8_t st = build_cell<MAXP, MAXV>(S, c, A, (uint32_t)w, &nf, &npop, &err);
+      A.wstate[w] = st;
+      if (st == CS_ERROR) {
+        atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
+      }
+      if (st == CS_DONE) {
+        emit = true;
+        A.cell_npop[c.idx] = npop;
+      } else {
+        nf = 0;
+      }
+    }
+    uint32_t total;
+    uint32_t excl = wave_excl_scan(nf, &total);
+    unsigned long long base = 0;
+    if (lane == 0 && total) base = atomicAdd(A.st_counter, (unsigned long long)total);
+    base = __shfl(base, 0, 64);
+    if (emit) {
+      const uint64_t fb = base + excl;
+      A.cell_nf[c.idx] = nf;
+      A.cell_fbase[c.idx] = fb;
+      if (fb + nf > A.st_cap) {
+        atomicOr(A.overflow, 1ull);
+      } else {
+        uint64_t o = fb;
+        for (int k = 0; k < c.np; k++) {
+          if (S.cyc[S.at(k)] != 3) continue;
+          double4 a = S.plA[S.at(k)];
+          uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(k)].w);
+          double4 fa = S.facc[S.at(k)];
+          double fnrm = fa.x > 0. ? 1. / (3. * fa.x) : 0.;
+          A.st_cell[o] = (uint32_t)c.idx;
+          A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+          A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
+          A.st_area[o] = fa.x;
+          A.st_cen[3 * o] = fa.y * fnrm;
+          A.st_cen[3 * o + 1] = fa.z * fnrm;
+          A.st_cen[3 * o + 2] = fa.w * fnrm;
+          A.st_norm[3 * o] = a.x;
+          A.st_norm[3 * o + 1] = a.y;
+          A.st_norm[3 * o + 2] = a.z;
+          o++;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace mv

@@ -1,0 +1,122 @@
+// mv_finalize.hpp -- face compaction and cell -> face links.
+//
+// flatten! + Voronoi::finalize (src/voronoi.rs:225-226, 241-263): faces are
+// ordered by left cell, then by plane index within the cell (the staging
+// area holds them in completion order, a scan of per-cell face counts gives
+// the final offsets); each face is linked to its left cell and, when it has
+// a right cell and no periodic shift, to the right cell; every cell's links
+// are ascending in face index.
+#pragma once
+#include "mv_common.hpp"
+
+namespace mv {
+
+struct FinalArgs {
+  // staging
+  const uint32_t *st_cell;
+  const int64_t *st_right;
+  const uint8_t *st_img;
+  const double *st_area, *st_cen, *st_norm;
+  uint64_t n_staged;
+  const uint64_t *cell_fbase;
+  const uint64_t *face_off;  // exclusive scan of cell_nf
+  double width[3];
+  // final faces
+  uint64_t *face_left;
+  int64_t *face_right;
+  double *face_area, *face_centroid, *face_normal, *face_shift;
+  uint8_t *face_has_shift;
+  uint32_t *link_in;  // incoming links per cell
+};
+
+__global__ void k_place_faces(FinalArgs A) {
+  size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= A.n_staged) return;
+  uint32_t c = A.st_cell[o];
+  size_t f = A.face_off[c] + (o - A.cell_fbase[c]);
+  int64_t right = A.st_right[o];
+  uint8_t img = A.st_img[o];
+  A.face_left[f] = c;
+  A.face_right[f] = right;
+  A.face_area[f] = A.st_area[o];
+  for (int k = 0; k < 3; k++) {
+    A.face_centroid[3 * f + k] = A.st_cen[3 * o + k];
+    A.face_normal[3 * f + k] = A.st_norm[3 * o + k];
+  }
+  d3 s;
+  bool has = right >= 0 && image_shift(img, A.width, &s);
+  if (!has) s = mk3(0, 0, 0);
+  A.face_has_shift[f] = has ? 1 : 0;
+  A.face_shift[3 * f] = s.x;
+  A.face_shift[3 * f + 1] = s.y;
+  A.face_shift[3 * f + 2] = s.z;
+  if (right >= 0 && !has) atomicAdd(&A.link_in[right], 1u);
+}
+
+__global__ void k_link_count(const uint32_t *__restrict__ cell_nf,
+                             const uint32_t *__restrict__ link_in, size_t n,
+                             uint64_t *__restrict__ link_cnt) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) link_cnt[c] = (uint64_t)cell_nf[c] + link_in[c];
+}
+
+__global__ void k_fill_links(const uint64_t *__restrict__ face_left,
+                             const int64_t *__restrict__ face_right,
+                             const uint8_t *__restrict__ face_has_shift,
+                             size_t nf, const uint64_t *__restrict__ link_off,
+                             uint32_t *__restrict__ fill,
+                             uint64_t *__restrict__ links) {
+  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  uint64_t l = face_left[f];
+  uint32_t p = atomicAdd(&fill[l], 1u);
+  links[link_off[l] + p] = f;
+  int64_t r = face_right[f];
+  if (r >= 0 && !face_has_shift[f]) {
+    uint32_t q = atomicAdd(&fill[r], 1u);
+    links[link_off[r] + q] = f;
+  }
+}
+
+// per-cell ascending order of the links
+__global__ void k_sort_links(const uint64_t *__restrict__ link_off,
+                             const uint64_t *__restrict__ link_cnt, size_t n,
+                             uint64_t *__restrict__ links) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  uint64_t b = link_off[c], e = b + link_cnt[c];
+  for (uint64_t i = b + 1; i < e; i++) {
+    uint64_t v = links[i];
+    uint64_t j = i;
+    while (j > b && links[j - 1] > v) {
+      links[j] = links[j - 1];
+      j--;
+    }
+    links[j] = v;
+  }
+}
+
+// VoronoiCell records: loc and idx of active cells (Default for inactive)
+__global__ void k_cell_records(const double4 *__restrict__ gpos, size_t n,
+                               const uint8_t *__restrict__ mask,
+                               double *__restrict__ cell_loc,
+                               uint64_t *__restrict__ cell_idx) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool act = mask == nullptr || mask[i];
+  double4 p = gpos[i];
+  cell_loc[3 * i] = act ? p.x : 0.;
+  cell_loc[3 * i + 1] = act ? p.y : 0.;
+  cell_loc[3 * i + 2] = act ? p.z : 0.;
+  cell_idx[i] = act ? (uint64_t)i : 0;
+}
+
+__global__ void k_exact_batch(const int64_t *__restrict__ t, size_t n,
+                              int8_t *__restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t *p = t + 15 * i;
+  out[i] = (int8_t)in_sphere_exact(p, p + 3, p + 6, p + 9, p + 12);
+}
+
+}  // namespace mv

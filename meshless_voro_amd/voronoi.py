@@ -1,0 +1,268 @@
+"""Host-side mirror of the meshless_voronoi crate's construction API, over
+the C ABI of libmvoro (include/mvoro.h).
+
+Names, argument meaning and error behaviour follow the reference:
+  Voronoi.build / build_partial ......... src/voronoi.rs:151-190
+  Voronoi accessors ..................... src/voronoi.rs:317-359
+  VoronoiCell ........................... src/voronoi/voronoi_cell.rs:107-165
+  VoronoiFace ........................... src/voronoi/voronoi_face.rs:241-300
+  Dimensionality ........................ src/voronoi.rs:30-50
+Where the reference panics, this module raises VoronoiError with the
+library's status code.  Results are also exposed as whole numpy arrays
+(the SoA layout the library returns), which is the efficient way to read a
+large tessellation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from enum import IntEnum
+from typing import Iterator, Optional, Sequence
+
+import numpy as np
+
+from . import _native
+
+__all__ = ["Dimensionality", "Voronoi", "VoronoiCell", "VoronoiFace", "VoronoiError",
+           "BuildOptions", "MODE_PLAIN", "MODE_LEARNED"]
+
+MODE_PLAIN = 0
+MODE_LEARNED = 1
+
+
+class Dimensionality(IntEnum):
+    """src/voronoi.rs:30-50"""
+    OneD = 1
+    TwoD = 2
+    ThreeD = 3
+
+    def vector_is_valid(self, v) -> bool:
+        if self == Dimensionality.OneD:
+            return v[1] == 0.0 and v[2] == 0.0
+        if self == Dimensionality.TwoD:
+            return v[2] == 0.0
+        return True
+
+
+class VoronoiError(RuntimeError):
+    """A build failed where the reference would panic."""
+
+    def __init__(self, status: int):
+        msg = _native.lib().mv_status_string(int(status)).decode()
+        super().__init__(f"mvoro status {status}: {msg}")
+        self.status = int(status)
+
+
+class BuildOptions:
+    """mv_options: construction mode and neighbour-list parameters."""
+
+    def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 4096,
+                 lanes: int = 0, occupancy16: int = 0):
+        self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
+        self.occupancy16 = occupancy16
+
+    def to_c(self) -> _native.mv_options:
+        o = _native.mv_options()
+        _native.lib().mv_default_options(C.byref(o))
+        o.mode, o.k0, o.k_max, o.lanes = self.mode, self.k0, self.k_max, self.lanes
+        o.reserved[0] = self.occupancy16
+        return o
+
+
+def _vec3(v) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+    return a
+
+
+def _as_array(ptr, n, ctype, dtype, shape=None):
+    if n == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        arr = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(n,))
+        out = np.array(arr, dtype=dtype, copy=True)
+    if shape is not None:
+        out = out.reshape(shape)
+    return out
+
+
+class Voronoi:
+    """The main Voronoi struct (src/voronoi.rs:117-126)."""
+
+    def __init__(self):
+        raise TypeError("use Voronoi.build or Voronoi.build_partial")
+
+    @classmethod
+    def _from_handle(cls, h: C.c_void_p) -> "Voronoi":
+        self = object.__new__(cls)
+        L = _native.lib()
+        try:
+            n = L.mv_n_cells(h)
+            F = L.mv_n_faces(h)
+            NL = L.mv_n_links(h)
+            a = (C.c_double * 3)()
+            w = (C.c_double * 3)()
+            L.mv_anchor(h, a)
+            L.mv_width(h, w)
+            self._anchor = np.array(a[:])
+            self._width = np.array(w[:])
+            self._dimensionality = Dimensionality(L.mv_dimensionality(h))
+            self._periodic = bool(L.mv_periodic(h))
+            d, u64, i64, u8, u32 = C.c_double, C.c_uint64, C.c_int64, C.c_uint8, C.c_uint32
+            self.cell_loc = _as_array(L.mv_cell_loc(h), 3 * n, d, np.float64, (n, 3))
+            self.cell_centroid = _as_array(L.mv_cell_centroid(h), 3 * n, d, np.float64, (n, 3))
+            self.cell_volume = _as_array(L.mv_cell_volume(h), n, d, np.float64)
+            self.cell_safety_radius = _as_array(L.mv_cell_safety_radius(h), n, d, np.float64)
+            self.cell_face_offset = _as_array(L.mv_cell_face_offset(h), n, u64, np.uint64)
+            self.cell_face_count = _as_array(L.mv_cell_face_count(h), n, u64, np.uint64)
+            self.cell_idx = _as_array(L.mv_cell_idx(h), n, u64, np.uint64)
+            self.cell_n_popped = _as_array(L.mv_cell_n_popped(h), n, u32, np.uint32)
+            self.face_left = _as_array(L.mv_face_left(h), F, u64, np.uint64)
+            self.face_right = _as_array(L.mv_face_right(h), F, i64, np.int64)
+            self.face_area = _as_array(L.mv_face_area(h), F, d, np.float64)
+            self.face_centroid = _as_array(L.mv_face_centroid(h), 3 * F, d, np.float64, (F, 3))
+            self.face_normal = _as_array(L.mv_face_normal(h), 3 * F, d, np.float64, (F, 3))
+            self.face_has_shift = _as_array(L.mv_face_has_shift(h), F, u8, np.uint8)
+            self.face_shift = _as_array(L.mv_face_shift(h), 3 * F, d, np.float64, (F, 3))
+            self.links = _as_array(L.mv_cell_face_connections(h), NL, u64, np.uint64)
+            st = _native.mv_stats()
+            L.mv_get_stats(h, C.byref(st))
+            self.stats = {k: getattr(st, k) for k, _ in _native.mv_stats._fields_}
+        finally:
+            L.mv_free(h)
+        return self
+
+    @classmethod
+    def _build(cls, generators, mask, anchor, width, dimensionality, periodic,
+               options: Optional[BuildOptions]):
+        xyz = np.ascontiguousarray(np.asarray(generators, dtype=np.float64).reshape(-1, 3))
+        n = xyz.shape[0]
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask).astype(np.uint8).reshape(-1))
+            if m.shape[0] < n:
+                raise VoronoiError(1)  # index out of bounds in the reference
+        a, w = _vec3(anchor), _vec3(width)
+        L = _native.lib()
+        opt = (options or BuildOptions()).to_c()
+        h = C.c_void_p()
+        st = L.mv_build(xyz.ctypes.data if n else None, n,
+                        m.ctypes.data if m is not None else None,
+                        a.ctypes.data, w.ctypes.data, int(Dimensionality(dimensionality)),
+                        int(bool(periodic)), C.byref(opt), C.byref(h))
+        if st != 0:
+            raise VoronoiError(st)
+        return cls._from_handle(h)
+
+    @classmethod
+    def build(cls, generators, anchor, width, dimensionality=Dimensionality.ThreeD,
+              periodic: bool = False, options: Optional[BuildOptions] = None) -> "Voronoi":
+        """Voronoi::build (src/voronoi.rs:151-159)."""
+        return cls._build(generators, None, anchor, width, dimensionality, periodic, options)
+
+    @classmethod
+    def build_partial(cls, generators, mask: Sequence[bool], anchor, width,
+                      dimensionality=Dimensionality.ThreeD, periodic: bool = False,
+                      options: Optional[BuildOptions] = None) -> "Voronoi":
+        """Voronoi::build_partial (src/voronoi.rs:181-190)."""
+        return cls._build(generators, mask, anchor, width, dimensionality, periodic, options)
+
+    # --- accessors (src/voronoi.rs:317-359) ---
+    def anchor(self) -> np.ndarray:
+        return self._anchor.copy()
+
+    def width(self) -> np.ndarray:
+        return self._width.copy()
+
+    def cells(self) -> "list[VoronoiCell]":
+        return [VoronoiCell(self, i) for i in range(self.cell_volume.shape[0])]
+
+    def faces(self) -> "list[VoronoiFace]":
+        return [VoronoiFace(self, i) for i in range(self.face_area.shape[0])]
+
+    def cell_face_connections(self) -> np.ndarray:
+        return self.links
+
+    def dimensionality(self) -> int:
+        return int(self._dimensionality)
+
+    def periodic(self) -> bool:
+        return self._periodic
+
+
+class VoronoiCell:
+    """View of one cell (src/voronoi/voronoi_cell.rs:107-165)."""
+
+    __slots__ = ("_v", "_i")
+
+    def __init__(self, v: Voronoi, i: int):
+        self._v, self._i = v, i
+
+    def loc(self):
+        return self._v.cell_loc[self._i]
+
+    def centroid(self):
+        return self._v.cell_centroid[self._i]
+
+    def volume(self) -> float:
+        return float(self._v.cell_volume[self._i])
+
+    def safety_radius(self) -> float:
+        return float(self._v.cell_safety_radius[self._i])
+
+    def face_connections_offset(self) -> int:
+        return int(self._v.cell_face_offset[self._i])
+
+    def face_count(self) -> int:
+        return int(self._v.cell_face_count[self._i])
+
+    def face_indices(self, voronoi: Voronoi) -> np.ndarray:
+        o = self.face_connections_offset()
+        return voronoi.links[o:o + self.face_count()]
+
+    def faces(self, voronoi: Voronoi) -> Iterator["VoronoiFace"]:
+        return (VoronoiFace(voronoi, int(i)) for i in self.face_indices(voronoi))
+
+    def neighbour_ids(self, voronoi: Voronoi) -> Iterator[int]:
+        """Neighbours across internal non-periodic faces
+        (voronoi_cell.rs:141-153)."""
+        me = int(voronoi.cell_idx[self._i])
+        for i in self.face_indices(voronoi):
+            if voronoi.face_has_shift[i] or voronoi.face_right[i] < 0:
+                continue
+            left = int(voronoi.face_left[i])
+            yield int(voronoi.face_right[i]) if left == me else left
+
+
+class VoronoiFace:
+    """View of one face (src/voronoi/voronoi_face.rs:241-300)."""
+
+    __slots__ = ("_v", "_i")
+
+    def __init__(self, v: Voronoi, i: int):
+        self._v, self._i = v, i
+
+    def left(self) -> int:
+        return int(self._v.face_left[self._i])
+
+    def right(self) -> Optional[int]:
+        r = int(self._v.face_right[self._i])
+        return None if r < 0 else r
+
+    def area(self) -> float:
+        return float(self._v.face_area[self._i])
+
+    def centroid(self):
+        return self._v.face_centroid[self._i]
+
+    def normal(self):
+        """Normal of the left cell's half-space: it points from the face
+        toward the left generator (voronoi_face.rs:26, convex_cell.rs:340)."""
+        return self._v.face_normal[self._i]
+
+    def shift(self):
+        return self._v.face_shift[self._i] if self._v.face_has_shift[self._i] else None
+
+    def is_periodic(self) -> bool:
+        return bool(self._v.face_has_shift[self._i])
+
+    def is_boundary(self) -> bool:
+        return self._v.face_right[self._i] < 0
