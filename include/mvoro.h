@@ -57,7 +57,8 @@ typedef struct mv_options {
   uint32_t mode;     /* MV_MODE_*; default MV_MODE_PLAIN */
   uint32_t k0;       /* neighbours in the first pass, self included
                         (plain mode); default 27 */
-  uint32_t k_max;    /* largest neighbour list tried; default 4096 */
+  uint32_t k_max;    /* largest neighbour list tried; default and
+                        maximum 32768 */
   uint32_t lanes;    /* cells in flight (thread-per-cell lanes); 0 = auto */
   uint32_t reserved[8];
 } mv_options;

@@ -316,6 +316,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   }
   // ConvexCell::build (convex_cell.rs:316-356)
   const uint32_t cnt = A.kcnt[w];
+  if (cnt == 0xFFFFFFFFu) return CS_CAPACITY;  // list did not fit the gather
   const size_t NW = A.nwork;
   uint64_t first = A.kpk[w];
   if (cnt == 0 || (uint64_t)pk_idx(first) != c.idx) {
@@ -365,33 +366,39 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   d3 cen = mk3(0, 0, 0);
   const d3 q = c.q;
   for (int vi = 0; vi < c.nv; vi++) {
-    double4 v4 = S.vtx[S.at(vi)];
-    uint32_t du = S.vdual[S.at(vi)];
-    int pl[3] = {dual_at(du, 0), dual_at(du, 1), dual_at(du, 2)};
-    double4 pa[3], pb[3];
-    for (int k = 0; k < 3; k++) {
-      pa[k] = S.plA[S.at(pl[k])];
-      pb[k] = S.plB[S.at(pl[k])];
-    }
-    d3 proj[6];
-    for (int k = 0; k < 3; k++) {
-      int k1 = k == 2 ? 0 : k + 1;
-      proj[2 * k] = plane_project_onto(xyz(pa[k]), xyz(pb[k]), q);
-      if (!project_onto_intersection(xyz(pa[k1]), pa[k1].w, xyz(pa[k]),
-                                     pa[k].w, q, &proj[2 * k + 1])) {
+    const uint32_t du = S.vdual[S.at(vi)];
+    const d3 vl = xyz(S.vtx[S.at(vi)]);
+    // projections P_t of DecompositionWithoutFaces::load_vertex, streamed:
+    // P_{2k} = q projected on plane dual[k], P_{2k+1} = q projected on the
+    // line dual[k+1] ^ dual[k]; tet t = (P_t, P_{t-1 mod 6}, vertex).
+    d3 prev;  // P_5 first
+    {
+      double4 a0 = S.plA[S.at(dual_at(du, 0))];
+      double4 a2 = S.plA[S.at(dual_at(du, 2))];
+      if (!project_onto_intersection(xyz(a0), a0.w, xyz(a2), a2.w, q, &prev)) {
         *err = E_DET_ZERO;
         return CS_ERROR;
       }
     }
-    const d3 vl = xyz(v4);
     for (int k = 0; k < 3; k++) {
-      const int pi = pl[k];
+      const int pi = dual_at(du, k);
+      const int pn = dual_at(du, k == 2 ? 0 : k + 1);
+      const double4 ak = S.plA[S.at(pi)];
+      const double4 bk = S.plB[S.at(pi)];
+      const d3 pface = plane_project_onto(xyz(ak), xyz(bk), q);
+      const double4 an = S.plA[S.at(pn)];
+      d3 pedge;
+      if (!project_onto_intersection(xyz(an), an.w, xyz(ak), ak.w, q, &pedge)) {
+        *err = E_DET_ZERO;
+        return CS_ERROR;
+      }
       const bool face = S.cyc[S.at(pi)] & 1;
-      double4 fa;
+      double4 fa = make_double4(0., 0., 0., 0.);
       if (face) fa = S.facc[S.at(pi)];
+      // tet 2k: (P_2k, P_2k-1, v); tet 2k+1: (P_2k+1, P_2k, v)
       for (int h = 0; h < 2; h++) {
-        const int t = 2 * k + h;
-        const d3 v0 = proj[t], v1 = proj[t == 0 ? 5 : t - 1];
+        const d3 v0 = h == 0 ? pface : pedge;
+        const d3 v1 = h == 0 ? prev : pface;
         double tv = signed_volume_tet(v0, v1, vl, q);
         vol += tv;
         cen = add3(cen, scl3(tv, add3(add3(add3(v0, v1), vl), q)));
@@ -408,6 +415,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
         S.facc[S.at(pi)] = fa;
         S.cyc[S.at(pi)] = 3;
       }
+      prev = pedge;
     }
   }
   const size_t gi = c.idx;

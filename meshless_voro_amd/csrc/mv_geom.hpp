@@ -101,187 +101,169 @@ __device__ __forceinline__ double signed_area_tri(d3 v0, d3 v1, d3 v2, d3 t) {
 }
 
 // ---------------------------------------------------------------------
-// Exact in-sphere sign (src/geometry.rs:271-314) on snapped 52-bit
-// integer coordinates.  |det| < 2^267, so 320-bit two's complement
-// integers hold every intermediate exactly.  A floating-point evaluation
-// of the same determinant with an error bound decides first; only when it
-// cannot, the integer expansion runs.  Both give the sign of the same
-// integer determinant.
+// Exact in-sphere sign (src/geometry.rs:271-314) on snapped 52-bit integer
+// coordinates.  The determinant is that of the 4x4 matrix M whose columns
+// are b-a, c-a, d-a, v-a, each extended by its squared norm.  Any exact
+// evaluation of det(M) has the reference's sign; here M is expanded by
+// Laplace along rows {x, y} against rows {z, |.|^2}:
+//   det M = sum_{j<k} (-1)^(1+j+k) m01(j,k) m23(~(j,k))
+// with m01 = 2x2 minors of the x/y rows (< 2^106, int128) and m23 = 2x2
+// minors of the z/norm rows (< 2^160, 192-bit); |det M| < 2^268 fits the
+// 320-bit accumulator.  A floating-point evaluation of the same expansion
+// with an error bound decides first; only ties and near-ties reach the
+// integer path.
 // ---------------------------------------------------------------------
-struct i320 {
+typedef unsigned __int128 u128;
+typedef __int128 s128;
+
+struct u192 {
+  uint64_t w[3];
+};
+struct u320 {
   uint64_t w[5];
 };
 
-__device__ __forceinline__ i320 i320_of(int64_t v) {
-  i320 r;
-  uint64_t e = v < 0 ? ~0ull : 0ull;
-  r.w[0] = (uint64_t)v;
-  r.w[1] = e;
-  r.w[2] = e;
-  r.w[3] = e;
-  r.w[4] = e;
+// |z| * w for w >= 0 (w < 2^107): 64 x 128 -> 192 bits
+__device__ __forceinline__ u192 mul_u64_u128(uint64_t a, u128 b) {
+  u128 lo = (u128)a * (uint64_t)b;
+  u128 hi = (u128)a * (uint64_t)(b >> 64);
+  u192 r;
+  r.w[0] = (uint64_t)lo;
+  u128 mid = (lo >> 64) + (u128)(uint64_t)hi;
+  r.w[1] = (uint64_t)mid;
+  r.w[2] = (uint64_t)(hi >> 64) + (uint64_t)(mid >> 64);
   return r;
 }
-__device__ __forceinline__ i320 i320_add(const i320 &a, const i320 &b) {
-  i320 r;
-  unsigned __int128 c = 0;
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    c += (unsigned __int128)a.w[i] + b.w[i];
+__device__ __forceinline__ u192 neg192(u192 a) {
+  u192 r;
+  u128 c = 1;
+  for (int i = 0; i < 3; i++) {
+    c += (u128)(~a.w[i]);
     r.w[i] = (uint64_t)c;
     c >>= 64;
   }
   return r;
 }
-__device__ __forceinline__ i320 i320_neg(const i320 &a) {
-  i320 r;
-  unsigned __int128 c = 1;
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    c += (unsigned __int128)(~a.w[i]);
+__device__ __forceinline__ u192 add192(u192 a, u192 b) {
+  u192 r;
+  u128 c = 0;
+  for (int i = 0; i < 3; i++) {
+    c += (u128)a.w[i] + b.w[i];
     r.w[i] = (uint64_t)c;
     c >>= 64;
   }
   return r;
 }
-__device__ __forceinline__ i320 i320_sub(const i320 &a, const i320 &b) {
-  return i320_add(a, i320_neg(b));
+// z * w as 192-bit two's complement
+__device__ __forceinline__ u192 zmul(int64_t z, u128 w) {
+  u192 m = mul_u64_u128((uint64_t)(z < 0 ? -z : z), w);
+  return z < 0 ? neg192(m) : m;
 }
-// product modulo 2^320; exact because every product here fits
-__device__ __forceinline__ i320 i320_mul(const i320 &a, const i320 &b) {
-  i320 r;
-#pragma unroll
-  for (int i = 0; i < 5; i++) r.w[i] = 0;
-#pragma unroll
+__device__ __forceinline__ void add320(u320 &acc, const u320 &b) {
+  u128 c = 0;
   for (int i = 0; i < 5; i++) {
-    unsigned __int128 carry = 0;
-#pragma unroll
-    for (int j = 0; i + j < 5; j++) {
-      unsigned __int128 t =
-          (unsigned __int128)a.w[i] * b.w[j] + r.w[i + j] + carry;
-      r.w[i + j] = (uint64_t)t;
+    c += (u128)acc.w[i] + b.w[i];
+    acc.w[i] = (uint64_t)c;
+    c >>= 64;
+  }
+}
+// acc += s * m01 * m23 (s = +1 or -1), all in two's complement
+__device__ __forceinline__ void acc_term(u320 &acc, int s, s128 m01,
+                                         u192 m23) {
+  bool neg = m01 < 0;
+  u128 a = (u128)(neg ? -m01 : m01);
+  if ((int64_t)m23.w[2] < 0) {
+    m23 = neg192(m23);
+    neg = !neg;
+  }
+  if (s < 0) neg = !neg;
+  // |m01| * |m23|: 2 x 3 limbs
+  uint64_t al[2] = {(uint64_t)a, (uint64_t)(a >> 64)};
+  u320 p;
+  for (int i = 0; i < 5; i++) p.w[i] = 0;
+  for (int i = 0; i < 2; i++) {
+    u128 carry = 0;
+    for (int j = 0; j < 3; j++) {
+      u128 t = (u128)al[i] * m23.w[j] + p.w[i + j] + carry;
+      p.w[i + j] = (uint64_t)t;
       carry = t >> 64;
     }
+    p.w[i + 3] += (uint64_t)carry;
   }
-  return r;
-}
-__device__ __forceinline__ int i320_sign(const i320 &a) {
-  if ((int64_t)a.w[4] < 0) return -1;
-  return (a.w[0] | a.w[1] | a.w[2] | a.w[3] | a.w[4]) ? 1 : 0;
-}
-__device__ __forceinline__ i320 i320_det2(const i320 &a, const i320 &b,
-                                          const i320 &c, const i320 &d) {
-  return i320_sub(i320_mul(a, d), i320_mul(b, c));
-}
-// rows (a0 a1 a2), (b0 b1 b2), (c0 c1 c2) expanded along the first row
-__device__ __noinline__ i320 i320_det3(const i320 &a0, const i320 &a1,
-                                       const i320 &a2, const i320 &b0,
-                                       const i320 &b1, const i320 &b2,
-                                       const i320 &c0, const i320 &c1,
-                                       const i320 &c2) {
-  i320 d = i320_mul(a0, i320_det2(b1, b2, c1, c2));
-  d = i320_sub(d, i320_mul(a1, i320_det2(b0, b2, c0, c2)));
-  d = i320_add(d, i320_mul(a2, i320_det2(b0, b1, c0, c1)));
-  return d;
+  if (neg) {
+    u128 c = 1;
+    for (int i = 0; i < 5; i++) {
+      c += (u128)(~p.w[i]);
+      p.w[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  add320(acc, p);
 }
 
-// lifted column: (x - a) per component and its squared norm
-__device__ __forceinline__ void i320_lift(const int64_t *x, const int64_t *a,
-                                          i320 out[4]) {
-  out[0] = i320_of(x[0] - a[0]);
-  out[1] = i320_of(x[1] - a[1]);
-  out[2] = i320_of(x[2] - a[2]);
-  i320 n = i320_mul(out[0], out[0]);
-  n = i320_add(n, i320_mul(out[1], out[1]));
-  n = i320_add(n, i320_mul(out[2], out[2]));
-  out[3] = n;
+__device__ __noinline__ int in_sphere_exact_int(const int64_t *x,
+                                                const int64_t *y,
+                                                const int64_t *z) {
+  // x, y, z: the 4 columns' coordinate differences
+  u128 w[4];
+  for (int j = 0; j < 4; j++) {
+    w[j] = (u128)((s128)x[j] * x[j]) + (u128)((s128)y[j] * y[j]) +
+           (u128)((s128)z[j] * z[j]);
+  }
+  const int J[6] = {0, 0, 0, 1, 1, 2}, K[6] = {1, 2, 3, 2, 3, 3};
+  const int CJ[6] = {2, 1, 1, 0, 0, 0}, CK[6] = {3, 3, 2, 3, 2, 1};
+  const int SG[6] = {1, -1, 1, 1, -1, 1};
+  u320 acc;
+  for (int i = 0; i < 5; i++) acc.w[i] = 0;
+  for (int t = 0; t < 6; t++) {
+    int j = J[t], k = K[t], cj = CJ[t], ck = CK[t];
+    s128 m01 = (s128)x[j] * y[k] - (s128)x[k] * y[j];
+    u192 m23 = add192(zmul(z[cj], w[ck]), neg192(zmul(z[ck], w[cj])));
+    acc_term(acc, SG[t], m01, m23);
+  }
+  if ((int64_t)acc.w[4] < 0) return -1;
+  return (acc.w[0] | acc.w[1] | acc.w[2] | acc.w[3] | acc.w[4]) ? 1 : 0;
 }
 
-__device__ __noinline__ int in_sphere_exact_int(const int64_t *a,
-                                                const int64_t *b,
-                                                const int64_t *c,
-                                                const int64_t *d,
-                                                const int64_t *v) {
-  i320 B[4], C[4], D[4], V[4];
-  i320_lift(b, a, B);
-  i320_lift(c, a, C);
-  i320_lift(d, a, D);
-  i320_lift(v, a, V);
-  i320 det = i320_of(0);
-  i320 t;
-  t = i320_det3(B[1], C[1], D[1], B[2], C[2], D[2], B[3], C[3], D[3]);
-  det = i320_sub(det, i320_mul(V[0], t));
-  t = i320_det3(B[0], C[0], D[0], B[2], C[2], D[2], B[3], C[3], D[3]);
-  det = i320_add(det, i320_mul(V[1], t));
-  t = i320_det3(B[0], C[0], D[0], B[1], C[1], D[1], B[3], C[3], D[3]);
-  det = i320_sub(det, i320_mul(V[2], t));
-  t = i320_det3(B[0], C[0], D[0], B[1], C[1], D[1], B[2], C[2], D[2]);
-  det = i320_add(det, i320_mul(V[3], t));
-  return i320_sign(det);
-}
-
-// Floating-point evaluation of the same 4x4 determinant, with the
-// permanent (same expansion on absolute values) for the error bound.
-// Inputs differ by less than 2^52 so the differences are exact; the
-// squared norms carry a relative error of at most 3 ulp.  The expansion
-// has 3 multiplications and at most 6 additions along every path, so the
-// computed value is within 16 u of the permanent (u = 2^-53); the filter
-// uses 1e-13, a factor of ~60 above that.
-__device__ __forceinline__ double det3f(double a0, double a1, double a2,
-                                        double b0, double b1, double b2,
-                                        double c0, double c1, double c2) {
-  return (a0 * (b1 * c2 - b2 * c1) - a1 * (b0 * c2 - b2 * c0)) +
-         a2 * (b0 * c1 - b1 * c0);
-}
-__device__ __forceinline__ double per3f(double a0, double a1, double a2,
-                                        double b0, double b1, double b2,
-                                        double c0, double c1, double c2) {
-  return (a0 * (b1 * c2 + b2 * c1) + a1 * (b0 * c2 + b2 * c0)) +
-         a2 * (b0 * c1 + b1 * c0);
-}
-
-__device__ __forceinline__ int in_sphere_exact(const int64_t *a,
-                                               const int64_t *b,
-                                               const int64_t *c,
-                                               const int64_t *d,
-                                               const int64_t *v) {
-  double B[4], C[4], D[4], V[4];
-  double aB[4], aC[4], aD[4], aV[4];
+// a, b, c, d, v: snapped coordinates (3 each).  Returns -1, 0, +1.
+__device__ __noinline__ int in_sphere_exact(const int64_t *a, const int64_t *b,
+                                            const int64_t *c, const int64_t *d,
+                                            const int64_t *v) {
+  int64_t x[4], y[4], z[4];
   const int64_t *pts[4] = {b, c, d, v};
-  double *outs[4] = {B, C, D, V};
-  double *aouts[4] = {aB, aC, aD, aV};
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    double x = (double)(pts[k][0] - a[0]);
-    double y = (double)(pts[k][1] - a[1]);
-    double z = (double)(pts[k][2] - a[2]);
-    outs[k][0] = x;
-    outs[k][1] = y;
-    outs[k][2] = z;
-    outs[k][3] = (x * x + y * y) + z * z;
-    aouts[k][0] = fabs(x);
-    aouts[k][1] = fabs(y);
-    aouts[k][2] = fabs(z);
-    aouts[k][3] = outs[k][3];
+  for (int j = 0; j < 4; j++) {
+    x[j] = pts[j][0] - a[0];
+    y[j] = pts[j][1] - a[1];
+    z[j] = pts[j][2] - a[2];
   }
-  double det =
-      ((-V[0] * det3f(B[1], C[1], D[1], B[2], C[2], D[2], B[3], C[3], D[3]) +
-        V[1] * det3f(B[0], C[0], D[0], B[2], C[2], D[2], B[3], C[3], D[3])) -
-       V[2] * det3f(B[0], C[0], D[0], B[1], C[1], D[1], B[3], C[3], D[3])) +
-      V[3] * det3f(B[0], C[0], D[0], B[1], C[1], D[1], B[2], C[2], D[2]);
-  double per =
-      ((aV[0] * per3f(aB[1], aC[1], aD[1], aB[2], aC[2], aD[2], aB[3], aC[3],
-                      aD[3]) +
-        aV[1] * per3f(aB[0], aC[0], aD[0], aB[2], aC[2], aD[2], aB[3], aC[3],
-                      aD[3])) +
-       aV[2] * per3f(aB[0], aC[0], aD[0], aB[1], aC[1], aD[1], aB[3], aC[3],
-                     aD[3])) +
-      aV[3] * per3f(aB[0], aC[0], aD[0], aB[1], aC[1], aD[1], aB[2], aC[2],
-                    aD[2]);
+  // floating-point filter on the same expansion; differences are exact
+  // in double, squared norms carry <= 3 ulp, and every path of the
+  // expansion adds <= 12 more roundings: |error| <= 16 u * permanent,
+  // checked here against 1e-13 * permanent.
+  double fx[4], fy[4], fz[4], fw[4];
+  for (int j = 0; j < 4; j++) {
+    fx[j] = (double)x[j];
+    fy[j] = (double)y[j];
+    fz[j] = (double)z[j];
+    fw[j] = (fx[j] * fx[j] + fy[j] * fy[j]) + fz[j] * fz[j];
+  }
+  const int J[6] = {0, 0, 0, 1, 1, 2}, K[6] = {1, 2, 3, 2, 3, 3};
+  const int CJ[6] = {2, 1, 1, 0, 0, 0}, CK[6] = {3, 3, 2, 3, 2, 1};
+  const double SG[6] = {1., -1., 1., 1., -1., 1.};
+  double det = 0., per = 0.;
+  for (int t = 0; t < 6; t++) {
+    int j = J[t], k = K[t], cj = CJ[t], ck = CK[t];
+    double m01 = fx[j] * fy[k] - fx[k] * fy[j];
+    double p01 = fabs(fx[j] * fy[k]) + fabs(fx[k] * fy[j]);
+    double m23 = fz[cj] * fw[ck] - fz[ck] * fw[cj];
+    double p23 = fabs(fz[cj] * fw[ck]) + fabs(fz[ck] * fw[cj]);
+    det += SG[t] * (m01 * m23);
+    per += p01 * p23;
+  }
   double bound = 1e-13 * per;
   if (det > bound) return 1;
   if (det < -bound) return -1;
-  return in_sphere_exact_int(a, b, c, d, v);
+  return in_sphere_exact_int(x, y, z);
 }
 
 }  // namespace mv

@@ -190,145 +190,266 @@ __global__ void k_scan_add(T *__restrict__ out, size_t n,
 }
 
 // ---------------------------------------------------------------------
-// k nearest stream entries (fixed length L) per work item.
+// k nearest stream entries (list length L) per work item, one wavefront
+// per item.
 //
 // Order: ascending (key, idx, image), key = wrapping_distance_2 of
-// src/rtree_nn.rs:138-146 (for non-periodic builds the reference's
-// distance_squared, src/rtree_nn.rs:124-134, gives the same bits).
-// Grid search: Chebyshev shells of grid cells around the cell of each
-// shifted query q + s, one shell at a time for every image, until the
-// L-th best key is below the squared distance to every unvisited cell.
-// The list lives in global memory, interleaved by work item:
-// entry j of item w is at [j * nwork + w].
+// src/rtree_nn.rs:138-146: dx = (q + s) - g, (dx0^2 + dx1^2) + dx2^2 (for
+// non-periodic builds the reference's distance_squared, src/rtree_nn.rs:
+// 124-134, gives the same bits).
+//
+// The wavefront gathers every entry with key <= R^2: for each image s the
+// grid rows (a, b, c0..c1) crossing the cube around q + s are contiguous
+// ranges of the sorted positions, read 64 at a time (coalesced); entries
+// inside the ball are appended to an LDS buffer with a ballot.  When the
+// ball holds at least L entries (or the whole stream) the L smallest are
+// exactly the first L of the stream; each lane ranks its entries by
+// counting the smaller ones in the buffer and writes those of rank < L.
+// Otherwise R grows (or shrinks when the buffer overflows) and the gather
+// is repeated.  The list is written interleaved: entry j of item w at
+// [j * nwork + w].
 // ---------------------------------------------------------------------
 __device__ __forceinline__ bool entry_less(double ka, uint64_t pa, double kb,
                                            uint64_t pb) {
   return ka < kb || (ka == kb && pa < pb);
 }
 
-struct ImageView {
-  double qs[3];
+// Gather every stream entry with key <= R^2 into (key, pk) buffers (LDS or
+// global).  Entries beyond `cap` are counted but not stored.  The count is
+// wave-uniform.
+__device__ __forceinline__ uint32_t knn_gather(
+    const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
+    const GridParams &g, const double *width, const double q[3], double R,
+    double *bkey, uint64_t *bpk, uint32_t cap) {
+  const uint32_t lane = __lane_id();
+  const double R2 = R * R;
+  const double rr = R + g.margin;
+  uint32_t cnt = 0;
+  for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++)
+    for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++)
+      for (int k = g.img_lo[2]; k <= g.img_hi[2]; k++) {
+        const int sv[3] = {i, j, k};
+        double qs[3];
+        int lo[3], hi[3];
+        bool skip = false;
+        for (int c = 0; c < 3; c++) {
+          qs[c] = q[c] + (double)sv[c] * width[c];
+          if (g.inv_h[c] == 0.) {
+            lo[c] = 0;
+            hi[c] = 0;
+            continue;
+          }
+          double a = floor((qs[c] - rr - g.lo[c]) * g.inv_h[c]);
+          double b = floor((qs[c] + rr - g.lo[c]) * g.inv_h[c]);
+          if (b < 0. || a > (double)(g.G[c] - 1)) {
+            skip = true;
+            break;
+          }
+          lo[c] = a < 0. ? 0 : (int)a;
+          hi[c] = b > (double)(g.G[c] - 1) ? g.G[c] - 1 : (int)b;
+        }
+        if (skip) continue;
+        const uint32_t img = (uint32_t)((i + 1) * 9 + (j + 1) * 3 + (k + 1));
+        for (int a = lo[0]; a <= hi[0]; a++)
+          for (int b = lo[1]; b <= hi[1]; b++) {
+            size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
+            uint32_t t0 = cell_start[row + lo[2]];
+            uint32_t t1 = cell_start[row + hi[2] + 1];
+            for (uint32_t base = t0; base < t1; base += 64) {
+              uint32_t t = base + lane;
+              bool keep = false;
+              double key = 0.;
+              uint64_t pk = 0;
+              if (t < t1) {
+                double4 gp = spos[t];
+                double d0 = qs[0] - gp.x, d1 = qs[1] - gp.y, d2 = qs[2] - gp.z;
+                key = (d0 * d0 + d1 * d1) + d2 * d2;
+                keep = key <= R2;
+                uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
+                pk = (gidx << 32) | ((uint64_t)t << 5) | img;
+              }
+              uint64_t m = __ballot(keep);
+              uint32_t pos = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+              if (keep && pos < cap) {
+                bkey[pos] = key;
+                bpk[pos] = pk;
+              }
+              cnt += (uint32_t)__popcll(m);
+            }
+          }
+      }
+  return cnt;
+}
+
+// Radius search: the ball must hold at least L entries (or the whole
+// stream) and at most `cap`.  Growth/shrink steps from the density, then
+// bisection between a too-small and a too-large radius.  Returns the count
+// (> cap when no radius fits).
+__device__ __forceinline__ uint32_t knn_fill(
+    const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
+    const GridParams &g, const double *width, const double q[3], double R,
+    uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap) {
+  double r_lo = 0., r_hi = 0.;
+  uint32_t cnt = 0;
+  for (int it = 0; it < 96; it++) {
+    cnt = knn_gather(spos, cell_start, g, width, q, R, bkey, bpk, cap);
+    bool all = (uint64_t)cnt >= g.total_stream;
+    if (cnt <= cap && (cnt >= Lw || all)) return cnt;
+    if (cnt > cap) {
+      r_hi = R;
+      if (r_lo > 0.) {
+        R = 0.5 * (r_lo + r_hi);
+      } else {
+        double f = cbrt(0.6 * (double)cap / (double)cnt);
+        R *= fmax(0.5, fmin(0.9, f));
+      }
+    } else {
+      r_lo = R;
+      if (r_hi > 0.) {
+        R = 0.5 * (r_lo + r_hi);
+      } else {
+        double f = cnt ? cbrt(1.4 * (double)Lw / (double)cnt) : 2.0;
+        R *= fmax(1.2, fmin(4.0, f));
+      }
+    }
+    if (r_hi > 0. && r_lo > 0. && r_hi - r_lo <= 1e-15 * r_hi) break;
+  }
+  return cnt > cap ? cnt : cap + 1;
+}
+
+__device__ __forceinline__ double knn_start_radius(
+    const uint32_t *__restrict__ cell_start, const GridParams &g,
+    const double q[3], uint32_t Lw) {
+  int nd = 0;
+  for (int c = 0; c < 3; c++) nd += g.inv_h[c] != 0.;
   int C[3];
+  for (int c = 0; c < 3; c++) {
+    double f = floor((q[c] - g.lo[c]) * g.inv_h[c]);
+    C[c] = g.inv_h[c] == 0. ? 0 : (int)fmax(0., fmin(f, (double)(g.G[c] - 1)));
+  }
+  uint32_t n27 = 0;
+  int cells = 0;
+  for (int a = max(0, C[0] - 1); a <= min(g.G[0] - 1, C[0] + 1); a++)
+    for (int b = max(0, C[1] - 1); b <= min(g.G[1] - 1, C[1] + 1); b++) {
+      size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
+      int c0 = max(0, C[2] - 1), c1 = min(g.G[2] - 1, C[2] + 1);
+      n27 += cell_start[row + c1 + 1] - cell_start[row + c0];
+      cells += c1 - c0 + 1;
+    }
+  double hv = 1.;
+  for (int c = 0; c < 3; c++)
+    if (g.inv_h[c] != 0.) hv *= g.h[c];
+  double rho = (double)max(n27, 1u) / ((double)cells * hv);
+  double vol = (1.3 * (double)Lw + 8.) / rho;
+  if (nd == 3) return cbrt(vol / 4.18879020478639);
+  if (nd == 2) return sqrt(vol / 3.14159265358979);
+  return 0.5 * vol;
+}
+
+// LDS variant: rank sort (each lane counts the smaller entries)
+template <int CAP>
+struct KnnLds {
+  double key[CAP];
+  uint64_t pk[CAP];
 };
 
-__device__ __forceinline__ void image_center(const GridParams &g,
-                                             const double *q, const double *w,
-                                             int i, int j, int k,
-                                             ImageView *iv) {
-  int s[3] = {i, j, k};
-  for (int c = 0; c < 3; c++) {
-    iv->qs[c] = q[c] + (double)s[c] * w[c];
-    double f = floor((iv->qs[c] - g.lo[c]) * g.inv_h[c]);
-    f = fmax(-1.0e8, fmin(1.0e8, f));
-    iv->C[c] = g.inv_h[c] == 0. ? 0 : (int)f;
-  }
-}
-
-// distance from the shifted query to the unvisited part of the grid after
-// the cells within Chebyshev radius r of its cell were visited; +inf when
-// nothing is left.
-__device__ __forceinline__ double image_bound(const GridParams &g,
-                                              const ImageView &iv, int r) {
-  double lb = INFINITY;
-  for (int c = 0; c < 3; c++) {
-    if (iv.C[c] - r > 0) {
-      double face = g.lo[c] + (double)(iv.C[c] - r) * g.h[c];
-      lb = fmin(lb, iv.qs[c] - face);
-    }
-    if (iv.C[c] + r < g.G[c] - 1) {
-      double face = g.lo[c] + (double)(iv.C[c] + r + 1) * g.h[c];
-      lb = fmin(lb, face - iv.qs[c]);
-    }
-  }
-  if (lb == INFINITY) return lb;
-  return fmax(0.0, lb - g.margin);
-}
-
+template <int CAP, int WPB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
           const uint32_t *__restrict__ cell_start, GridParams g,
           BoxParams box, const uint32_t *__restrict__ work, uint32_t nwork,
-          uint32_t L, double *__restrict__ kkey, uint64_t *__restrict__ kpk,
+          uint32_t L, const uint32_t *__restrict__ Lper,
+          const float *__restrict__ Rper, uint64_t *__restrict__ kpk,
           uint32_t *__restrict__ kcnt) {
-  uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwork) return;
-  const uint32_t slot = work[w];
-  const double4 qq = spos[slot];
-  const double q[3] = {qq.x, qq.y, qq.z};
-  uint32_t cnt = 0;
-  double worst_k = INFINITY;
-  uint64_t worst_p = ~0ull;
-  const size_t NW = nwork;
-  for (int r = 0;; r++) {
-    bool any_left = false;
-    double lb_next = INFINITY;
-    for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++)
-      for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++)
-        for (int k = g.img_lo[2]; k <= g.img_hi[2]; k++) {
-          ImageView iv;
-          image_center(g, q, box.width, i, j, k, &iv);
-          double lb_prev = r == 0 ? 0.0 : image_bound(g, iv, r - 1);
-          if (lb_prev == INFINITY) continue;  // image fully visited
-          double lb_r = image_bound(g, iv, r);
-          if (lb_r < INFINITY) any_left = true;
-          lb_next = fmin(lb_next, lb_r);
-          if (cnt == L && lb_prev * lb_prev > worst_k * (1.0 + 1e-12))
-            continue;  // nothing in shell r can enter the list
-          const uint32_t img = (uint32_t)((i + 1) * 9 + (j + 1) * 3 + (k + 1));
-          int a0 = max(0, iv.C[0] - r), a1 = min(g.G[0] - 1, iv.C[0] + r);
-          int b0 = max(0, iv.C[1] - r), b1 = min(g.G[1] - 1, iv.C[1] + r);
-          int c0 = max(0, iv.C[2] - r), c1 = min(g.G[2] - 1, iv.C[2] + r);
-          for (int a = a0; a <= a1; a++) {
-            int da = abs(a - iv.C[0]);
-            for (int b = b0; b <= b1; b++) {
-              int db = abs(b - iv.C[1]);
-              bool full = max(da, db) == r;
-              int cl[2];
-              int ncl = 0;
-              if (!full) {
-                int ca = iv.C[2] - r, cb = iv.C[2] + r;
-                if (ca >= c0 && ca <= c1) cl[ncl++] = ca;
-                if (cb != ca && cb >= c0 && cb <= c1) cl[ncl++] = cb;
-              }
-              int cstart = full ? c0 : 0, cend = full ? c1 : ncl - 1;
-              for (int ci = cstart; ci <= cend; ci++) {
-                int c = full ? ci : cl[ci];
-                size_t cell = ((size_t)a * g.G[1] + b) * g.G[2] + c;
-                uint32_t t0 = cell_start[cell], t1 = cell_start[cell + 1];
-                for (uint32_t t = t0; t < t1; t++) {
-                  double4 gp = spos[t];
-                  double d0 = iv.qs[0] - gp.x, d1 = iv.qs[1] - gp.y,
-                         d2 = iv.qs[2] - gp.z;
-                  double key = (d0 * d0 + d1 * d1) + d2 * d2;
-                  uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
-                  uint64_t pk = (gidx << 32) | ((uint64_t)t << 5) | img;
-                  if (cnt == L && !entry_less(key, pk, worst_k, worst_p))
-                    continue;
-                  uint32_t pos = cnt < L ? cnt : L - 1;
-                  while (pos > 0) {
-                    double pkey = kkey[(size_t)(pos - 1) * NW + w];
-                    uint64_t ppk = kpk[(size_t)(pos - 1) * NW + w];
-                    if (!entry_less(key, pk, pkey, ppk)) break;
-                    kkey[(size_t)pos * NW + w] = pkey;
-                    kpk[(size_t)pos * NW + w] = ppk;
-                    pos--;
-                  }
-                  kkey[(size_t)pos * NW + w] = key;
-                  kpk[(size_t)pos * NW + w] = pk;
-                  if (cnt < L) cnt++;
-                  if (cnt == L) {
-                    worst_k = kkey[(size_t)(L - 1) * NW + w];
-                    worst_p = kpk[(size_t)(L - 1) * NW + w];
-                  }
-                }
-              }
+  __shared__ KnnLds<CAP> lds[WPB];
+  const uint32_t lane = __lane_id();
+  const uint32_t wib = threadIdx.x >> 6;
+  KnnLds<CAP> &buf = lds[wib];
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib; w < nwork; w += nwaves) {
+    const double4 qq = spos[work[w]];
+    const double q[3] = {qq.x, qq.y, qq.z};
+    const uint32_t Lw = Lper ? Lper[w] : L;
+    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
+                                       : knn_start_radius(cell_start, g, q, Lw);
+    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lw, buf.key,
+                            buf.pk, CAP);
+    __builtin_amdgcn_wave_barrier();
+    if (cnt > (uint32_t)CAP) {
+      if (lane == 0) kcnt[w] = 0xFFFFFFFFu;  // does not fit this variant
+      continue;
+    }
+    const uint32_t outn = min(cnt, Lw);
+    for (uint32_t e = lane; e < cnt; e += 64) {
+      const double ke = buf.key[e];
+      const uint64_t pe = buf.pk[e];
+      uint32_t rank = 0;
+      for (uint32_t t = 0; t < cnt; t++)
+        rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
+      if (rank < outn) kpk[(size_t)rank * nwork + w] = pe;
+    }
+    if (lane == 0) kcnt[w] = outn;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Global-memory variant for long lists: each wavefront owns `cap` entries of
+// scratch, sorts them with a bitonic network (padded to a power of two with
+// +inf keys) and writes the first L.
+__global__ void __launch_bounds__(256)
+    k_knn_big(const double4 *__restrict__ spos,
+              const uint32_t *__restrict__ cell_start, GridParams g,
+              BoxParams box, const uint32_t *__restrict__ work,
+              uint32_t nwork, uint32_t L, const uint32_t *__restrict__ Lper,
+              uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
+              double *__restrict__ gkey, uint64_t *__restrict__ gpk,
+              uint32_t cap) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  double *bkey = gkey + (size_t)wid * cap;
+  uint64_t *bpk = gpk + (size_t)wid * cap;
+  for (uint32_t w = wid; w < nwork; w += nwaves) {
+    const double4 qq = spos[work[w]];
+    const double q[3] = {qq.x, qq.y, qq.z};
+    const uint32_t Lw = Lper ? Lper[w] : L;
+    double R = knn_start_radius(cell_start, g, q, Lw);
+    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lw, bkey, bpk, cap);
+    if (cnt > cap) {
+      if (lane == 0) kcnt[w] = 0xFFFFFFFFu;
+      continue;
+    }
+    uint32_t n2 = 64;
+    while (n2 < cnt) n2 <<= 1;
+    for (uint32_t e = cnt + lane; e < n2; e += 64) {
+      bkey[e] = INFINITY;
+      bpk[e] = ~0ull;
+    }
+    __threadfence();
+    for (uint32_t k = 2; k <= n2; k <<= 1)
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = lane; i < n2; i += 64) {
+          uint32_t l = i ^ j;
+          if (l > i) {
+            double ki = bkey[i], kl = bkey[l];
+            uint64_t pi = bpk[i], pl = bpk[l];
+            bool up = (i & k) == 0;
+            bool sw = up ? entry_less(kl, pl, ki, pi) : entry_less(ki, pi, kl, pl);
+            if (sw) {
+              bkey[i] = kl;
+              bkey[l] = ki;
+              bpk[i] = pl;
+              bpk[l] = pi;
             }
           }
         }
-    if (!any_left) break;
-    if (cnt == L && worst_k < lb_next * lb_next * (1.0 - 1e-12)) break;
+        __threadfence();
+      }
+    const uint32_t outn = min(cnt, Lw);
+    for (uint32_t e = lane; e < outn; e += 64) kpk[(size_t)e * nwork + w] = bpk[e];
+    if (lane == 0) kcnt[w] = outn;
+    __threadfence();
   }
-  kcnt[w] = cnt;
 }
 
 // work-list compaction by flag (ascending order kept)

@@ -356,6 +356,34 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
   return g;
 }
 
+// neighbour lists: the LDS gather buffer must hold more than L entries;
+// long lists (and cells that did not fit) use the global-memory variant
+constexpr uint32_t BIG_KNN_CAP = 65536;
+constexpr uint32_t BIG_KNN_WAVES = 1024;
+
+void launch_knn(const double4 *spos, const uint32_t *cell_start,
+                const GridParams &grid, const BoxParams &box,
+                const uint32_t *wl, uint32_t cnt, uint32_t L,
+                const uint32_t *Lper, const float *Rper, uint64_t *kpk,
+                uint32_t *kcnt, bool big, Temps &TT, hipStream_t s) {
+  if (cnt == 0) return;
+  if (!big && L <= 256) {
+    hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kpk, kcnt);
+  } else if (!big && L <= 1024) {
+    hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kpk, kcnt);
+  } else {
+    uint32_t waves = std::min<uint32_t>(cnt, BIG_KNN_WAVES);
+    waves = (waves + 3) / 4 * 4;
+    double *gkey = need(TT.get<double>((size_t)waves * BIG_KNN_CAP));
+    uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * BIG_KNN_CAP));
+    hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, grid,
+                       box, wl, cnt, L, Lper, kpk, kcnt, gkey, gpk, BIG_KNN_CAP);
+  }
+  HIPCHK(hipGetLastError());
+}
+
 void build_impl(mv_tess *T, const double *d_xyz, size_t n,
                 const uint8_t *d_mask, const double anchor_in[3],
                 const double width_in[3], uint32_t dim, uint32_t periodic,
@@ -496,7 +524,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   };
 
   uint32_t L = opt.k0 ? opt.k0 : 27;
-  const uint32_t kmax = opt.k_max ? opt.k_max : 4096;
+  const uint32_t kmax = std::min<uint32_t>(opt.k_max ? opt.k_max : 32768, 32768);
   uint64_t redone = 0, bigcells = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
@@ -527,12 +555,9 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
       passes++;
       auto run = [&](uint32_t *wl, uint32_t cnt, uint8_t *state, bool big) {
         Timer tk(s);
-        double *kkey = need(TT.get<double>((size_t)Lp * cnt));
         uint64_t *kpk = need(TT.get<uint64_t>((size_t)Lp * cnt));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
-        hipLaunchKernelGGL(k_knn, dim3(nblk(cnt)), dim3(256), 0, s, spos, cell_start, grid,
-                           box, wl, cnt, Lp, kkey, kpk, kcnt);
-        HIPCHK(hipGetLastError());
+        launch_knn(spos, cell_start, grid, box, wl, cnt, Lp, nullptr, nullptr, kpk, kcnt, big, TT, s);
         ms_knn += tk.stop();
         Timer tc(s);
         ClipArgs A;
@@ -713,6 +738,7 @@ mv_status run_build(const double *d_xyz, size_t n, const uint8_t *d_mask,
   mv_options o;
   mv_default_options(&o);
   if (opt) o = *opt;
+  if (o.mode != MV_MODE_PLAIN) return MV_E_ARG;
   mv_tess *T = new (std::nothrow) mv_tess();
   if (!T) return MV_E_OOM;
   try {
@@ -739,7 +765,7 @@ void mv_default_options(mv_options *opt) {
   memset(opt, 0, sizeof *opt);
   opt->mode = MV_MODE_PLAIN;
   opt->k0 = 27;
-  opt->k_max = 4096;
+  opt->k_max = 32768;
   opt->lanes = 0;
 }
 

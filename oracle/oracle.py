@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build(force: bool = False) -> None:
+def compile_lib(force: bool = False) -> None:
     """Compile liboracle.so / liboracle_flops.so with the Makefile here."""
     libs = [os.path.join(_HERE, n) for n in ("liboracle.so", "liboracle_flops.so")]
     if force or not all(os.path.exists(p) for p in libs):
@@ -89,7 +89,7 @@ def lib(flops: bool = False):
     if name not in _libs:
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            build()
+            compile_lib()
         L = C.CDLL(path)
         L.orc_build.restype = C.c_int
         L.orc_build.argtypes = [
