@@ -59,9 +59,17 @@ typedef struct mv_options {
                         (plain mode); default 27 */
   uint32_t k_max;    /* largest neighbour list tried; default and
                         maximum 32768 */
-  uint32_t lanes;    /* cells in flight (thread-per-cell lanes); 0 = auto */
-  uint32_t reserved[8];
+  uint32_t lanes;    /* thread-per-cell lanes in flight; 0 = auto */
+  uint32_t grid_occupancy16; /* generators per grid cell x 16; 0 = 32 */
+  uint32_t flags;    /* MV_FLAG_* */
+  uint32_t reserved[6];
 } mv_options;
+
+/* Run every cell on the thread-per-cell kernel, which sums each integral in
+ * the reference's exact order (bit-identical volumes, centroids and face
+ * areas); the default one-cell-per-wavefront kernel gives the same
+ * connectivity with integrals equal up to rounding. */
+#define MV_FLAG_EXACT_ORDER 1u
 
 /* Fill *opt with the defaults. */
 void mv_default_options(mv_options *opt);

@@ -15,7 +15,7 @@ _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libmvoro.so")
 SOURCES = [os.path.join(_PKG, "csrc", f) for f in (
     "mvoro.hip", "mv_common.hpp", "mv_geom.hpp", "mv_grid.hpp", "mv_clip.hpp",
-    "mv_finalize.hpp")]
+    "mv_clipw.hpp", "mv_finalize.hpp")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -44,7 +44,8 @@ _lib = None
 
 class mv_options(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("k0", C.c_uint32), ("k_max", C.c_uint32),
-                ("lanes", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+                ("lanes", C.c_uint32), ("grid_occupancy16", C.c_uint32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 class mv_stats(C.Structure):

@@ -22,7 +22,8 @@ struct ClipArgs {
   uint64_t total_stream;
   const uint32_t *work;  // slots to build
   uint32_t nwork;
-  const uint64_t *kpk;   // neighbour lists, [j * nwork + w]
+  const uint64_t *kpk;   // neighbour lists, item-major
+  const uint64_t *kofs;  // list offset per item (nullptr: w * L)
   const uint32_t *kcnt;
   uint32_t L;
   // lane state
@@ -88,6 +89,7 @@ struct CellRegs {
   int np, nv;
   int cstart, clen;  // SimpleCycle start and len
   double sr;         // safety radius
+  uint32_t nexact;   // exact predicate evaluations
 };
 
 // Vertex::from_dual (src/voronoi/convex_cell.rs:28-48)
@@ -187,11 +189,21 @@ template <int MAXP, int MAXV>
 __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
                                   const ClipArgs &A, d3 n, d3 p, double d,
                                   double errb, uint64_t ref, d3 ngb) {
-  int i = 0, num_v = c.nv;
-  while (i < num_v) {
+  // Every vertex's decision depends on that vertex alone, so all of them
+  // are classified first (independent loads, no stores); the reference's
+  // swap-with-last scan is then replayed on the resulting bit mask, which
+  // moves the same vertices to the same positions.
+  static_assert(MAXV <= 1024, "mask width");
+  constexpr int MW = (MAXV + 63) / 64;
+  uint64_t m[MW];
+#pragma unroll
+  for (int k = 0; k < MW; k++) m[k] = 0;
+  double keepmax = -INFINITY;
+  bool nan_keep = false;
+  for (int i = 0; i < c.nv; i++) {
     double4 v = S.vtx[S.at(i)];
     double cv = dot3(n, xyz(v)) - d;
-    int clip;
+    bool clipped;
     if (fabs(cv) < errb) {
       // in-sphere test on snapped coordinates (convex_cell.rs:370-382)
       uint32_t du = S.vdual[S.at(i)];
@@ -201,24 +213,46 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
       iloc(A.box, right_loc(S, c, A, dual_at(du, 1)), ic);
       iloc(A.box, right_loc(S, c, A, dual_at(du, 2)), id);
       iloc(A.box, ngb, iv);
-      clip = in_sphere_exact(ia, ib, ic, id, iv);
-      atomicAdd(A.exact_counter, 1ull);
+      clipped = in_sphere_exact(ia, ib, ic, id, iv) < 0;
+      c.nexact++;
     } else {
-      clip = cv < 0. ? -1 : (cv > 0. ? 1 : 0);  // NaN keeps the vertex
+      clipped = cv < 0.;  // NaN keeps the vertex, as in the reference
     }
-    if (clip < 0) {
+    if (clipped) {
+      m[i >> 6] |= 1ull << (i & 63);
+    } else {
+      nan_keep |= isnan(v.w);
+      keepmax = fmax(keepmax, v.w);
+    }
+  }
+  uint64_t any = 0;
+#pragma unroll
+  for (int k = 0; k < MW; k++) any |= m[k];
+  if (!any) return 0;
+  if (nan_keep) return E_NAN;
+  // replay of the swap-with-last scan (convex_cell.rs:365-390)
+  int i = 0, num_v = c.nv;
+  while (i < num_v) {
+    bool bi = (m[i >> 6] >> (i & 63)) & 1ull;
+    if (bi) {
       num_v -= 1;
-      double4 t = S.vtx[S.at(num_v)];
-      uint32_t td = S.vdual[S.at(num_v)];
-      S.vtx[S.at(num_v)] = v;
-      S.vdual[S.at(num_v)] = S.vdual[S.at(i)];
-      S.vtx[S.at(i)] = t;
-      S.vdual[S.at(i)] = td;
+      bool bn = (m[num_v >> 6] >> (num_v & 63)) & 1ull;
+      if (num_v != i) {
+        // position i takes the last element; the clipped vertex's data is
+        // dead apart from its dual (used by the boundary walk)
+        S.vtx[S.at(i)] = S.vtx[S.at(num_v)];
+        uint32_t di = S.vdual[S.at(i)];
+        S.vdual[S.at(i)] = S.vdual[S.at(num_v)];
+        S.vdual[S.at(num_v)] = di;
+        if (!bn) {
+          m[i >> 6] &= ~(1ull << (i & 63));
+          m[num_v >> 6] |= 1ull << (num_v & 63);
+        }
+      }
     } else {
       i += 1;
     }
   }
-  if (num_v == c.nv) return 0;
   // push the plane and grow the cycle
   if (c.np >= MAXP) return 255;
   const int p_idx = c.np;
@@ -250,17 +284,22 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
   c.nv = num_v;
   if (c.nv + c.clen > MAXV) return 255;
   int cur = c.cstart;
+  double newmax = keepmax;
   for (int t = 0; t < c.clen; t++) {
     int next = S.cyc[S.at(cur)];
     double4 nvtx;
     if (!make_vertex(S, c, A.box.dim, cur, next, p_idx, &nvtx))
       return E_DET_ZERO;
+    if (isnan(nvtx.w)) return E_NAN;
+    newmax = fmax(newmax, nvtx.w);
     S.vtx[S.at(c.nv)] = nvtx;
     S.vdual[S.at(c.nv)] = pack_dual(cur, next, p_idx);
     c.nv++;
     cur = next;
   }
-  if (!update_safety_radius(S, c)) return E_NAN;
+  // update_safety_radius (convex_cell.rs:443-453): the max over the kept
+  // and the new vertices, tracked above
+  c.sr = 2. * sqrt(newmax);
   return 0;
 }
 
@@ -298,6 +337,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   c.np = 6;
   c.cstart = 0;
   c.clen = 0;
+  c.nexact = 0;
   const uint8_t D[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
                            {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
   c.nv = 8;
@@ -317,8 +357,8 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   // ConvexCell::build (convex_cell.rs:316-356)
   const uint32_t cnt = A.kcnt[w];
   if (cnt == 0xFFFFFFFFu) return CS_CAPACITY;  // list did not fit the gather
-  const size_t NW = A.nwork;
-  uint64_t first = A.kpk[w];
+  const uint64_t *list = A.kpk + (A.kofs ? A.kofs[w] : (uint64_t)w * A.L);
+  uint64_t first = list[0];
   if (cnt == 0 || (uint64_t)pk_idx(first) != c.idx) {
     *err = E_SELF_NOT_FIRST;
     return CS_ERROR;
@@ -326,7 +366,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   bool stopped = false;
   uint32_t j = 1;
   for (; j < cnt; j++) {
-    uint64_t pk = A.kpk[(size_t)j * NW + w];
+    uint64_t pk = list[j];
     double4 g = A.spos[pk_slot(pk)];
     d3 ngb = xyz(g);
     d3 s;
@@ -445,6 +485,7 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
   S.lane = gl;
   S.lanes = A.lanes;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  unsigned long long my_exact = 0;
   // wave-uniform trip count: every lane of a wavefront runs the same
   // iterations, so the face-offset scan below sees a converged wavefront
   for (size_t w0 = gl - lane; w0 < A.nwork; w0 += stride) {
@@ -459,6 +500,7 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
       c.idx = (uint64_t)__double_as_longlong(qq.w);
       uint32_t err = 0, npop = 0;
       uint8_t st = build_cell<MAXP, MAXV>(S, c, A, (uint32_t)w, &nf, &npop, &err);
+      my_exact += c.nexact;
       A.wstate[w] = st;
       if (st == CS_ERROR) {
         atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
@@ -504,6 +546,7 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
       }
     }
   }
+  if (my_exact) atomicAdd(A.exact_counter, my_exact);
 }
 
 }  // namespace mv

@@ -1,0 +1,614 @@
+// mv_clipw.hpp -- per-cell construction with one cell per wavefront.
+//
+// Same algorithm as mv_clip.hpp (the reference's ConvexCell::build,
+// clip_by_plane, compute_boundary, update_safety_radius and decomposition:
+// src/voronoi/convex_cell.rs:294-453, 131-187), laid out for CDNA:
+//   - vertex slot s of the cell's vertex array lives in lane s (x, y, z,
+//     radius2, dual in registers), so one plane test is one VALU pass;
+//   - the cell's half-spaces live in LDS (n, d | p, reference word);
+//   - the SimpleCycle pointer array lives in one register (lane p holds
+//     ptrs[p]); the reference's sequential steps -- the swap-with-last scan,
+//     the boundary walk, the new-vertex order -- run as wave-uniform scalar
+//     code over ballots and lane reads, so the vertex array, the plane list
+//     and every clip decision are the reference's;
+//   - the neighbour planes are prepared 64 at a time, one per lane, and
+//     consumed in stream order;
+//   - waves pull cells from a device work queue (persistent kernel).
+// Cells that need more than 64 vertices or 64 planes report CS_CAPACITY and
+// are rebuilt by the thread-per-cell kernel with larger capacity.
+//
+// Integrals: each lane evaluates the six tets of its vertex and stages them
+// in LDS; every face and the cell then sum their tets in the reference's
+// order (vertex order, then tet order), so volumes, centroids and face
+// areas are the reference's bits.
+#pragma once
+#include "mv_clip.hpp"
+#include "mv_common.hpp"
+
+namespace mv {
+
+constexpr int WCAP = 64;  // vertices and planes per cell on this path
+constexpr int WCH = 16;   // vertices per decomposition chunk
+
+struct WaveLds {
+  double4 pa[WCAP];      // plane: n.xyz, d
+  double4 pb[WCAP];      // plane: p.xyz, reference word
+  // per (tet, vertex) of a chunk of WCH vertices:
+  double4 fcf[6][WCH];   // area, area * (v0 + v1 + v2)
+  double4 fcv[6][WCH];   // volume, volume * (v0 + v1 + v2 + gen)
+};
+
+struct WaveArgs {
+  const double4 *spos;
+  const uint8_t *mask;
+  BoxParams box;
+  uint64_t total_stream;
+  const uint32_t *work;
+  uint32_t nwork;
+  const uint64_t *kpk;    // neighbour lists
+  const uint64_t *kofs;   // list offset per work item (nullptr: w * L)
+  const uint32_t *kcnt;
+  uint32_t L;
+  unsigned int *queue;    // work-queue head
+  uint8_t *wstate;
+  double *cell_vol, *cell_cen, *cell_sr;
+  uint32_t *cell_nf;
+  uint64_t *cell_fbase;
+  uint32_t *cell_npop;
+  uint32_t *st_cell;
+  int64_t *st_right;
+  uint8_t *st_img;
+  double *st_area, *st_cen, *st_norm;
+  uint64_t st_cap;
+  unsigned long long *st_counter;
+  unsigned long long *err_word;
+  unsigned long long *exact_counter;
+  unsigned long long *overflow;
+};
+
+__device__ __forceinline__ int rl_i(int v, int l) {
+  return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+__device__ __forceinline__ uint64_t rl_u64(uint64_t v, int l) {
+  int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffull), l);
+  int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return ((uint64_t)(unsigned int)hi << 32) | (uint64_t)(unsigned int)lo;
+}
+// DPP move of a double (both halves with the same control)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, true);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+// Maximum over the wavefront of non-negative values (0 for inactive
+// lanes): row_shr 1/2/4/8 within rows of 16, then row_bcast 15/31 across
+// rows; lane 63 holds the result.  DPP keeps the reduction in the VALU.
+__device__ __forceinline__ double wave_max_nonneg(double v) {
+  v = fmax(v, dpp_d<0x111>(v));
+  v = fmax(v, dpp_d<0x112>(v));
+  v = fmax(v, dpp_d<0x114>(v));
+  v = fmax(v, dpp_d<0x118>(v));
+  v = fmax(v, dpp_d<0x142>(v));
+  v = fmax(v, dpp_d<0x143>(v));
+  return rl_d(v, 63);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ void lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// HalfSpace::right_loc of a stored plane (half_space.rs:53-65)
+__device__ __forceinline__ d3 w_right_loc(const WaveLds &S, const WaveArgs &A,
+                                          d3 q, int j) {
+  double4 b = S.pb[j];
+  uint64_t ref = (uint64_t)__double_as_longlong(b.w);
+  if (ref == BOX_REF) {
+    double4 a = S.pa[j];
+    d3 pr = plane_project_onto(xyz(a), xyz(b), q);
+    return sub3(scl3(2., pr), q);
+  }
+  d3 l = xyz(A.spos[pk_slot(ref)]);
+  d3 s;
+  if (image_shift(pk_img(ref), A.box.width, &s)) l = add3(l, s);
+  return l;
+}
+
+// Vertex::from_dual with the planes in LDS
+__device__ __forceinline__ bool w_vertex(const WaveLds &S, d3 q, int dim,
+                                         int i, int j, int k, d3 *loc,
+                                         double *r2) {
+  double4 a = S.pa[i], b = S.pa[j], e = S.pa[k];
+  if (!intersect_planes(xyz(a), a.w, xyz(b), b.w, xyz(e), e.w, loc))
+    return false;
+  d3 dl = *loc;
+  if (dim == 1) {
+    dl.y = 0.;
+    dl.z = 0.;
+  } else if (dim == 2) {
+    dl.z = 0.;
+  }
+  d3 df = sub3(q, dl);
+  *r2 = dot3(df, df);
+  return true;
+}
+
+// Vertex::from_dual for (i, j, new plane) with the new plane in registers
+__device__ __forceinline__ bool w_vertex_np(const WaveLds &S, d3 q, int dim,
+                                            int i, int j, d3 n, double d,
+                                            d3 *loc, double *r2) {
+  double4 a = S.pa[i], b = S.pa[j];
+  if (!intersect_planes(xyz(a), a.w, xyz(b), b.w, n, d, loc)) return false;
+  d3 dl = *loc;
+  if (dim == 1) {
+    dl.y = 0.;
+    dl.z = 0.;
+  } else if (dim == 2) {
+    dl.z = 0.;
+  }
+  d3 df = sub3(q, dl);
+  *r2 = dot3(df, df);
+  return true;
+}
+
+// wave-uniform SimpleCycle ops on the pointer register `cyc`
+__device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
+  cyc = lane == at ? val : cyc;
+}
+
+__global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
+  __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
+  WaveLds &S = lds_all[threadIdx.x >> 6];
+  const int lane = (int)__lane_id();
+  const BoxParams &B = A.box;
+  const int dim = B.dim;
+  unsigned long long my_exact = 0;
+
+  uint32_t next = 0;
+  if (lane == 0) next = atomicAdd(A.queue, 1u);
+  next = (uint32_t)rl_i((int)next, 0);
+  for (;;) {
+    const uint32_t w = next;
+    if (w >= A.nwork) break;
+    // claim the following item now; its latency overlaps this cell
+    uint32_t nx = 0;
+    if (lane == 0) nx = atomicAdd(A.queue, 1u);
+
+    const double4 qq = A.spos[A.work[w]];
+    const d3 q = xyz(qq);
+    const uint64_t idx = (uint64_t)__double_as_longlong(qq.w);
+    const uint32_t cnt = A.kcnt[w];
+    const uint64_t* list = A.kpk + (A.kofs ? A.kofs[w] : (uint64_t)w * A.L);
+    uint8_t state = CS_DONE;
+    uint32_t err = 0, npop = 0, nexact = 0;
+
+    // ---- ConvexCell::init (convex_cell.rs:294-312)
+    if (lane < 6) {
+      S.pa[lane] = make_double4(B.pn[lane][0], B.pn[lane][1], B.pn[lane][2], B.pd[lane]);
+      S.pb[lane] = make_double4(B.pp[lane][0], B.pp[lane][1], B.pp[lane][2],
+                                __longlong_as_double((long long)BOX_REF));
+    }
+    lds_fence();
+    int np = 6, nv = 8, cstart = 0, clen = 0;
+    int cyc = lane;  // ptrs[p] of SimpleCycle::new
+    double vx = 0., vy = 0., vz = 0., vr = -INFINITY;
+    int vd = 0;
+    {
+      // init vertex duals (2,5,0) (5,3,0) (1,5,2) (5,1,3) (4,2,0) (4,0,3)
+      // (2,4,1) (4,3,1), packed 8 bits each
+      const uint32_t DU[8] = {0x00000502u, 0x00000305u, 0x00020501u, 0x00030105u,
+                              0x00000204u, 0x00030004u, 0x00010402u, 0x00010304u};
+      uint32_t du = DU[lane & 7];
+      bool ok = true;
+      if (lane < 8) {
+        d3 l;
+        double r2;
+        ok = w_vertex(S, q, dim, du & 0xFF, (du >> 8) & 0xFF, (du >> 16) & 0xFF, &l, &r2);
+        vx = l.x;
+        vy = l.y;
+        vz = l.z;
+        vr = r2;
+        vd = (int)du;
+      }
+      if (__ballot(!ok)) {
+        state = CS_ERROR;
+        err = E_DET_ZERO;
+      }
+    }
+    double sr = 0.;
+    if (state == CS_DONE) {
+      if (__ballot(lane < nv && isnan(vr))) {
+        state = CS_ERROR;
+        err = E_NAN;
+      }
+      sr = 2. * sqrt(wave_max_nonneg(lane < nv ? vr : 0.));
+    }
+    if (state == CS_DONE && cnt == 0xFFFFFFFFu) state = CS_CAPACITY;
+    if (state == CS_DONE && (cnt == 0 || (uint64_t)pk_idx(list[0]) != idx)) {
+      state = CS_ERROR;
+      err = E_SELF_NOT_FIRST;
+    }
+
+    // ---- ConvexCell::build (convex_cell.rs:316-356)
+    bool stopped = false;
+    for (uint32_t j0 = 1; state == CS_DONE && !stopped && j0 < cnt; j0 += 64) {
+      // prepare the planes of entries j0 .. j0+63, one per lane
+      const uint32_t je = j0 + (uint32_t)lane;
+      uint64_t ppk = 0;
+      d3 pn = mk3(0, 0, 0), pp = mk3(0, 0, 0);
+      double pdist = 1., pd = 0., perrb = 0.;
+      if (je < cnt) {
+        ppk = list[je];
+        d3 ngb = xyz(A.spos[pk_slot(ppk)]);
+        d3 s;
+        if (image_shift(pk_img(ppk), B.width, &s)) ngb = add3(ngb, s);
+        d3 dx = sub3(q, ngb);
+        pdist = sqrt(dot3(dx, dx));
+        pn = div3(dx, pdist);
+        pp = scl3(0.5, add3(q, ngb));
+        // HalfSpace::new (half_space.rs:24-34)
+        perrb = 1e-13 * (1. + dot3(abs3(pn), abs3(pp)));
+        pd = dot3(pn, pp);
+      }
+      const int nb = (int)min(64u, cnt - j0);
+      for (int t = 0; t < nb; t++) {
+        const double dist = rl_d(pdist, t);
+        if (!(isfinite(dist) && dist > 0.0)) {
+          state = CS_ERROR;
+          err = E_DEGENERATE;
+          break;
+        }
+        if (sr < dist) {
+          stopped = true;
+          npop = j0 + (uint32_t)t + 1;
+          break;
+        }
+        const d3 n = mk3(rl_d(pn.x, t), rl_d(pn.y, t), rl_d(pn.z, t));
+        const double d = rl_d(pd, t);
+        const double errb = rl_d(perrb, t);
+        // classify this lane's vertex (clip_by_plane, convex_cell.rs:368-390)
+        const bool act = lane < nv;
+        const double cv = dot3(n, mk3(vx, vy, vz)) - d;
+        const bool near = act && fabs(cv) < errb;
+        bool clipped = act && !near && cv < 0.;
+        const uint64_t ref = rl_u64(ppk, t);
+        if (__ballot(near)) {
+          d3 ngb = xyz(A.spos[pk_slot(ref)]);
+          d3 sh;
+          if (image_shift(pk_img(ref), B.width, &sh)) ngb = add3(ngb, sh);
+          if (near) {
+            int64_t ia[3], ib[3], ic[3], id[3], iv[3];
+            iloc(B, q, ia);
+            iloc(B, w_right_loc(S, A, q, vd & 0xFF), ib);
+            iloc(B, w_right_loc(S, A, q, (vd >> 8) & 0xFF), ic);
+            iloc(B, w_right_loc(S, A, q, (vd >> 16) & 0xFF), id);
+            iloc(B, ngb, iv);
+            clipped = in_sphere_exact(ia, ib, ic, id, iv) < 0;
+            nexact++;
+          }
+        }
+        uint64_t M = __ballot(clipped);
+        if (!M) continue;
+        // kept vertices' radius maximum, and the NaN panic of
+        // update_safety_radius
+        const bool kept = act && !clipped;
+        if (__ballot(kept && isnan(vr))) {
+          state = CS_ERROR;
+          err = E_NAN;
+          break;
+        }
+        const double keepmax = wave_max_nonneg(kept ? vr : 0.);
+        // replay of the swap-with-last scan on the mask: `src` = the slot
+        // each position takes its vertex from
+        int src = lane;
+        int i = 0, num_v = nv;
+        uint64_t m = M;
+        for (;;) {
+          // next position >= i (below num_v) holding a clipped vertex
+          const uint64_t below = num_v >= 64 ? ~0ull : ((1ull << num_v) - 1ull);
+          const uint64_t cand = m & below & (~0ull << i);
+          if (!cand) break;
+          i = __builtin_ctzll(cand);
+          num_v -= 1;
+          if (num_v != i) {
+            const int a = rl_i(src, i), b = rl_i(src, num_v);
+            src = lane == i ? b : (lane == num_v ? a : src);
+            if (!((m >> num_v) & 1ull)) {
+              m &= ~(1ull << i);
+              m |= 1ull << num_v;
+            }
+          }
+        }
+        vx = __shfl(vx, src, 64);
+        vy = __shfl(vy, src, 64);
+        vz = __shfl(vz, src, 64);
+        vr = __shfl(vr, src, 64);
+        vd = __shfl(vd, src, 64);
+        // push the plane (index np); SimpleCycle::grow is the identity
+        // pointer already held by lane np
+        if (np >= WCAP) {
+          state = CS_CAPACITY;
+          break;
+        }
+        const int p_idx = np;
+        {
+          const d3 p = mk3(rl_d(pp.x, t), rl_d(pp.y, t), rl_d(pp.z, t));
+          if (lane == 0) {
+            S.pa[p_idx] = make_double4(n.x, n.y, n.z, d);
+            S.pb[p_idx] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)ref));
+          }
+        }
+        np += 1;
+        // compute_boundary (convex_cell.rs:421-441) over positions
+        // num_v .. nv-1
+        {
+          const int d0 = rl_i(vd, num_v);
+          const int a0 = d0 & 0xFF, b0 = (d0 >> 8) & 0xFF, c0 = (d0 >> 16) & 0xFF;
+          int cur = cstart;
+          for (int r = 0; r < clen; r++) {
+            const int nx = rl_i(cyc, cur);
+            wc_set(cyc, lane, cur, cur);
+            cur = nx;
+          }
+          clen = 3;
+          cstart = a0;
+          wc_set(cyc, lane, a0, b0);
+          wc_set(cyc, lane, b0, c0);
+          wc_set(cyc, lane, c0, a0);
+          for (int r = num_v + 1; r < nv && state == CS_DONE; r++) {
+            int ix = r;
+            for (;;) {
+              if (ix >= nv) {
+                state = CS_ERROR;
+                err = E_WALK;
+                break;
+              }
+              const int du = rl_i(vd, ix);
+              const int tri[3] = {du & 0xFF, (du >> 8) & 0xFF, (du >> 16) & 0xFF};
+              const int ptr[3] = {rl_i(cyc, tri[0]), rl_i(cyc, tri[1]), rl_i(cyc, tri[2])};
+              const bool con[3] = {ptr[0] != tri[0], ptr[1] != tri[1], ptr[2] != tri[2]};
+              bool ok = false;
+              for (int e = 0; e < 3 && !ok; e++) {
+                const int f = e == 2 ? 0 : e + 1, g = e == 0 ? 2 : e - 1;
+                if (!con[e] && con[f] && con[g] && ptr[g] == tri[f]) {
+                  wc_set(cyc, lane, tri[g], tri[e]);
+                  wc_set(cyc, lane, tri[e], tri[f]);
+                  clen += 1;
+                  ok = true;
+                } else if (con[e] && con[f] && con[g] && ptr[g] == tri[f] &&
+                           ptr[f] == tri[e]) {
+                  wc_set(cyc, lane, tri[g], tri[e]);
+                  wc_set(cyc, lane, tri[f], tri[f]);
+                  if (cstart == tri[f]) cstart = tri[e];
+                  clen -= 1;
+                  ok = true;
+                }
+              }
+              if (ok) {
+                if (ix > r) {
+                  const int dr = rl_i(vd, r);
+                  vd = lane == r ? du : (lane == ix ? dr : vd);
+                }
+                break;
+              }
+              ix++;
+            }
+          }
+          if (state != CS_DONE) break;
+        }
+        nv = num_v;
+        if (nv + clen > WCAP) {
+          state = CS_CAPACITY;
+          break;
+        }
+        // new vertices (cur, next, p_idx) in cycle order from the start
+        int vc = 0, vn = 0;
+        {
+          int cur = cstart;
+          for (int r = 0; r < clen; r++) {
+            const int nx = rl_i(cyc, cur);
+            vc = lane == nv + r ? cur : vc;
+            vn = lane == nv + r ? nx : vn;
+            cur = nx;
+          }
+        }
+        const bool mk = lane >= nv && lane < nv + clen;
+        bool ok = true;
+        double newr = 0.;
+        if (mk) {
+          d3 l;
+          double r2;
+          ok = w_vertex_np(S, q, dim, vc, vn, n, d, &l, &r2);
+          vx = l.x;
+          vy = l.y;
+          vz = l.z;
+          vr = r2;
+          vd = (int)pack_dual(vc, vn, p_idx);
+          newr = r2;
+        }
+        if (__ballot(!ok)) {
+          state = CS_ERROR;
+          err = E_DET_ZERO;
+          break;
+        }
+        if (__ballot(mk && isnan(newr))) {
+          state = CS_ERROR;
+          err = E_NAN;
+          break;
+        }
+        nv += clen;
+        // update_safety_radius (convex_cell.rs:443-453)
+        sr = 2. * sqrt(fmax(keepmax, wave_max_nonneg(newr)));
+      }
+    }
+    if (state == CS_DONE && !stopped) {
+      if ((uint64_t)cnt < A.total_stream)
+        state = CS_NEED_MORE;
+      else
+        npop = cnt;
+    }
+
+    // ---- decomposition and integrals (convex_cell.rs:131-187,
+    // voronoi_cell.rs:40-100, integrals.rs:91-105, voronoi_face.rs:30-43)
+    uint32_t nf = 0;
+    uint64_t facemask = 0;
+    double fa = 0., fx = 0., fy = 0., fz = 0.;
+    if (state == CS_DONE) {
+      // which planes build a face (voronoi_cell.rs:53-73)
+      bool cons = false;
+      if (lane < np) {
+        const double4 a = S.pa[lane];
+        const uint64_t ref = (uint64_t)__double_as_longlong(S.pb[lane].w);
+        cons = true;
+        if (dim == 1 && !(a.y == 0. && a.z == 0.)) cons = false;
+        if (dim == 2 && !(a.z == 0.)) cons = false;
+        if (cons && ref != BOX_REF && pk_img(ref) == 13) {
+          const uint32_t right = pk_idx(ref);
+          cons = (uint64_t)right > idx || (A.mask != nullptr && !A.mask[right]);
+        }
+      }
+      // vertices in chunks of WCH: each lane evaluates the six tets of its
+      // vertex and stages them in LDS; faces (lane p = plane p) and the cell
+      // (every lane, broadcast reads) then accumulate them in the
+      // reference's order: vertex order, tet order within a vertex
+      double vol = 0., cx = 0., cy = 0., cz = 0.;
+      bool ok = true, has = false;
+      for (int c0 = 0; c0 < nv && ok; c0 += WCH) {
+        const int c1 = min(nv, c0 + WCH);
+        bool lok = true;
+        if (lane >= c0 && lane < c1) {
+          const int sl = lane - c0;
+          const d3 vl = mk3(vx, vy, vz);
+          const int pl[3] = {vd & 0xFF, (vd >> 8) & 0xFF, (vd >> 16) & 0xFF};
+          d3 prev;
+          {
+            const double4 a0 = S.pa[pl[0]], a2 = S.pa[pl[2]];
+            lok = project_onto_intersection(xyz(a0), a0.w, xyz(a2), a2.w, q, &prev);
+          }
+          for (int k = 0; k < 3 && lok; k++) {
+            const int pi = pl[k], pn2 = pl[k == 2 ? 0 : k + 1];
+            const double4 ak = S.pa[pi], bk = S.pb[pi], an = S.pa[pn2];
+            const d3 pface = plane_project_onto(xyz(ak), xyz(bk), q);
+            d3 pedge;
+            lok = project_onto_intersection(xyz(an), an.w, xyz(ak), ak.w, q, &pedge);
+            if (!lok) break;
+            for (int h = 0; h < 2; h++) {
+              const d3 v0 = h == 0 ? pface : pedge;
+              const d3 v1 = h == 0 ? prev : pface;
+              const double tv = signed_volume_tet(v0, v1, vl, q);
+              const d3 tc = scl3(tv, add3(add3(add3(v0, v1), vl), q));
+              const double a = signed_area_tri(v0, v1, vl, q);
+              const d3 sc = scl3(a, add3(add3(v0, v1), vl));
+              S.fcv[2 * k + h][sl] = make_double4(tv, tc.x, tc.y, tc.z);
+              S.fcf[2 * k + h][sl] = make_double4(a, sc.x, sc.y, sc.z);
+            }
+            prev = pedge;
+          }
+        }
+        if (__ballot(!lok)) {
+          ok = false;
+          break;
+        }
+        lds_fence();
+        for (int s2 = c0; s2 < c1; s2++) {
+          const int sl = s2 - c0;
+          // VolumeCentroidIntegral::collect (integrals.rs:91-95), 6 tets
+#pragma unroll
+          for (int t = 0; t < 6; t++) {
+            const double4 c = S.fcv[t][sl];
+            vol += c.x;
+            cx += c.y;
+            cy += c.z;
+            cz += c.w;
+          }
+          // VoronoiFaceIntegral::collect for the two tets on plane `lane`
+          const int du = rl_i(vd, s2);
+          const int k = (du & 0xFF) == lane ? 0
+                        : ((du >> 8) & 0xFF) == lane ? 1
+                        : ((du >> 16) & 0xFF) == lane ? 2 : -1;
+          if (k >= 0) {
+            const double4 f0 = S.fcf[2 * k][sl], f1 = S.fcf[2 * k + 1][sl];
+            fa += f0.x;
+            fx += f0.y;
+            fy += f0.z;
+            fz += f0.w;
+            fa += f1.x;
+            fx += f1.y;
+            fy += f1.z;
+            fz += f1.w;
+            has = true;
+          }
+        }
+        lds_fence();
+      }
+      if (!ok) {
+        state = CS_ERROR;
+        err = E_DET_ZERO;
+      } else {
+        facemask = __ballot(lane < np && cons && has);
+        nf = (uint32_t)__popcll(facemask);
+        if (lane == 0) {
+          // VolumeCentroidIntegral::finalize (integrals.rs:97-105)
+          const double norm = vol > 0. ? 0.25 / vol : 0.;
+          A.cell_vol[idx] = vol;
+          A.cell_cen[3 * idx] = cx * norm;
+          A.cell_cen[3 * idx + 1] = cy * norm;
+          A.cell_cen[3 * idx + 2] = cz * norm;
+          A.cell_sr[idx] = sr;
+          A.cell_npop[idx] = npop;
+        }
+      }
+    }
+    if (lane == 0) {
+      A.wstate[w] = state;
+      if (state == CS_ERROR)
+        atomicMin(A.err_word, (unsigned long long)((idx << 8) | err));
+    }
+    my_exact += nexact;
+    if (state == CS_DONE) {
+      unsigned long long base = 0;
+      if (lane == 0) {
+        base = nf ? atomicAdd(A.st_counter, (unsigned long long)nf) : 0ull;
+        A.cell_nf[idx] = nf;
+        A.cell_fbase[idx] = base;
+      }
+      base = rl_u64(base, 0);
+      if (base + nf > A.st_cap) {
+        if (lane == 0) atomicOr(A.overflow, 1ull);
+      } else if ((facemask >> lane) & 1ull) {
+        const uint64_t o = base + (uint64_t)__popcll(facemask & ((1ull << lane) - 1ull));
+        const double4 a = S.pa[lane];
+        const uint64_t ref = (uint64_t)__double_as_longlong(S.pb[lane].w);
+        const double fnrm = fa > 0. ? 1. / (3. * fa) : 0.;
+        A.st_cell[o] = (uint32_t)idx;
+        A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+        A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
+        A.st_area[o] = fa;
+        A.st_cen[3 * o] = fx * fnrm;
+        A.st_cen[3 * o + 1] = fy * fnrm;
+        A.st_cen[3 * o + 2] = fz * fnrm;
+        A.st_norm[3 * o] = a.x;
+        A.st_norm[3 * o + 1] = a.y;
+        A.st_norm[3 * o + 2] = a.z;
+      }
+    }
+    next = (uint32_t)rl_i((int)nx, 0);
+    lds_fence();
+  }
+  my_exact = (unsigned long long)wave_sum((double)my_exact);
+  if (lane == 0 && my_exact) atomicAdd(A.exact_counter, my_exact);
+}
+
+}  // namespace mv

@@ -206,8 +206,8 @@ __global__ void k_scan_add(T *__restrict__ out, size_t n,
 // exactly the first L of the stream; each lane ranks its entries by
 // counting the smaller ones in the buffer and writes those of rank < L.
 // Otherwise R grows (or shrinks when the buffer overflows) and the gather
-// is repeated.  The list is written interleaved: entry j of item w at
-// [j * nwork + w].
+// is repeated.  Lists are item-major: entry j of item w at kofs[w] + j
+// (w * L without offsets).
 // ---------------------------------------------------------------------
 __device__ __forceinline__ bool entry_less(double ka, uint64_t pa, double kb,
                                            uint64_t pb) {
@@ -359,8 +359,8 @@ __global__ void __launch_bounds__(256)
           const uint32_t *__restrict__ cell_start, GridParams g,
           BoxParams box, const uint32_t *__restrict__ work, uint32_t nwork,
           uint32_t L, const uint32_t *__restrict__ Lper,
-          const float *__restrict__ Rper, uint64_t *__restrict__ kpk,
-          uint32_t *__restrict__ kcnt) {
+          const float *__restrict__ Rper, const uint64_t *__restrict__ kofs,
+          uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt) {
   __shared__ KnnLds<CAP> lds[WPB];
   const uint32_t lane = __lane_id();
   const uint32_t wib = threadIdx.x >> 6;
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(256)
       uint32_t rank = 0;
       for (uint32_t t = 0; t < cnt; t++)
         rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
-      if (rank < outn) kpk[(size_t)rank * nwork + w] = pe;
+      if (rank < outn) kpk[(kofs ? kofs[w] : (uint64_t)w * L) + rank] = pe;
     }
     if (lane == 0) kcnt[w] = outn;
     __builtin_amdgcn_wave_barrier();
@@ -401,7 +401,8 @@ __global__ void __launch_bounds__(256)
               const uint32_t *__restrict__ cell_start, GridParams g,
               BoxParams box, const uint32_t *__restrict__ work,
               uint32_t nwork, uint32_t L, const uint32_t *__restrict__ Lper,
-              uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
+              const uint64_t *__restrict__ kofs, uint64_t *__restrict__ kpk,
+              uint32_t *__restrict__ kcnt,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
               uint32_t cap) {
   const uint32_t lane = __lane_id();
@@ -446,7 +447,8 @@ __global__ void __launch_bounds__(256)
         __threadfence();
       }
     const uint32_t outn = min(cnt, Lw);
-    for (uint32_t e = lane; e < outn; e += 64) kpk[(size_t)e * nwork + w] = bpk[e];
+    uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
+    for (uint32_t e = lane; e < outn; e += 64) out[e] = bpk[e];
     if (lane == 0) kcnt[w] = outn;
     __threadfence();
   }

@@ -19,6 +19,7 @@
 
 #include "mvoro.h"
 #include "mv_clip.hpp"
+#include "mv_clipw.hpp"
 #include "mv_common.hpp"
 #include "mv_finalize.hpp"
 #include "mv_grid.hpp"
@@ -364,22 +365,22 @@ constexpr uint32_t BIG_KNN_WAVES = 1024;
 void launch_knn(const double4 *spos, const uint32_t *cell_start,
                 const GridParams &grid, const BoxParams &box,
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
-                const uint32_t *Lper, const float *Rper, uint64_t *kpk,
-                uint32_t *kcnt, bool big, Temps &TT, hipStream_t s) {
+                const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
+                uint64_t *kpk, uint32_t *kcnt, bool big, Temps &TT, hipStream_t s) {
   if (cnt == 0) return;
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kpk, kcnt);
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt);
   } else if (!big && L <= 1024) {
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kpk, kcnt);
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt);
   } else {
     uint32_t waves = std::min<uint32_t>(cnt, BIG_KNN_WAVES);
     waves = (waves + 3) / 4 * 4;
     double *gkey = need(TT.get<double>((size_t)waves * BIG_KNN_CAP));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * BIG_KNN_CAP));
     hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, grid,
-                       box, wl, cnt, L, Lper, kpk, kcnt, gkey, gpk, BIG_KNN_CAP);
+                       box, wl, cnt, L, Lper, kofs, kpk, kcnt, gkey, gpk, BIG_KNN_CAP);
   }
   HIPCHK(hipGetLastError());
 }
@@ -450,7 +451,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     if (!std::isfinite(bb[c])) throw BuildError{MV_E_DEGENERATE_POINTS};
 
   BoxParams box = make_box(anchor, width, (int)periodic, (int)dim);
-  double occ = opt.reserved[0] ? (double)opt.reserved[0] / 16.0 : 2.0;
+  double occ = opt.grid_occupancy16 ? (double)opt.grid_occupancy16 / 16.0 : 2.0;
   GridParams grid = make_grid(bb, n, width, (int)dim, (int)periodic, occ);
   size_t ncell = (size_t)grid.G[0] * grid.G[1] * grid.G[2];
 
@@ -499,6 +500,8 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   unsigned long long *counters = need(TT.get<unsigned long long>(4));
   // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow
   uint8_t *wstate = need(TT.get<uint8_t>(n));
+  unsigned int *queue = need(TT.get<unsigned int>(1));
+  const bool exact_order = (opt.flags & MV_FLAG_EXACT_ORDER) != 0;
   uint8_t *bstate = need(TT.get<uint8_t>(n));
 
   // lane state
@@ -557,9 +560,49 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         Timer tk(s);
         uint64_t *kpk = need(TT.get<uint64_t>((size_t)Lp * cnt));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
-        launch_knn(spos, cell_start, grid, box, wl, cnt, Lp, nullptr, nullptr, kpk, kcnt, big, TT, s);
+        launch_knn(spos, cell_start, grid, box, wl, cnt, Lp, nullptr, nullptr, nullptr, kpk, kcnt,
+                   big, TT, s);
         ms_knn += tk.stop();
         Timer tc(s);
+        if (!big && !exact_order) {
+          WaveArgs W;
+          memset(&W, 0, sizeof W);
+          W.spos = spos;
+          W.mask = d_mask;
+          W.box = box;
+          W.total_stream = grid.total_stream;
+          W.work = wl;
+          W.nwork = cnt;
+          W.kpk = kpk;
+          W.kofs = nullptr;
+          W.kcnt = kcnt;
+          W.L = Lp;
+          W.queue = queue;
+          W.wstate = state;
+          W.cell_vol = T->d_cell_vol;
+          W.cell_cen = T->d_cell_cen;
+          W.cell_sr = T->d_cell_sr;
+          W.cell_nf = cell_nf;
+          W.cell_fbase = cell_fbase;
+          W.cell_npop = T->d_npop;
+          W.st_cell = st_cell;
+          W.st_right = st_right;
+          W.st_img = st_img;
+          W.st_area = st_area;
+          W.st_cen = st_cen;
+          W.st_norm = st_norm;
+          W.st_cap = st_cap;
+          W.st_counter = counters;
+          W.err_word = counters + 1;
+          W.exact_counter = counters + 2;
+          W.overflow = counters + 3;
+          HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
+          uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * 4u);
+          hipLaunchKernelGGL(k_clip_wave, dim3(blocks), dim3(256), 0, s, W);
+          HIPCHK(hipGetLastError());
+          ms_clip += tc.stop();
+          return;
+        }
         ClipArgs A;
         memset(&A, 0, sizeof A);
         A.spos = spos;
@@ -569,6 +612,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         A.work = wl;
         A.nwork = cnt;
         A.kpk = kpk;
+        A.kofs = nullptr;
         A.kcnt = kcnt;
         A.L = Lp;
         uint32_t lanes;

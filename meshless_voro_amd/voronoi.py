@@ -56,15 +56,17 @@ class BuildOptions:
     """mv_options: construction mode and neighbour-list parameters."""
 
     def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 32768,
-                 lanes: int = 0, occupancy16: int = 0):
+                 lanes: int = 0, occupancy16: int = 0, exact_order: bool = False):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
+        self.exact_order = exact_order
 
     def to_c(self) -> _native.mv_options:
         o = _native.mv_options()
         _native.lib().mv_default_options(C.byref(o))
         o.mode, o.k0, o.k_max, o.lanes = self.mode, self.k0, self.k_max, self.lanes
-        o.reserved[0] = self.occupancy16
+        o.grid_occupancy16 = self.occupancy16
+        o.flags = 1 if self.exact_order else 0
         return o
 
 

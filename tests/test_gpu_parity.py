@@ -191,3 +191,23 @@ def test_large_uniform_properties():
     # sample parity with the oracle on a 65,536-seed prefix of the stream
     xs = x[:65536]
     check(*both(xs, periodic=True))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_exact_order_mode_is_bit_identical(periodic):
+    """MV_FLAG_EXACT_ORDER: the thread-per-cell kernel sums every integral in
+    the reference's order, so all outputs equal the oracle's bit for bit."""
+    x = W.uniform(20000, 555)
+    g, o = both(x, periodic=periodic, options=BuildOptions(exact_order=True))
+    assert_connectivity_equal(g, o)
+    for f in ("cell_volume", "cell_centroid", "face_area", "face_centroid", "face_normal",
+              "cell_safety_radius", "face_shift"):
+        assert np.array_equal(getattr(g, f), getattr(o, f)), f
+
+
+def test_default_mode_deterministic():
+    x = W.uniform(50000, 77)
+    a = Voronoi.build(x, ZERO, ONE, 3, True)
+    b = Voronoi.build(x, ZERO, ONE, 3, True)
+    for f in ("cell_volume", "cell_centroid", "face_area", "face_centroid", "links"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
