@@ -53,6 +53,8 @@ enum {
                          the security test still decides every cell */
 };
 
+struct mv_model;
+
 typedef struct mv_options {
   uint32_t mode;     /* MV_MODE_*; default MV_MODE_PLAIN */
   uint32_t k0;       /* neighbours in the first pass, self included
@@ -62,7 +64,8 @@ typedef struct mv_options {
   uint32_t lanes;    /* thread-per-cell lanes in flight; 0 = auto */
   uint32_t grid_occupancy16; /* generators per grid cell x 16; 0 = 32 */
   uint32_t flags;    /* MV_FLAG_* */
-  uint32_t reserved[6];
+  const struct mv_model *model; /* MV_MODE_LEARNED: proposal weights */
+  uint32_t reserved[4];
 } mv_options;
 
 /* Run every cell on the thread-per-cell kernel, which sums each integral in
@@ -165,6 +168,64 @@ void mv_get_stats(const mv_tess *t, mv_stats *s);
 const uint32_t *mv_cell_n_popped(mv_tess *t);
 
 const char *mv_status_string(mv_status s);
+
+/* ---------------------------------------------------------------------
+ * Learned neighbour-count proposal (no counterpart in the reference; it
+ * only sizes each cell's first neighbour list in MV_MODE_LEARNED, the
+ * security test still decides every cell).
+ * A 16 -> 64 -> 64 -> 2 ReLU MLP with BF16 weights; output 0 = log of the
+ * list length a cell needs (the neighbour-stream entries it clips with, self
+ * included: a list that long plus the distance of the next entry decides the
+ * cell), output 1 = log of its final safety radius over the local spacing
+ * (the start radius of its neighbour gather).  This struct is also the
+ * on-disk checkpoint format (little-endian, version 1). */
+#define MV_MODEL_MAGIC 0x4b4c564du /* "MVLK" */
+typedef struct mv_model {
+  uint32_t magic, version;
+  float mu[16], sigma[16];        /* feature normalisation */
+  uint16_t w1[16 * 64];           /* bf16, [input][output] */
+  uint16_t w2[64 * 64];
+  uint16_t w3[64 * 2];
+  float b1[64], b2[64], b3[2];
+  float tau;                      /* quantile of the training loss */
+  float kscale;                   /* multiplier on the proposed k */
+  float rscale;                   /* multiplier on the proposed gather
+                                     radius; 0 = start from the grid
+                                     density instead */
+} mv_model;
+
+typedef struct mv_train_config {
+  uint32_t steps;   /* Adam steps */
+  uint32_t batch;   /* rows per step (multiple of 16) */
+  uint32_t seed;    /* initialisation and batch sampling */
+  float lr;         /* Adam learning rate */
+  float tau;        /* pinball-loss quantile */
+} mv_train_config;
+
+/* The 16 proposal features of every generator (host in/out; features
+ * n x 16 and local spacing n, in generator order). */
+mv_status mv_model_features(const double *xyz, size_t n,
+                            const double anchor[3], const double width[3],
+                            uint32_t dimensionality, uint32_t periodic,
+                            float *features, float *hloc);
+
+/* Train on the GPU (BF16 MFMA forward and backward kernels, FP32 Adam):
+ * features n x 16, targets n x 2 (host).  *model receives the weights. */
+mv_status mv_model_train(const float *features, const float *targets,
+                         size_t n, const mv_train_config *cfg,
+                         mv_model *model, float *final_loss);
+
+/* Inference on the GPU: out n x 2 (host). */
+mv_status mv_model_predict(const mv_model *model, const float *features,
+                           size_t n, float *out);
+
+/* One forward and backward pass of the training kernels on the given rows
+ * (n multiple of 16): mean pinball loss and its gradient with respect to
+ * the FP32 parameters, laid out w1 (16x64), b1, w2 (64x64), b2, w3 (64x2),
+ * b3 -- 5378 floats. */
+mv_status mv_model_loss_grad(const mv_model *model, const float *features,
+                             const float *targets, size_t n, float tau,
+                             float *loss, float *grad);
 
 /* Exact in-sphere sign on snapped integer coordinates, evaluated on the
  * device for `count` 5-tuples (a, b, c, d, v; 15 int64 per tuple, host

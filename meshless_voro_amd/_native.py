@@ -15,7 +15,7 @@ _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libmvoro.so")
 SOURCES = [os.path.join(_PKG, "csrc", f) for f in (
     "mvoro.hip", "mv_common.hpp", "mv_geom.hpp", "mv_grid.hpp", "mv_clip.hpp",
-    "mv_clipw.hpp", "mv_finalize.hpp")]
+    "mv_clipw.hpp", "mv_finalize.hpp", "mv_learn.hpp")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -45,7 +45,25 @@ _lib = None
 class mv_options(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("k0", C.c_uint32), ("k_max", C.c_uint32),
                 ("lanes", C.c_uint32), ("grid_occupancy16", C.c_uint32),
-                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("flags", C.c_uint32), ("model", C.c_void_p),
+                ("reserved", C.c_uint32 * 4)]
+
+
+MV_MODEL_MAGIC = 0x4B4C564D
+
+
+class mv_model(C.Structure):
+    _fields_ = [("magic", C.c_uint32), ("version", C.c_uint32),
+                ("mu", C.c_float * 16), ("sigma", C.c_float * 16),
+                ("w1", C.c_uint16 * (16 * 64)), ("w2", C.c_uint16 * (64 * 64)),
+                ("w3", C.c_uint16 * (64 * 2)), ("b1", C.c_float * 64),
+                ("b2", C.c_float * 64), ("b3", C.c_float * 2),
+                ("tau", C.c_float), ("kscale", C.c_float), ("rscale", C.c_float)]
+
+
+class mv_train_config(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("seed", C.c_uint32),
+                ("lr", C.c_float), ("tau", C.c_float)]
 
 
 class mv_stats(C.Structure):
@@ -73,7 +91,8 @@ EXPORTS = [
     "mv_cell_idx", "mv_face_left", "mv_face_right", "mv_face_area",
     "mv_face_centroid", "mv_face_normal", "mv_face_has_shift", "mv_face_shift",
     "mv_cell_face_connections", "mv_get_device_view", "mv_get_stats",
-    "mv_cell_n_popped", "mv_status_string", "mv_in_sphere_exact",
+    "mv_cell_n_popped", "mv_status_string", "mv_model_features", "mv_model_train",
+    "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact",
 ]
 
 
@@ -110,6 +129,16 @@ def lib():
     L.mv_get_stats.argtypes = [vp, C.POINTER(mv_stats)]
     L.mv_status_string.restype = C.c_char_p
     L.mv_status_string.argtypes = [C.c_int]
+    L.mv_model_features.restype = C.c_int
+    L.mv_model_features.argtypes = [vp, sz, vp, vp, u32, u32, vp, vp]
+    L.mv_model_train.restype = C.c_int
+    L.mv_model_train.argtypes = [vp, vp, sz, C.POINTER(mv_train_config),
+                                 C.POINTER(mv_model), C.POINTER(C.c_float)]
+    L.mv_model_predict.restype = C.c_int
+    L.mv_model_predict.argtypes = [C.POINTER(mv_model), vp, sz, vp]
+    L.mv_model_loss_grad.restype = C.c_int
+    L.mv_model_loss_grad.argtypes = [C.POINTER(mv_model), vp, vp, sz, C.c_float,
+                                     C.POINTER(C.c_float), vp]
     L.mv_in_sphere_exact.restype = C.c_int
     L.mv_in_sphere_exact.argtypes = [vp, sz, vp]
     _lib = L

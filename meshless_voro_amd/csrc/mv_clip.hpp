@@ -25,6 +25,7 @@ struct ClipArgs {
   const uint64_t *kpk;   // neighbour lists, item-major
   const uint64_t *kofs;  // list offset per item (nullptr: w * L)
   const uint32_t *kcnt;
+  const double *kdist;   // distance of the first entry left out (< 0: none)
   uint32_t L;
   // lane state
   double4 *plA;   // plane: n.xyz, d
@@ -393,7 +394,12 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
       return CS_ERROR;
     }
   }
-  if (!stopped && (uint64_t)cnt < A.total_stream) return CS_NEED_MORE;
+  if (!stopped && (uint64_t)cnt < A.total_stream) {
+    // list exhausted: the stop test on the first entry left out
+    if (!(c.sr < A.kdist[w])) return CS_NEED_MORE;
+    stopped = true;
+    j = cnt;
+  }
   *npop = stopped ? j + 1 : cnt;
 
   // decomposition and integrals (convex_cell.rs:131-187,

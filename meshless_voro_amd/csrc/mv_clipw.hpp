@@ -48,6 +48,7 @@ struct WaveArgs {
   const uint64_t *kpk;    // neighbour lists
   const uint64_t *kofs;   // list offset per work item (nullptr: w * L)
   const uint32_t *kcnt;
+  const double *kdist;    // distance of the first entry left out (< 0: none)
   uint32_t L;
   unsigned int *queue;    // work-queue head
   uint8_t *wstate;
@@ -454,10 +455,14 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
       }
     }
     if (state == CS_DONE && !stopped) {
-      if ((uint64_t)cnt < A.total_stream)
-        state = CS_NEED_MORE;
-      else
+      // list exhausted: the first entry left out decides (its distance
+      // comes with the list), else the whole stream was used
+      if ((uint64_t)cnt >= A.total_stream)
         npop = cnt;
+      else if (sr < A.kdist[w])
+        npop = cnt + 1;
+      else
+        state = CS_NEED_MORE;
     }
 
     // ---- decomposition and integrals (convex_cell.rs:131-187,

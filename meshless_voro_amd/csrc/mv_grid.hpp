@@ -214,6 +214,21 @@ __device__ __forceinline__ bool entry_less(double ka, uint64_t pa, double kb,
   return ka < kb || (ka == kb && pa < pb);
 }
 
+// Distance from q to a stream entry, computed exactly as the clipper does
+// for its stop test (dx = q - (g + shift), sqrt of the dot product): the
+// first entry left out of a truncated list is the one the clipper would
+// test next, so `sr < entry_dist(...)` is the reference's stop decision
+// for it (convex_cell.rs:331-333) without holding the entry itself.
+__device__ __forceinline__ double entry_dist(const double4 *__restrict__ spos,
+                                            const double *width, const double q[3],
+                                            uint64_t pk) {
+  d3 ngb = xyz(spos[pk_slot(pk)]);
+  d3 s;
+  if (image_shift(pk_img(pk), width, &s)) ngb = add3(ngb, s);
+  const d3 dx = sub3(mk3(q[0], q[1], q[2]), ngb);
+  return sqrt(dot3(dx, dx));
+}
+
 // Gather every stream entry with key <= R^2 into (key, pk) buffers (LDS or
 // global).  Entries beyond `cap` are counted but not stored.  The count is
 // wave-uniform.
@@ -360,7 +375,8 @@ __global__ void __launch_bounds__(256)
           BoxParams box, const uint32_t *__restrict__ work, uint32_t nwork,
           uint32_t L, const uint32_t *__restrict__ Lper,
           const float *__restrict__ Rper, const uint64_t *__restrict__ kofs,
-          uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt) {
+          uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
+          double *__restrict__ kdist) {
   __shared__ KnnLds<CAP> lds[WPB];
   const uint32_t lane = __lane_id();
   const uint32_t wib = threadIdx.x >> 6;
@@ -387,8 +403,12 @@ __global__ void __launch_bounds__(256)
       for (uint32_t t = 0; t < cnt; t++)
         rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
       if (rank < outn) kpk[(kofs ? kofs[w] : (uint64_t)w * L) + rank] = pe;
+      if (rank == outn) kdist[w] = entry_dist(spos, box.width, q, pe);
     }
-    if (lane == 0) kcnt[w] = outn;
+    if (lane == 0) {
+      kcnt[w] = outn;
+      if (cnt == outn) kdist[w] = -1.;  // nothing left out: no certificate
+    }
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -402,7 +422,7 @@ __global__ void __launch_bounds__(256)
               BoxParams box, const uint32_t *__restrict__ work,
               uint32_t nwork, uint32_t L, const uint32_t *__restrict__ Lper,
               const uint64_t *__restrict__ kofs, uint64_t *__restrict__ kpk,
-              uint32_t *__restrict__ kcnt,
+              uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
               uint32_t cap) {
   const uint32_t lane = __lane_id();
@@ -449,7 +469,10 @@ __global__ void __launch_bounds__(256)
     const uint32_t outn = min(cnt, Lw);
     uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
     for (uint32_t e = lane; e < outn; e += 64) out[e] = bpk[e];
-    if (lane == 0) kcnt[w] = outn;
+    if (lane == 0) {
+      kcnt[w] = outn;
+      kdist[w] = cnt > outn ? entry_dist(spos, box.width, q, bpk[outn]) : -1.;
+    }
     __threadfence();
   }
 }
@@ -477,6 +500,57 @@ __global__ void k_flag_active(const double4 *__restrict__ spos, size_t n,
   uint32_t i = (uint32_t)__double_as_longlong(spos[s].w);
   flag[s] = (mask == nullptr || mask[i]) ? 1u : 0u;
   ident[s] = (uint32_t)s;
+}
+
+// work lists carry (slot, list length) pairs
+__global__ void k_fill_u32(uint32_t *__restrict__ a, uint32_t n, uint32_t v) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_compact_pairs(const uint32_t *__restrict__ in_w,
+                                const uint32_t *__restrict__ in_L,
+                                const uint32_t *__restrict__ flag,
+                                const uint32_t *__restrict__ pos, uint32_t n,
+                                uint32_t mult, uint32_t cap,
+                                uint32_t *__restrict__ out_w,
+                                uint32_t *__restrict__ out_L) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) {
+    out_w[pos[i]] = in_w[i];
+    uint64_t L = (uint64_t)in_L[i] * mult;
+    out_L[pos[i]] = (uint32_t)(L > cap ? cap : L);
+  }
+}
+// counting sort of a work list by list length, longest first (the
+// persistent kernel then starts the expensive cells early); bucket = L / 8
+__global__ void k_len_hist(const uint32_t *__restrict__ L, uint32_t n,
+                           uint32_t nb, uint32_t *__restrict__ hist) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&hist[nb - 1 - min(L[i] / 8, nb - 1)], 1u);
+}
+__global__ void k_len_scatter(const uint32_t *__restrict__ w,
+                              const uint32_t *__restrict__ L,
+                              const float *__restrict__ R, uint32_t n,
+                              uint32_t nb, const uint32_t *__restrict__ start,
+                              uint32_t *__restrict__ fill,
+                              uint32_t *__restrict__ ow,
+                              uint32_t *__restrict__ oL, float *__restrict__ oR) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t b = nb - 1 - min(L[i] / 8, nb - 1);
+  uint32_t p = start[b] + atomicAdd(&fill[b], 1u);
+  ow[p] = w[i];
+  oL[p] = L[i];
+  if (R) oR[p] = R[i];
+}
+// values by generator index from values by slot
+__global__ void k_unpermute(const uint32_t *__restrict__ order, size_t n,
+                            int width, const float *__restrict__ in,
+                            float *__restrict__ out) {
+  size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  size_t i = order[s];
+  for (int c = 0; c < width; c++) out[i * width + c] = in[s * width + c];
 }
 
 }  // namespace mv

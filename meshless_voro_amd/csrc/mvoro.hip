@@ -23,6 +23,7 @@
 #include "mv_common.hpp"
 #include "mv_finalize.hpp"
 #include "mv_grid.hpp"
+#include "mv_learn.hpp"
 
 using namespace mv;
 
@@ -366,23 +367,120 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start,
                 const GridParams &grid, const BoxParams &box,
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
-                uint64_t *kpk, uint32_t *kcnt, bool big, Temps &TT, hipStream_t s) {
+                uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
+                hipStream_t s) {
   if (cnt == 0) return;
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt);
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else if (!big && L <= 1024) {
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt);
+                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else {
     uint32_t waves = std::min<uint32_t>(cnt, BIG_KNN_WAVES);
     waves = (waves + 3) / 4 * 4;
     double *gkey = need(TT.get<double>((size_t)waves * BIG_KNN_CAP));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * BIG_KNN_CAP));
     hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, grid,
-                       box, wl, cnt, L, Lper, kofs, kpk, kcnt, gkey, gpk, BIG_KNN_CAP);
+                       box, wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, BIG_KNN_CAP);
   }
   HIPCHK(hipGetLastError());
+}
+
+// The counting-sort grid of a build (device buffers owned by `TT`).
+struct GridData {
+  double anchor[3], width[3];  // after the 1D/2D normalisation
+  double bb[6];
+  BoxParams box;
+  GridParams grid;
+  size_t ncell = 0;
+  double4 *gpos = nullptr, *spos = nullptr;
+  uint32_t *cell_start = nullptr, *order = nullptr, *slot_of = nullptr;
+};
+
+void build_grid(GridData &D, const double *d_xyz, size_t n,
+                const double anchor_in[3], const double width_in[3],
+                uint32_t dim, uint32_t periodic, double occ, Temps &TT,
+                hipStream_t s) {
+  // voronoi.rs:200-210
+  for (int c = 0; c < 3; c++) {
+    D.anchor[c] = anchor_in[c];
+    D.width[c] = width_in[c];
+  }
+  if (dim == 1) {
+    D.anchor[1] = -0.5;
+    D.width[1] = 1.;
+  }
+  if (dim == 1 || dim == 2) {
+    D.anchor[2] = -0.5;
+    D.width[2] = 1.;
+  }
+  D.box = make_box(D.anchor, D.width, (int)periodic, (int)dim);
+  D.gpos = need(TT.get<double4>(n));
+  const unsigned nb = nblk(n);
+  double *bbp = need(TT.get<double>(6 * (size_t)nb));
+  double *bbd = need(TT.get<double>(6));
+  hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, s, d_xyz, n, (int)dim, D.gpos, bbp);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, bbp, (int)nb, bbd);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(D.bb, bbd, sizeof D.bb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int c = 0; c < 6; c++)
+    if (!std::isfinite(D.bb[c])) throw BuildError{MV_E_DEGENERATE_POINTS};
+  D.grid = make_grid(D.bb, n, D.width, (int)dim, (int)periodic, occ);
+  D.ncell = (size_t)D.grid.G[0] * D.grid.G[1] * D.grid.G[2];
+  uint32_t *cellid = need(TT.get<uint32_t>(n));
+  uint32_t *counts = need(TT.get<uint32_t>(D.ncell + 1));
+  uint32_t *fill = need(TT.get<uint32_t>(D.ncell));
+  D.cell_start = need(TT.get<uint32_t>(D.ncell + 1));
+  D.order = need(TT.get<uint32_t>(n));
+  D.spos = need(TT.get<double4>(n));
+  D.slot_of = need(TT.get<uint32_t>(n));
+  HIPCHK(hipMemsetAsync(counts, 0, (D.ncell + 1) * sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(fill, 0, D.ncell * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, s, D.gpos, n, D.grid, cellid, counts);
+  scan_excl<uint32_t, uint32_t>(counts, D.cell_start, D.ncell + 1, s);
+  hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, s, cellid, n, D.cell_start, fill,
+                     D.order);
+  hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(D.ncell)), dim3(256), 0, s, D.cell_start,
+                     D.ncell, D.order);
+  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, D.gpos, D.order, n, D.spos,
+                     D.slot_of);
+  HIPCHK(hipGetLastError());
+}
+
+FeatParams feat_params(const GridData &D, size_t n, int dim) {
+  FeatParams fp;
+  double vol = 1.;
+  int nd = 0;
+  for (int c = 0; c < 3; c++) {
+    fp.box_lo[c] = D.anchor[c];
+    fp.box_hi[c] = D.anchor[c] + D.width[c];
+    if (c < dim) {
+      vol *= D.width[c];
+      nd++;
+    }
+  }
+  fp.h_global = std::pow(vol / (double)std::max<size_t>(n, 1), 1.0 / std::max(nd, 1));
+  return fp;
+}
+
+void upload_model(const mv_model *m, DevModel *d, hipStream_t s) {
+  DevModel h;
+  memset(&h, 0, sizeof h);
+  for (int i = 0; i < NF; i++) {
+    h.mu[i] = m->mu[i];
+    h.sigma[i] = m->sigma[i] > 0.f ? m->sigma[i] : 1.f;
+  }
+  memcpy(h.w1, m->w1, sizeof h.w1);
+  memcpy(h.w2, m->w2, sizeof h.w2);
+  for (int i = 0; i < NH; i++)
+    for (int o = 0; o < NO; o++) h.w3[i * NOP + o] = (short)m->w3[i * NO + o];
+  memcpy(h.b1, m->b1, sizeof h.b1);
+  memcpy(h.b2, m->b2, sizeof h.b2);
+  for (int o = 0; o < NO; o++) h.b3[o] = m->b3[o];
+  HIPCHK(hipMemcpyAsync(d, &h, sizeof h, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
 }
 
 void build_impl(mv_tess *T, const double *d_xyz, size_t n,
@@ -391,23 +489,9 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
                 const mv_options &opt) {
   auto t_start = std::chrono::steady_clock::now();
   hipStream_t s = 0;
-  // voronoi.rs:200-210
-  double anchor[3] = {anchor_in[0], anchor_in[1], anchor_in[2]};
-  double width[3] = {width_in[0], width_in[1], width_in[2]};
-  if (dim == 1) {
-    anchor[1] = -0.5;
-    width[1] = 1.;
-  }
-  if (dim == 1 || dim == 2) {
-    anchor[2] = -0.5;
-    width[2] = 1.;
-  }
-  memcpy(T->anchor, anchor, sizeof anchor);
-  memcpy(T->width, width, sizeof width);
   T->dim = dim;
   T->periodic = periodic;
   T->n = n;
-
   size_t nn = n ? n : 1;
   T->d_cell_loc = T->own<double>(3 * nn);
   T->d_cell_cen = T->own<double>(3 * nn);
@@ -421,6 +505,20 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   HIPCHK(hipMemsetAsync(T->d_cell_vol, 0, nn * sizeof(double), s));
   HIPCHK(hipMemsetAsync(T->d_cell_sr, 0, nn * sizeof(double), s));
   HIPCHK(hipMemsetAsync(T->d_npop, 0, nn * sizeof(uint32_t), s));
+  {
+    double a[3] = {anchor_in[0], anchor_in[1], anchor_in[2]};
+    double w[3] = {width_in[0], width_in[1], width_in[2]};
+    if (dim == 1) {
+      a[1] = -0.5;
+      w[1] = 1.;
+    }
+    if (dim == 1 || dim == 2) {
+      a[2] = -0.5;
+      w[2] = 1.;
+    }
+    memcpy(T->anchor, a, sizeof a);
+    memcpy(T->width, w, sizeof w);
+  }
   if (n == 0) {
     T->d_face_left = T->own<uint64_t>(1);
     T->d_face_right = T->own<int64_t>(1);
@@ -435,57 +533,79 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   }
 
   Temps TT;
-  // ---- prepare + bounding box
+  // ---- grid
   Timer tg(s);
-  double4 *gpos = need(TT.get<double4>(n));
-  unsigned nb = nblk(n);
-  double *bbp = need(TT.get<double>(6 * (size_t)nb));
-  double *bbd = need(TT.get<double>(6));
-  hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, s, d_xyz, n, (int)dim, gpos, bbp);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, bbp, (int)nb, bbd);
-  HIPCHK(hipGetLastError());
-  double bb[6];
-  HIPCHK(hipMemcpyAsync(bb, bbd, sizeof bb, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  for (int c = 0; c < 6; c++)
-    if (!std::isfinite(bb[c])) throw BuildError{MV_E_DEGENERATE_POINTS};
-
-  BoxParams box = make_box(anchor, width, (int)periodic, (int)dim);
-  double occ = opt.grid_occupancy16 ? (double)opt.grid_occupancy16 / 16.0 : 2.0;
-  GridParams grid = make_grid(bb, n, width, (int)dim, (int)periodic, occ);
-  size_t ncell = (size_t)grid.G[0] * grid.G[1] * grid.G[2];
-
-  // ---- counting-sort grid
-  uint32_t *cellid = need(TT.get<uint32_t>(n));
-  uint32_t *counts = need(TT.get<uint32_t>(ncell + 1));
-  uint32_t *cell_start = need(TT.get<uint32_t>(ncell + 1));
-  uint32_t *fill = need(TT.get<uint32_t>(ncell));
-  uint32_t *order = need(TT.get<uint32_t>(n));
-  double4 *spos = need(TT.get<double4>(n));
-  uint32_t *slot_of = need(TT.get<uint32_t>(n));
-  HIPCHK(hipMemsetAsync(counts, 0, (ncell + 1) * sizeof(uint32_t), s));
-  HIPCHK(hipMemsetAsync(fill, 0, ncell * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, s, gpos, n, grid, cellid, counts);
-  scan_excl<uint32_t, uint32_t>(counts, cell_start, ncell + 1, s);
-  hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, s, cellid, n, cell_start, fill, order);
-  hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(ncell)), dim3(256), 0, s, cell_start, ncell, order);
-  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, gpos, order, n, spos, slot_of);
-  HIPCHK(hipGetLastError());
+  GridData D;
+  const double occ = opt.grid_occupancy16 ? (double)opt.grid_occupancy16 / 16.0 : 2.0;
+  build_grid(D, d_xyz, n, anchor_in, width_in, dim, periodic, occ, TT, s);
+  const BoxParams &box = D.box;
+  const GridParams &grid = D.grid;
+  const double4 *spos = D.spos;
+  const uint32_t *cell_start = D.cell_start;
+  const unsigned nb = nblk(n);
 
   // ---- initial work list: active slots in grid order
   uint32_t *flag = need(TT.get<uint32_t>(n));
   uint32_t *ident = need(TT.get<uint32_t>(n));
   uint32_t *pos = need(TT.get<uint32_t>(n));
-  uint32_t *work = need(TT.get<uint32_t>(n));
-  uint32_t *work2 = need(TT.get<uint32_t>(n));
-  uint32_t *bigw = need(TT.get<uint32_t>(n));
-  uint32_t *bigw2 = need(TT.get<uint32_t>(n));
+  // [0]: first-pass list, [1]/[2]: main lists of later passes, [3]/[4]:
+  // lists of the big (global-memory) variants
+  uint32_t *lists[5], *lens[5];
+  for (int k = 0; k < 5; k++) {
+    lists[k] = need(TT.get<uint32_t>(n));
+    lens[k] = need(TT.get<uint32_t>(n));
+  }
   hipLaunchKernelGGL(k_flag_active, dim3(nb), dim3(256), 0, s, spos, n, d_mask, flag, ident);
   uint32_t nwork = scan_excl<uint32_t, uint32_t>(flag, pos, n, s);
   if (nwork)
-    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, s, ident, flag, pos, (uint32_t)n, work);
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, s, ident, flag, pos, (uint32_t)n,
+                       lists[0]);
   HIPCHK(hipGetLastError());
   T->stats.ms_grid = tg.stop();
+
+  // ---- list lengths of the first pass
+  const uint32_t kmax = std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : 32768, 32768), 2);
+  const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
+  float *Rfirst = nullptr;
+  uint32_t lbound = k0;  // upper bound of the current main list's lengths
+  double ms_model = 0;
+  if (opt.mode == MV_MODE_LEARNED && nwork) {
+    Timer tm(s);
+    float *F = need(TT.get<float>((size_t)nwork * NF));
+    float *hl = need(TT.get<float>(nwork));
+    float *Y = need(TT.get<float>((size_t)nwork * NO));
+    DevModel *dm = need(TT.get<DevModel>(1));
+    upload_model(opt.model, dm, s);
+    hipLaunchKernelGGL(k_features, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start, grid,
+                       box, feat_params(D, n, (int)dim), lists[0], nwork, F, hl);
+    const uint32_t tiles = (nwork + 15) / 16;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
+                       0, s, dm, F, nwork, Y);
+    float *R0 = need(TT.get<float>(nwork));
+    const float kscale = opt.model->kscale > 0.f ? opt.model->kscale : 1.f;
+    lbound = std::min<uint32_t>(256, kmax);
+    const float rscale = opt.model->rscale > 0.f ? opt.model->rscale : 0.f;
+    hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
+                       std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
+    // longest lists first
+    const uint32_t NB = lbound / 8 + 1;
+    uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
+    uint32_t *bst = need(TT.get<uint32_t>(NB + 1));
+    uint32_t *bfill = need(TT.get<uint32_t>(NB));
+    HIPCHK(hipMemsetAsync(hist, 0, (NB + 1) * sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(bfill, 0, NB * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_len_hist, dim3(nblk(nwork)), dim3(256), 0, s, lens[0], nwork, NB, hist);
+    scan_excl<uint32_t, uint32_t>(hist, bst, NB + 1, s);
+    Rfirst = need(TT.get<float>(nwork));
+    hipLaunchKernelGGL(k_len_scatter, dim3(nblk(nwork)), dim3(256), 0, s, lists[0], lens[0], R0,
+                       nwork, NB, bst, bfill, lists[1], lens[1], Rfirst);
+    HIPCHK(hipGetLastError());
+    std::swap(lists[0], lists[1]);
+    std::swap(lens[0], lens[1]);
+    ms_model = tm.stop();
+  } else if (nwork) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nwork)), dim3(256), 0, s, lens[0], nwork, k0);
+  }
 
   // ---- per-cell outputs and face staging
   uint32_t *cell_nf = need(TT.get<uint32_t>(n));
@@ -493,21 +613,14 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   HIPCHK(hipMemsetAsync(cell_nf, 0, n * sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cell_fbase, 0, n * sizeof(uint64_t), s));
   uint64_t st_cap = std::max<uint64_t>(4096, 16ull * nwork);
-  uint32_t *st_cell = nullptr;
-  int64_t *st_right = nullptr;
-  uint8_t *st_img = nullptr;
-  double *st_area = nullptr, *st_cen = nullptr, *st_norm = nullptr;
   unsigned long long *counters = need(TT.get<unsigned long long>(4));
   // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow
   uint8_t *wstate = need(TT.get<uint8_t>(n));
+  uint8_t *bstate = need(TT.get<uint8_t>(n));
   unsigned int *queue = need(TT.get<unsigned int>(1));
   const bool exact_order = (opt.flags & MV_FLAG_EXACT_ORDER) != 0;
-  uint8_t *bstate = need(TT.get<uint8_t>(n));
-
-  // lane state
   const uint32_t lanes_main = opt.lanes ? opt.lanes : (uint32_t)device_cus() * 512u;
   const uint32_t lanes_big = 8192;
-
   struct LaneBufs {
     double4 *plA, *plB, *vtx, *facc;
     uint32_t *vdual;
@@ -525,15 +638,19 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     L.vdual = need(TT.get<uint32_t>((size_t)maxv * lanes));
     return L;
   };
-
-  uint32_t L = opt.k0 ? opt.k0 : 27;
-  const uint32_t kmax = std::min<uint32_t>(opt.k_max ? opt.k_max : 32768, 32768);
+  LaneBufs LM{}, LB{};
+  bool have_main = false, have_big = false;
   uint64_t redone = 0, bigcells = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
+  const uint32_t nwork0 = nwork;
 
+  uint32_t *st_cell = nullptr;
+  int64_t *st_right = nullptr;
+  uint8_t *st_img = nullptr;
+  double *st_area = nullptr, *st_cen = nullptr, *st_norm = nullptr;
+  const uint32_t lmin0 = opt.mode == MV_MODE_LEARNED ? std::min<uint32_t>(16, kmax) : k0;
   for (int attempt = 0;; attempt++) {
-    // (re)allocate staging
     st_cell = need(TT.get<uint32_t>(st_cap));
     st_right = need(TT.get<int64_t>(st_cap));
     st_img = need(TT.get<uint8_t>(st_cap));
@@ -544,23 +661,31 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     unsigned long long err_init = ~0ull;
     HIPCHK(hipMemcpyAsync(counters + 1, &err_init, sizeof err_init, hipMemcpyHostToDevice, s));
 
-    uint32_t nw = nwork, nbig = 0;
-    uint32_t *cw = work, *cw2 = work2, *cb = bigw, *cb2 = bigw2;
-    uint32_t Lp = L;
-    LaneBufs LM{}, LB{};
-    bool have_main = false, have_big = false;
+    // the first pass reads lists[0], which is never written, so that an
+    // attempt can start over; later passes alternate between two buffers
+    uint32_t nw = nwork0, nbig = 0;
+    uint32_t *cw = lists[0], *cl = lens[0], *cb = lists[3], *cbl = lens[3];
+    int ping_m = 1, ping_b = 4;
+    // bounds on the list lengths of the main and big lists (all clamped to
+    // kmax; a list whose lower bound is kmax cannot grow any more)
+    uint32_t lb_main = std::min(lbound, kmax), lb_big = lb_main;
+    uint32_t lmin_main = std::min(lmin0, kmax), lmin_big = kmax;
+    const float *Rmain = Rfirst;
     bool overflowed = false;
     passes = 0;
     redone = 0;
     bigcells = 0;
     while (nw + nbig > 0) {
-      if (Lp > kmax) throw BuildError{MV_E_CAPACITY};
       passes++;
-      auto run = [&](uint32_t *wl, uint32_t cnt, uint8_t *state, bool big) {
+      auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
+                     uint8_t *state, bool big) {
         Timer tk(s);
-        uint64_t *kpk = need(TT.get<uint64_t>((size_t)Lp * cnt));
+        uint64_t *kofs = need(TT.get<uint64_t>(cnt));
+        const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
+        uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
-        launch_knn(spos, cell_start, grid, box, wl, cnt, Lp, nullptr, nullptr, nullptr, kpk, kcnt,
+        double *kdist = need(TT.get<double>(cnt));
+        launch_knn(spos, cell_start, grid, box, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist,
                    big, TT, s);
         ms_knn += tk.stop();
         Timer tc(s);
@@ -574,9 +699,10 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
           W.work = wl;
           W.nwork = cnt;
           W.kpk = kpk;
-          W.kofs = nullptr;
+          W.kofs = kofs;
           W.kcnt = kcnt;
-          W.L = Lp;
+          W.kdist = kdist;
+          W.L = 0;
           W.queue = queue;
           W.wstate = state;
           W.cell_vol = T->d_cell_vol;
@@ -612,9 +738,10 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         A.work = wl;
         A.nwork = cnt;
         A.kpk = kpk;
-        A.kofs = nullptr;
+        A.kofs = kofs;
         A.kcnt = kcnt;
-        A.L = Lp;
+        A.kdist = kdist;
+        A.L = 0;
         uint32_t lanes;
         if (!big) {
           lanes = std::min<uint32_t>(lanes_main, (cnt + 255) / 256 * 256);
@@ -660,8 +787,9 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         HIPCHK(hipGetLastError());
         ms_clip += tc.stop();
       };
-      if (nw) run(cw, nw, wstate, false);
-      if (nbig) run(cb, nbig, bstate, true);
+      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false);
+      if (nbig) run(cb, cbl, nbig, lb_big, nullptr, bstate, true);
+      Rmain = nullptr;
       unsigned long long cnts[4];
       HIPCHK(hipMemcpyAsync(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -676,25 +804,55 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         overflowed = true;
         break;
       }
-      // next lists: main NEED_MORE; big NEED_MORE + main CAPACITY
-      uint32_t nw_next = compact(cw, wstate, nw, CS_NEED_MORE, cw2, s);
-      uint32_t nb_next = compact(cb, bstate, nbig, CS_NEED_MORE, cb2, s);
-      uint32_t ncap = compact(cw, wstate, nw, CS_CAPACITY, cb2 + nb_next, s);
-      uint32_t nbcap = compact(cb, bstate, nbig, CS_CAPACITY, cw2 + nw_next, s);
+      // next lists: the cells whose security radius was not reached get a
+      // doubled list; cells over the on-chip capacity move to the big list
+      auto take = [&](const uint32_t *w, const uint32_t *L, const uint8_t *st, uint32_t cnt,
+                      uint8_t want, uint32_t mult, uint32_t *ow, uint32_t *oL) -> uint32_t {
+        if (cnt == 0) return 0;
+        Temps T2;
+        uint32_t *fl = need(T2.get<uint32_t>(cnt));
+        uint32_t *ps = need(T2.get<uint32_t>(cnt));
+        hipLaunchKernelGGL(k_flag_state, dim3(nblk(cnt)), dim3(256), 0, s, st, cnt, want, fl);
+        uint32_t c = scan_excl<uint32_t, uint32_t>(fl, ps, cnt, s);
+        if (c)
+          hipLaunchKernelGGL(k_compact_pairs, dim3(nblk(cnt)), dim3(256), 0, s, w, L, fl, ps,
+                             cnt, mult, kmax, ow, oL);
+        HIPCHK(hipGetLastError());
+        return c;
+      };
+      uint32_t *cw2 = lists[ping_m], *cl2 = lens[ping_m];
+      uint32_t *cb2 = lists[ping_b], *cbl2 = lens[ping_b];
+      const uint32_t nw_next = take(cw, cl, wstate, nw, CS_NEED_MORE, 2, cw2, cl2);
+      const uint32_t nb_next = take(cb, cbl, bstate, nbig, CS_NEED_MORE, 2, cb2, cbl2);
+      const uint32_t ncap = take(cw, cl, wstate, nw, CS_CAPACITY, 1, cb2 + nb_next, cbl2 + nb_next);
+      const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, 1, cw2 + nw_next,
+                                  cl2 + nw_next);
       if (nbcap) throw BuildError{MV_E_CAPACITY};
+      // a cell that reached kmax entries without its security radius
+      if ((nw_next && lmin_main >= kmax) || (nb_next && lmin_big >= kmax))
+        throw BuildError{MV_E_CAPACITY};
       bigcells += ncap;
       redone += nw_next + nb_next;
+      const uint32_t lb_main_old = lb_main, lmin_main_old = lmin_main;
+      lb_main = std::min(2 * lb_main, kmax);
+      lmin_main = std::min(2 * lmin_main, kmax);
+      lb_big = std::min(std::max(2 * lb_big, lb_main_old), kmax);
+      lmin_big = std::min(nb_next ? std::min(2 * lmin_big, kmax) : kmax,
+                          ncap ? lmin_main_old : kmax);
       nw = nw_next;
       nbig = nb_next + ncap;
-      std::swap(cw, cw2);
-      std::swap(cb, cb2);
-      if (nw + nbig) Lp *= 2;
+      cw = cw2;
+      cl = cl2;
+      cb = cb2;
+      cbl = cbl2;
+      ping_m = 3 - ping_m;
+      ping_b = 7 - ping_b;
     }
     if (!overflowed) {
       unsigned long long cnts[4];
       HIPCHK(hipMemcpy(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost));
       T->stats.exact_tests = cnts[2];
-      T->stats.k_last = Lp;
+      T->stats.k_last = lb_main;
       break;
     }
     if (attempt > 4) throw BuildError{MV_E_CAPACITY};
@@ -704,7 +862,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   T->stats.passes = passes;
   T->stats.cells_redone = redone;
   T->stats.big_cells = bigcells;
-  T->stats.ms_knn = ms_knn;
+  T->stats.ms_knn = ms_knn + ms_model;
   T->stats.ms_clip = ms_clip;
 
   // ---- faces and links
@@ -733,7 +891,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     FA.n_staged = F;
     FA.cell_fbase = cell_fbase;
     FA.face_off = face_off;
-    for (int c = 0; c < 3; c++) FA.width[c] = width[c];
+    for (int c = 0; c < 3; c++) FA.width[c] = D.width[c];
     FA.face_left = T->d_face_left;
     FA.face_right = T->d_face_right;
     FA.face_area = T->d_face_area;
@@ -758,7 +916,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff,
                        T->d_cell_fcnt, n, T->d_links);
   }
-  hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, gpos, n, d_mask,
+  hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, D.gpos, n, d_mask,
                      T->d_cell_loc, T->d_cell_idx);
   HIPCHK(hipGetLastError());
   T->stats.ms_finalize = tf.stop();
@@ -782,7 +940,10 @@ mv_status run_build(const double *d_xyz, size_t n, const uint8_t *d_mask,
   mv_options o;
   mv_default_options(&o);
   if (opt) o = *opt;
-  if (o.mode != MV_MODE_PLAIN) return MV_E_ARG;
+  if (o.mode != MV_MODE_PLAIN && o.mode != MV_MODE_LEARNED) return MV_E_ARG;
+  if (o.mode == MV_MODE_LEARNED &&
+      (!o.model || o.model->magic != MV_MODEL_MAGIC || o.model->version != 1))
+    return MV_E_ARG;
   mv_tess *T = new (std::nothrow) mv_tess();
   if (!T) return MV_E_OOM;
   try {
@@ -794,6 +955,147 @@ mv_status run_build(const double *d_xyz, size_t n, const uint8_t *d_mask,
     return e.st;
   }
   *out = T;
+  return MV_OK;
+}
+
+// ---------------------------------------------------------------------
+// learned proposal: host side of the training and inference kernels
+// ---------------------------------------------------------------------
+// FP32 parameter block of the trainer: w1 16x64 | b1 | w2 64x64 | b2 |
+// w3 64x16 (2 used) | b3 16 (2 used)
+constexpr int P_W1 = 0, P_B1 = P_W1 + NF * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH,
+              P_W3 = P_B2 + NH, P_B3 = P_W3 + NH * NOP, P_TOTAL = P_B3 + NOP;
+
+uint16_t host_f2bf(float x) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+float host_bf2f(uint16_t h) {
+  uint32_t b = (uint32_t)h << 16;
+  float x;
+  memcpy(&x, &b, 4);
+  return x;
+}
+
+void model_to_params(const mv_model &m, std::vector<float> &P) {
+  P.assign(P_TOTAL, 0.f);
+  for (int i = 0; i < NF * NH; i++) P[P_W1 + i] = host_bf2f(m.w1[i]);
+  for (int i = 0; i < NH * NH; i++) P[P_W2 + i] = host_bf2f(m.w2[i]);
+  for (int i = 0; i < NH; i++)
+    for (int o = 0; o < NO; o++) P[P_W3 + i * NOP + o] = host_bf2f(m.w3[i * NO + o]);
+  for (int i = 0; i < NH; i++) {
+    P[P_B1 + i] = m.b1[i];
+    P[P_B2 + i] = m.b2[i];
+  }
+  for (int o = 0; o < NO; o++) P[P_B3 + o] = m.b3[o];
+}
+
+void params_to_model(const std::vector<float> &P, mv_model &m) {
+  for (int i = 0; i < NF * NH; i++) m.w1[i] = host_f2bf(P[P_W1 + i]);
+  for (int i = 0; i < NH * NH; i++) m.w2[i] = host_f2bf(P[P_W2 + i]);
+  for (int i = 0; i < NH; i++)
+    for (int o = 0; o < NO; o++) m.w3[i * NO + o] = host_f2bf(P[P_W3 + i * NOP + o]);
+  for (int i = 0; i < NH; i++) {
+    m.b1[i] = P[P_B1 + i];
+    m.b2[i] = P[P_B2 + i];
+  }
+  for (int o = 0; o < NO; o++) m.b3[o] = P[P_B3 + o];
+}
+
+// Device buffers of one forward + backward pass over `B` rows.
+struct TrainBufs {
+  int B;
+  float *X, *Tg, *Z1, *A1, *Z2, *A2, *Y, *dY, *dA2, *dA1, *lrow;
+};
+
+TrainBufs train_bufs(int B, Temps &TT) {
+  TrainBufs t;
+  t.B = B;
+  t.X = need(TT.get<float>((size_t)B * NF));
+  t.Tg = need(TT.get<float>((size_t)B * NO));
+  t.Z1 = need(TT.get<float>((size_t)B * NH));
+  t.A1 = need(TT.get<float>((size_t)B * NH));
+  t.Z2 = need(TT.get<float>((size_t)B * NH));
+  t.A2 = need(TT.get<float>((size_t)B * NH));
+  t.Y = need(TT.get<float>((size_t)B * NOP));
+  t.dY = need(TT.get<float>((size_t)B * NOP));
+  t.dA2 = need(TT.get<float>((size_t)B * NH));
+  t.dA1 = need(TT.get<float>((size_t)B * NH));
+  t.lrow = need(TT.get<float>((size_t)B));
+  return t;
+}
+
+void gemm(const float *A, long sa0, long sa1, const float *Bm, long sb0, long sb1, float *C,
+          int ldc, int M, int N, int K, hipStream_t s) {
+  const int tiles = (M / 16) * (N / 16);
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((tiles + 3) / 4), dim3(256), 0, s, A, sa0, sa1, Bm, sb0,
+                     sb1, C, ldc, M, N, K, 0);
+}
+
+// forward and backward of the pinball loss over the rows in t.X / t.Tg;
+// P: parameters, G: gradient (same layout)
+void forward_backward(const TrainBufs &t, const float *P, float *G, float tau, hipStream_t s) {
+  const int B = t.B;
+  auto g1 = [](long n) { return dim3((unsigned)((n + 255) / 256)); };
+  gemm(t.X, NF, 1, P + P_W1, NH, 1, t.Z1, NH, B, NH, NF, s);
+  hipLaunchKernelGGL(k_bias_relu, g1((long)B * NH), dim3(256), 0, s, t.Z1, t.A1, P + P_B1, B,
+                     NH, 1);
+  gemm(t.A1, NH, 1, P + P_W2, NH, 1, t.Z2, NH, B, NH, NH, s);
+  hipLaunchKernelGGL(k_bias_relu, g1((long)B * NH), dim3(256), 0, s, t.Z2, t.A2, P + P_B2, B,
+                     NH, 1);
+  gemm(t.A2, NH, 1, P + P_W3, NOP, 1, t.Y, NOP, B, NOP, NH, s);
+  hipLaunchKernelGGL(k_bias_relu, g1((long)B * NOP), dim3(256), 0, s, t.Y, (float *)nullptr,
+                     P + P_B3, B, NOP, 0);
+  hipLaunchKernelGGL(k_pinball, g1(B), dim3(256), 0, s, t.Y, t.Tg, B, tau, t.dY, t.lrow);
+  // layer 3
+  gemm(t.A2, 1, NH, t.dY, NOP, 1, G + P_W3, NOP, NH, NOP, B, s);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, s, t.dY, B, NOP, G + P_B3);
+  gemm(t.dY, NOP, 1, P + P_W3, 1, NOP, t.dA2, NH, B, NH, NOP, s);
+  hipLaunchKernelGGL(k_relu_grad, g1((long)B * NH), dim3(256), 0, s, t.Z2, t.dA2, B * NH);
+  // layer 2
+  gemm(t.A1, 1, NH, t.dA2, NH, 1, G + P_W2, NH, NH, NH, B, s);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, s, t.dA2, B, NH, G + P_B2);
+  gemm(t.dA2, NH, 1, P + P_W2, 1, NH, t.dA1, NH, B, NH, NH, s);
+  hipLaunchKernelGGL(k_relu_grad, g1((long)B * NH), dim3(256), 0, s, t.Z1, t.dA1, B * NH);
+  // layer 1
+  gemm(t.X, 1, NF, t.dA1, NH, 1, G + P_W1, NH, NF, NH, B, s);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, s, t.dA1, B, NH, G + P_B1);
+  HIPCHK(hipGetLastError());
+}
+
+uint64_t splitmix(uint64_t &st) {
+  uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+double unif(uint64_t &st) { return ((splitmix(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+double gauss(uint64_t &st) {
+  const double u = unif(st), v = unif(st);
+  return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+}
+
+mv_status device_ready() {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return MV_E_DEVICE;
+  }
+  return MV_OK;
+}
+
+template <typename F>
+mv_status guarded(F &&f) {
+  try {
+    f();
+  } catch (const BuildError &e) {
+    (void)hipDeviceSynchronize();
+    (void)hipGetLastError();
+    return e.st;
+  }
   return MV_OK;
 }
 
@@ -956,6 +1258,222 @@ const char *mv_status_string(mv_status s) {
     case MV_E_CAPACITY: return "per-cell capacity exceeded";
   }
   return "unknown status";
+}
+
+mv_status mv_model_features(const double *xyz, size_t n, const double anchor[3],
+                            const double width[3], uint32_t dimensionality,
+                            uint32_t periodic, float *features, float *hloc) {
+  if (!valid_args(n, dimensionality, anchor, width) || (n && (!xyz || !features || !hloc)))
+    return MV_E_ARG;
+  if (n == 0) return MV_OK;
+  if (mv_status st = device_ready()) return st;
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    double *d_xyz = need(TT.get<double>(3 * n));
+    HIPCHK(hipMemcpyAsync(d_xyz, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    GridData D;
+    build_grid(D, d_xyz, n, anchor, width, dimensionality, periodic ? 1u : 0u, 2.0, TT, s);
+    uint32_t *flag = need(TT.get<uint32_t>(n));
+    uint32_t *ident = need(TT.get<uint32_t>(n));
+    hipLaunchKernelGGL(k_flag_active, dim3(nblk(n)), dim3(256), 0, s, D.spos, n,
+                       (const uint8_t *)nullptr, flag, ident);
+    float *F = need(TT.get<float>(n * NF));
+    float *hl = need(TT.get<float>(n));
+    float *Fo = need(TT.get<float>(n * NF));
+    float *ho = need(TT.get<float>(n));
+    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start, D.grid,
+                       D.box, feat_params(D, n, (int)dimensionality), ident, (uint32_t)n, F, hl);
+    hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, NF, F, Fo);
+    hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, 1, hl, ho);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(features, Fo, n * NF * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hloc, ho, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+mv_status mv_model_predict(const mv_model *model, const float *features, size_t n,
+                           float *out) {
+  if (!model || model->magic != MV_MODEL_MAGIC || model->version != 1) return MV_E_ARG;
+  if (n && (!features || !out)) return MV_E_ARG;
+  if (n == 0) return MV_OK;
+  if (n >= (1ull << 31)) return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    DevModel *dm = need(TT.get<DevModel>(1));
+    upload_model(model, dm, s);
+    float *F = need(TT.get<float>(n * NF));
+    float *Y = need(TT.get<float>(n * NO));
+    HIPCHK(hipMemcpyAsync(F, features, n * NF * sizeof(float), hipMemcpyHostToDevice, s));
+    const uint32_t tiles = (uint32_t)((n + 15) / 16);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
+                       0, s, dm, F, (uint32_t)n, Y);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, Y, n * NO * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+mv_status mv_model_loss_grad(const mv_model *model, const float *features,
+                             const float *targets, size_t n, float tau, float *loss,
+                             float *grad) {
+  if (!model || model->magic != MV_MODEL_MAGIC || model->version != 1) return MV_E_ARG;
+  if (n == 0 || n % 16 != 0 || n > (1u << 24) || !features || !targets || !loss || !grad)
+    return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    const int B = (int)n;
+    std::vector<float> P;
+    model_to_params(*model, P);
+    float *dP = need(TT.get<float>(P_TOTAL));
+    float *dG = need(TT.get<float>(P_TOTAL));
+    HIPCHK(hipMemcpyAsync(dP, P.data(), P_TOTAL * sizeof(float), hipMemcpyHostToDevice, s));
+    TrainBufs t = train_bufs(B, TT);
+    // normalised inputs, as the inference kernel forms them
+    std::vector<float> X((size_t)B * NF);
+    for (int r = 0; r < B; r++)
+      for (int c = 0; c < NF; c++) {
+        const float sg = model->sigma[c] > 0.f ? model->sigma[c] : 1.f;
+        X[(size_t)r * NF + c] = (features[(size_t)r * NF + c] - model->mu[c]) / sg;
+      }
+    HIPCHK(hipMemcpyAsync(t.X, X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(t.Tg, targets, (size_t)B * NO * sizeof(float), hipMemcpyHostToDevice,
+                          s));
+    forward_backward(t, dP, dG, tau, s);
+    std::vector<float> G(P_TOTAL), lr(B);
+    HIPCHK(hipMemcpyAsync(G.data(), dG, P_TOTAL * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(lr.data(), t.lrow, B * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double L = 0;
+    for (float v : lr) L += v;
+    *loss = (float)(L / B);
+    float *g = grad;
+    for (int i = 0; i < NF * NH; i++) *g++ = G[P_W1 + i];
+    for (int i = 0; i < NH; i++) *g++ = G[P_B1 + i];
+    for (int i = 0; i < NH * NH; i++) *g++ = G[P_W2 + i];
+    for (int i = 0; i < NH; i++) *g++ = G[P_B2 + i];
+    for (int i = 0; i < NH; i++)
+      for (int o = 0; o < NO; o++) *g++ = G[P_W3 + i * NOP + o];
+    for (int o = 0; o < NO; o++) *g++ = G[P_B3 + o];
+  });
+}
+
+mv_status mv_model_train(const float *features, const float *targets, size_t n,
+                         const mv_train_config *cfg, mv_model *model, float *final_loss) {
+  if (!features || !targets || !cfg || !model || n < 16 || n >= (1ull << 31)) return MV_E_ARG;
+  if (cfg->batch == 0 || cfg->batch % 16 != 0 || cfg->batch > (1u << 20) || cfg->steps == 0 ||
+      !(cfg->lr > 0.f) || !(cfg->tau > 0.f && cfg->tau < 1.f))
+    return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    // normalisation constants
+    mv_model m;
+    memset(&m, 0, sizeof m);
+    m.magic = MV_MODEL_MAGIC;
+    m.version = 1;
+    m.tau = cfg->tau;
+    m.kscale = 1.f;
+    m.rscale = 0.f;
+    for (int c = 0; c < NF; c++) {
+      double a = 0, b = 0;
+      for (size_t r = 0; r < n; r++) {
+        const double v = features[r * NF + c];
+        a += v;
+        b += v * v;
+      }
+      const double mu = a / n, var = std::max(b / n - mu * mu, 0.0);
+      m.mu[c] = (float)mu;
+      m.sigma[c] = var > 1e-12 ? (float)std::sqrt(var) : 1.f;
+    }
+    // He initialisation; output bias at the target quantile
+    uint64_t rs = 0x6d76ull ^ ((uint64_t)cfg->seed << 20);
+    std::vector<float> P(P_TOTAL, 0.f);
+    for (int i = 0; i < NF * NH; i++) P[P_W1 + i] = (float)(gauss(rs) * std::sqrt(2.0 / NF));
+    for (int i = 0; i < NH * NH; i++) P[P_W2 + i] = (float)(gauss(rs) * std::sqrt(2.0 / NH));
+    for (int i = 0; i < NH; i++)
+      for (int o = 0; o < NO; o++)
+        P[P_W3 + i * NOP + o] = (float)(gauss(rs) * std::sqrt(1.0 / NH) * 0.1);
+    for (int o = 0; o < NO; o++) {
+      std::vector<float> col(n);
+      for (size_t r = 0; r < n; r++) col[r] = targets[r * NO + o];
+      const size_t q = std::min(n - 1, (size_t)(cfg->tau * (double)n));
+      std::nth_element(col.begin(), col.begin() + q, col.end());
+      P[P_B3 + o] = col[q];
+    }
+    float *dF = need(TT.get<float>(n * NF));
+    float *dT = need(TT.get<float>(n * NO));
+    float *dmu = need(TT.get<float>(NF));
+    float *dsg = need(TT.get<float>(NF));
+    float *dP = need(TT.get<float>(P_TOTAL));
+    float *dG = need(TT.get<float>(P_TOTAL));
+    float *dm1 = need(TT.get<float>(P_TOTAL));
+    float *dm2 = need(TT.get<float>(P_TOTAL));
+    HIPCHK(hipMemcpyAsync(dF, features, n * NF * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dT, targets, n * NO * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dmu, m.mu, sizeof m.mu, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dsg, m.sigma, sizeof m.sigma, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dP, P.data(), P_TOTAL * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(dm1, 0, P_TOTAL * sizeof(float), s));
+    HIPCHK(hipMemsetAsync(dm2, 0, P_TOTAL * sizeof(float), s));
+    const int B = (int)cfg->batch;
+    TrainBufs t = train_bufs(B, TT);
+    // the row indices of every step, drawn up front
+    const uint32_t chunk = 256;
+    std::vector<uint32_t> idx((size_t)B * chunk);
+    uint32_t *didx = need(TT.get<uint32_t>((size_t)B * chunk));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    double c1 = 1, c2 = 1;
+    for (uint32_t step = 0; step < cfg->steps; step++) {
+      const uint32_t j = step % chunk;
+      if (j == 0) {
+        HIPCHK(hipStreamSynchronize(s));
+        for (auto &v : idx) v = (uint32_t)(splitmix(rs) % n);
+        HIPCHK(hipMemcpyAsync(didx, idx.data(), idx.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+      }
+      hipLaunchKernelGGL(k_gather_rows, dim3((B * NF + 255) / 256), dim3(256), 0, s, dF, dT,
+                         didx + (size_t)j * B, B, dmu, dsg, t.X, t.Tg);
+      forward_backward(t, dP, dG, cfg->tau, s);
+      c1 *= b1;
+      c2 *= b2;
+      // cosine decay of the step size over the run
+      const float lr = (float)(cfg->lr * 0.5 *
+                               (1.0 + std::cos(3.141592653589793 * step / cfg->steps)));
+      hipLaunchKernelGGL(k_adam, dim3((P_TOTAL + 255) / 256), dim3(256), 0, s, dP, dG, dm1, dm2,
+                         P_TOTAL, lr, b1, b2, eps, (float)(1 - c1), (float)(1 - c2));
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(P.data(), dP, P_TOTAL * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    params_to_model(P, m);
+    *model = m;
+    if (final_loss) {
+      // loss of the BF16 checkpoint over every row, through the inference kernel
+      DevModel *dmod = need(TT.get<DevModel>(1));
+      upload_model(&m, dmod, s);
+      float *Y = need(TT.get<float>(n * NO));
+      const uint32_t tiles = (uint32_t)((n + 15) / 16);
+      hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)),
+                         dim3(256), 0, s, dmod, dF, (uint32_t)n, Y);
+      HIPCHK(hipGetLastError());
+      std::vector<float> Yh(n * NO);
+      HIPCHK(hipMemcpyAsync(Yh.data(), Y, n * NO * sizeof(float), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      double L = 0;
+      for (size_t i = 0; i < n * NO; i++) {
+        const double e = (double)targets[i] - Yh[i];
+        L += e > 0 ? cfg->tau * e : (cfg->tau - 1.0) * e;
+      }
+      *final_loss = (float)(L / n);
+    }
+  });
 }
 
 mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,

@@ -56,10 +56,14 @@ class BuildOptions:
     """mv_options: construction mode and neighbour-list parameters."""
 
     def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 32768,
-                 lanes: int = 0, occupancy16: int = 0, exact_order: bool = False):
+                 lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
+                 model=None):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
         self.exact_order = exact_order
+        # learned mode: a meshless_voro_amd.learned.Model (None: the
+        # committed checkpoint)
+        self.model = model
 
     def to_c(self) -> _native.mv_options:
         o = _native.mv_options()
@@ -67,6 +71,11 @@ class BuildOptions:
         o.mode, o.k0, o.k_max, o.lanes = self.mode, self.k0, self.k_max, self.lanes
         o.grid_occupancy16 = self.occupancy16
         o.flags = 1 if self.exact_order else 0
+        if self.mode == MODE_LEARNED:
+            from . import learned
+            m = self.model if self.model is not None else learned.default_model()
+            o.model = C.addressof(m.raw)
+            self._keep = m  # the struct must outlive the call
         return o
 
 

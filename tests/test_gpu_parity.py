@@ -74,6 +74,32 @@ def test_grid_jitter_16():
 
 
 @pytest.mark.parametrize("periodic", [False, True])
+def test_perfect_grid_64(periodic):
+    """262,144-seed perfect grid (exact dyadic positions): every cell is a
+    cube of side 1/64 decided through the exact predicate."""
+    x = W.grid(64)
+    g, o = both(x, periodic=periodic)
+    check(g, o)
+    assert np.allclose(g.cell_volume, 64.0 ** -3, rtol=1e-12)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_grid_jitter_64_fails_like_the_oracle(periodic):
+    """1% of a 64^3 perfect grid moved by 1e-12: the filtered and the exact
+    decisions disagree about near-co-spherical vertices and the boundary
+    walk of compute_boundary (convex_cell.rs:421-441) finds no extension;
+    the reference panics there and the oracle reports it.  The GPU path
+    must report the same error, not a different tessellation."""
+    x = W.grid_jitter(64)
+    print(f"case grid_jitter_64 periodic={periodic}", flush=True)
+    with pytest.raises(O.OracleError) as eo:
+        O.build(x, ZERO, ONE, 3, periodic)
+    with pytest.raises(VoronoiError) as eg:
+        Voronoi.build(x, ZERO, ONE, 3, periodic)
+    assert eo.value.status == eg.value.status == 6
+
+
+@pytest.mark.parametrize("periodic", [False, True])
 def test_partial(periodic):
     x = W.uniform(5000, 7)
     m = W.partial_mask(5000, 0.1, 8)
