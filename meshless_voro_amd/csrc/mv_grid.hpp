@@ -232,6 +232,16 @@ __device__ __forceinline__ double entry_dist(const double4 *__restrict__ spos,
 // Gather every stream entry with key <= R^2 into (key, pk) buffers (LDS or
 // global).  Entries beyond `cap` are counted but not stored.  The count is
 // wave-uniform.
+//
+// Latency-parallel walk: for each periodic image the grid rows (a, b) that
+// reach the ball are spread over the lanes (one row per lane, 64 at a time);
+// each lane trims its row's c-range to the ball's chord over that column
+// and reads the row's slot range with two loads, all rows at once.  The
+// candidates of all rows are then numbered consecutively (wave prefix sum)
+// and read 64 at a time, each lane finding its row by a binary search over
+// the rows' offsets held in the other lanes' registers.  A gather thus waits
+// on one dependent load per 64 rows plus one per 64 candidates, instead of
+// two per row.
 __device__ __forceinline__ uint32_t knn_gather(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
     const GridParams &g, const double *width, const double q[3], double R,
@@ -239,6 +249,7 @@ __device__ __forceinline__ uint32_t knn_gather(
   const uint32_t lane = __lane_id();
   const double R2 = R * R;
   const double rr = R + g.margin;
+  const double rr2 = rr * rr;
   uint32_t cnt = 0;
   for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++)
     for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++)
@@ -265,33 +276,76 @@ __device__ __forceinline__ uint32_t knn_gather(
         }
         if (skip) continue;
         const uint32_t img = (uint32_t)((i + 1) * 9 + (j + 1) * 3 + (k + 1));
-        for (int a = lo[0]; a <= hi[0]; a++)
-          for (int b = lo[1]; b <= hi[1]; b++) {
-            size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
-            uint32_t t0 = cell_start[row + lo[2]];
-            uint32_t t1 = cell_start[row + hi[2] + 1];
-            for (uint32_t base = t0; base < t1; base += 64) {
-              uint32_t t = base + lane;
-              bool keep = false;
-              double key = 0.;
-              uint64_t pk = 0;
-              if (t < t1) {
-                double4 gp = spos[t];
-                double d0 = qs[0] - gp.x, d1 = qs[1] - gp.y, d2 = qs[2] - gp.z;
-                key = (d0 * d0 + d1 * d1) + d2 * d2;
-                keep = key <= R2;
-                uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
-                pk = (gidx << 32) | ((uint64_t)t << 5) | img;
+        const int nb_ = hi[1] - lo[1] + 1;
+        const int nrows = (hi[0] - lo[0] + 1) * nb_;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+          const int r = r0 + (int)lane;
+          uint32_t t0 = 0, len = 0;
+          if (r < nrows) {
+            const int a = lo[0] + r / nb_, b = lo[1] + r % nb_;
+            // squared distance from qs to the column of cells (a, b)
+            double d2 = 0.;
+            if (g.inv_h[0] != 0.) {
+              const double x0 = g.lo[0] + (double)a * g.h[0], x1 = x0 + g.h[0];
+              const double dx = fmax(0., fmax(x0 - qs[0], qs[0] - x1));
+              d2 += dx * dx;
+            }
+            if (g.inv_h[1] != 0.) {
+              const double y0 = g.lo[1] + (double)b * g.h[1], y1 = y0 + g.h[1];
+              const double dy = fmax(0., fmax(y0 - qs[1], qs[1] - y1));
+              d2 += dy * dy;
+            }
+            if (d2 <= rr2) {
+              int c0 = lo[2], c1 = hi[2];
+              if (g.inv_h[2] != 0.) {
+                const double hc = sqrt(rr2 - d2);
+                const double za = floor((qs[2] - hc - g.lo[2]) * g.inv_h[2]);
+                const double zb = floor((qs[2] + hc - g.lo[2]) * g.inv_h[2]);
+                c0 = max(c0, za < 0. ? 0 : (int)fmin(za, (double)g.G[2]));
+                c1 = min(c1, zb < 0. ? -1 : (int)fmin(zb, (double)g.G[2]));
               }
-              uint64_t m = __ballot(keep);
-              uint32_t pos = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-              if (keep && pos < cap) {
-                bkey[pos] = key;
-                bpk[pos] = pk;
+              if (c0 <= c1) {
+                const size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
+                t0 = cell_start[row + c0];
+                len = cell_start[row + c1 + 1] - t0;
               }
-              cnt += (uint32_t)__popcll(m);
             }
           }
+          uint32_t total;
+          const uint32_t off = wave_excl_scan(len, &total);
+          for (uint32_t cb = 0; cb < total; cb += 64) {
+            const uint32_t gi = cb + lane;
+            // largest row r with off[r] <= gi (rows past the end hold
+            // off = total, empty rows share the next row's offset)
+            uint32_t rsel = 0;
+#pragma unroll
+            for (uint32_t st = 32; st > 0; st >>= 1) {
+              const uint32_t o = (uint32_t)__shfl((int)off, (int)(rsel + st), 64);
+              if (rsel + st < 64 && o <= gi) rsel += st;
+            }
+            const uint32_t orow = (uint32_t)__shfl((int)off, (int)rsel, 64);
+            const uint32_t trow = (uint32_t)__shfl((int)t0, (int)rsel, 64);
+            bool keep = false;
+            double key = 0.;
+            uint64_t pk = 0;
+            if (gi < total) {
+              const uint32_t t = trow + (gi - orow);
+              const double4 gp = spos[t];
+              const double d0 = qs[0] - gp.x, d1 = qs[1] - gp.y, dz = qs[2] - gp.z;
+              key = (d0 * d0 + d1 * d1) + dz * dz;
+              keep = key <= R2;
+              const uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
+              pk = (gidx << 32) | ((uint64_t)t << 5) | img;
+            }
+            const uint64_t m = __ballot(keep);
+            const uint32_t pos = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (keep && pos < cap) {
+              bkey[pos] = key;
+              bpk[pos] = pk;
+            }
+            cnt += (uint32_t)__popcll(m);
+          }
+        }
       }
   return cnt;
 }
@@ -522,26 +576,60 @@ __global__ void k_compact_pairs(const uint32_t *__restrict__ in_w,
   }
 }
 // counting sort of a work list by list length, longest first (the
-// persistent kernel then starts the expensive cells early); bucket = L / 8
-__global__ void k_len_hist(const uint32_t *__restrict__ L, uint32_t n,
-                           uint32_t nb, uint32_t *__restrict__ hist) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&hist[nb - 1 - min(L[i] / 8, nb - 1)], 1u);
+// persistent kernel then starts the expensive cells early); bucket = L / 8.
+// Each block counts LEN_PER_BLOCK items in LDS and touches the global
+// counters once per bucket, so the global atomics do not serialise.
+constexpr uint32_t LEN_PER_BLOCK = 4096;
+constexpr uint32_t LEN_MAX_BUCKETS = 1024;
+__global__ void __launch_bounds__(256)
+    k_len_hist(const uint32_t *__restrict__ L, uint32_t n, uint32_t nb,
+               uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[LEN_MAX_BUCKETS];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t i0 = blockIdx.x * LEN_PER_BLOCK;
+  const uint32_t i1 = min(n, i0 + LEN_PER_BLOCK);
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    atomicAdd(&h[nb - 1 - min(L[i] / 8, nb - 1)], 1u);
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
 }
-__global__ void k_len_scatter(const uint32_t *__restrict__ w,
-                              const uint32_t *__restrict__ L,
-                              const float *__restrict__ R, uint32_t n,
-                              uint32_t nb, const uint32_t *__restrict__ start,
-                              uint32_t *__restrict__ fill,
-                              uint32_t *__restrict__ ow,
-                              uint32_t *__restrict__ oL, float *__restrict__ oR) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t b = nb - 1 - min(L[i] / 8, nb - 1);
-  uint32_t p = start[b] + atomicAdd(&fill[b], 1u);
-  ow[p] = w[i];
-  oL[p] = L[i];
-  if (R) oR[p] = R[i];
+__global__ void __launch_bounds__(256)
+    k_len_scatter(const uint32_t *__restrict__ w, const uint32_t *__restrict__ L,
+                  const float *__restrict__ R, uint32_t n, uint32_t nb,
+                  const uint32_t *__restrict__ start, uint32_t *__restrict__ fill,
+                  uint32_t *__restrict__ ow, uint32_t *__restrict__ oL,
+                  float *__restrict__ oR) {
+  __shared__ uint32_t h[LEN_MAX_BUCKETS];
+  constexpr uint32_t PER_THREAD = LEN_PER_BLOCK / 256;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t i0 = blockIdx.x * LEN_PER_BLOCK;
+  uint32_t bucket[PER_THREAD], rank[PER_THREAD];
+#pragma unroll
+  for (uint32_t k = 0; k < PER_THREAD; k++) {
+    const uint32_t i = i0 + k * 256 + threadIdx.x;
+    bucket[k] = 0xFFFFFFFFu;
+    if (i < n) {
+      bucket[k] = nb - 1 - min(L[i] / 8, nb - 1);
+      rank[k] = atomicAdd(&h[bucket[k]], 1u);
+    }
+  }
+  __syncthreads();
+  // reserve this block's share of each bucket
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    h[b] = h[b] ? start[b] + atomicAdd(&fill[b], h[b]) : 0u;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < PER_THREAD; k++) {
+    if (bucket[k] == 0xFFFFFFFFu) continue;
+    const uint32_t i = i0 + k * 256 + threadIdx.x;
+    const uint32_t p = h[bucket[k]] + rank[k];
+    ow[p] = w[i];
+    oL[p] = L[i];
+    if (R) oR[p] = R[i];
+  }
 }
 // values by generator index from values by slot
 __global__ void k_unpermute(const uint32_t *__restrict__ order, size_t n,

@@ -588,16 +588,16 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
                        std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
     // longest lists first
-    const uint32_t NB = lbound / 8 + 1;
+    const uint32_t NB = std::min<uint32_t>(lbound / 8 + 1, LEN_MAX_BUCKETS);
     uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
     uint32_t *bst = need(TT.get<uint32_t>(NB + 1));
     uint32_t *bfill = need(TT.get<uint32_t>(NB));
     HIPCHK(hipMemsetAsync(hist, 0, (NB + 1) * sizeof(uint32_t), s));
     HIPCHK(hipMemsetAsync(bfill, 0, NB * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_len_hist, dim3(nblk(nwork)), dim3(256), 0, s, lens[0], nwork, NB, hist);
+    hipLaunchKernelGGL(k_len_hist, dim3((nwork + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK), dim3(256), 0, s, lens[0], nwork, NB, hist);
     scan_excl<uint32_t, uint32_t>(hist, bst, NB + 1, s);
     Rfirst = need(TT.get<float>(nwork));
-    hipLaunchKernelGGL(k_len_scatter, dim3(nblk(nwork)), dim3(256), 0, s, lists[0], lens[0], R0,
+    hipLaunchKernelGGL(k_len_scatter, dim3((nwork + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK), dim3(256), 0, s, lists[0], lens[0], R0,
                        nwork, NB, bst, bfill, lists[1], lens[1], Rfirst);
     HIPCHK(hipGetLastError());
     std::swap(lists[0], lists[1]);
