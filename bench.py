@@ -12,6 +12,16 @@ not reached) is reported beside it, with their connectivity checked equal.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints one JSON line on stdout.
+
+Roofline of the dominant kernel: its average launch duration comes from HIP
+events the library records around each launch on the build's stream
+(mv_stats), over the timed builds; the algorithmic work per launch is the
+per-unit figure (FP64 flops per completed cell for the clip kernel; bytes
+per list for the neighbour-list kernel) times the units one launch
+processes.  `traffic` is the HBM bytes per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE (scaled by the same counters on a plain copy of
+known size, see --calibrate-copy), read from profiles/traffic_<kernel>.md
+when that file exists.
 """
 from __future__ import annotations
 
@@ -19,6 +29,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -44,7 +55,33 @@ def parse():
     ap.add_argument("--modes", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=524288)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (elementwise kernel) so that "
+                         "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE can be scaled")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy of known size for scaling the HBM counters: y = x + 0
+    over 2^27 doubles (1 GiB read, 1 GiB written) in torch's elementwise
+    kernel."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    del x, y
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
 
 
 def build_device(L, xt, n, mode, periodic=True):
@@ -119,16 +156,19 @@ def main():
     xt = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
     torch.cuda.synchronize()
 
-    modes = {"plain": MODE_PLAIN}
-    if args.modes in ("auto", "learned", "both"):
+    modes = {}
+    learned_note = None
+    if args.modes in ("auto", "both", "plain"):
+        modes["plain"] = MODE_PLAIN
+    if args.modes in ("auto", "both", "learned"):
         try:
-            h, _ = build_device(L, xt[:4096].contiguous(), 4096, MODE_LEARNED)
-            L.mv_free(h)
+            from meshless_voro_amd import learned
+            learned.default_model()
             modes["learned"] = MODE_LEARNED
-        except Exception:
-            pass
-    if args.modes == "plain":
-        modes = {"plain": MODE_PLAIN}
+        except Exception as e:  # no checkpoint: report the plain mode alone
+            if args.modes == "learned":
+                raise
+            learned_note = f"learned mode unavailable: {e}"
 
     results = {}
     digests = {}
@@ -150,39 +190,53 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms = e0.elapsed_time(e1) / args.steps
+        def tot(k):
+            return float(np.sum([s[k] for s in stats]))
         results[name] = {"ms_per_step": ms, "wall_ms_per_step": 1e3 * wall / args.steps,
                          "cells_per_s": n / (ms / 1e3), "stats": stats[-1],
                          "ms_clip": float(np.mean([s["ms_clip"] for s in stats])),
-                         "ms_knn": float(np.mean([s["ms_knn"] for s in stats]))}
+                         "ms_knn": float(np.mean([s["ms_knn"] for s in stats])),
+                         "clip_wave": {"launches": tot("clip_wave_launches"),
+                                       "cells": tot("clip_wave_cells"),
+                                       "ms": tot("ms_clip_wave")},
+                         "knn": {"launches": tot("knn_launches"), "queries": tot("knn_queries"),
+                                 "entries": tot("knn_entries"), "ms": tot("ms_knn_kernels")}}
         h, _ = build_device(L, xt, n, mode)
         digests[name] = connectivity_digest(L, h)
         L.mv_free(h)
 
-    same = True
-    if "learned" in digests:
+    same = None
+    if "learned" in digests and "plain" in digests:
         a, b = digests["plain"], digests["learned"]
         same = all(np.array_equal(p, q) for p, q in zip(a, b))
     best = max(results, key=lambda k: results[k]["cells_per_s"])
     R = results[best]
 
-    # roofline of the dominant kernel of the best mode
-    if R["ms_clip"] >= R["ms_knn"]:
-        t = R["ms_clip"] / 1e3
-        achieved = FLOPS_PER_CELL * n / t / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 (vector and matrix peaks coincide on MI355X)",
-                "kernel": "k_clip", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "per_unit": f"{FLOPS_PER_CELL:.0f} FP64 flops/cell x {n} cells per build",
-                "kernel_ms_per_build": R["ms_clip"]}
+    # roofline of the dominant kernel of the best mode, per launch
+    cw, kn = R["clip_wave"], R["knn"]
+    if cw["ms"] >= kn["ms"] and cw["launches"] > 0:
+        avg_ms = cw["ms"] / cw["launches"]
+        units = cw["cells"] / cw["launches"]
+        achieved = FLOPS_PER_CELL * units / (avg_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64 VALU (the FP64 vector and matrix peaks "
+                                          "coincide on MI355X)",
+                "kernel": "k_clip_wave", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": traffic_from_profile("k_clip_wave"),
+                "per_unit": f"{FLOPS_PER_CELL:.0f} FP64 flops per completed cell",
+                "units_per_launch": units, "avg_launch_ms": avg_ms,
+                "launches": cw["launches"]}
     else:
-        t = R["ms_knn"] / 1e3
-        k = R["stats"]["k_last"]
-        bytes_ = n * (32 + 8 * 27 + 4)
-        achieved = bytes_ / t / 1e9
+        avg_ms = kn["ms"] / kn["launches"]
+        q = kn["queries"] / kn["launches"]
+        ent = kn["entries"] / kn["launches"]
+        bytes_ = 32 * q + 8 * ent + 12 * q
+        achieved = bytes_ / (avg_ms / 1e3) / 1e9
         roof = {"bound": "hbm", "kernel": "k_knn", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "per_unit": "252 B/query (position read, 27-entry list + count written)",
-                "kernel_ms_per_build": R["ms_knn"]}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic_from_profile("k_knn"),
+                "per_unit": "32 B query position + 8 B per list entry + 12 B count/certificate",
+                "units_per_launch": q, "avg_launch_ms": avg_ms, "launches": kn["launches"]}
 
     out = {
         "metric": METRIC, "value": R["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
@@ -196,10 +250,15 @@ def main():
                       "ms_clip": v["ms_clip"], "ms_knn": v["ms_knn"],
                       "passes": v["stats"]["passes"], "cells_redone": v["stats"]["cells_redone"]}
                   for k, v in results.items()},
-        "speedup_vs_plain": R["cells_per_s"] / results["plain"]["cells_per_s"],
+        "speedup_vs_plain": (R["cells_per_s"] / results["plain"]["cells_per_s"]
+                             if "plain" in results else None),
         "same_topology": same,
         "roofline": roof,
     }
+    if learned_note:
+        out["note"] = learned_note
+    if args.calibrate_copy:
+        calibrate_copy()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)

@@ -159,6 +159,16 @@ typedef struct mv_stats {
   uint64_t exact_tests;     /* exact in-sphere evaluations */
   uint64_t big_cells;       /* cells run with the large-capacity kernel */
   double ms_grid, ms_knn, ms_clip, ms_finalize, ms_total;
+  /* per-kernel figures of the two hot kernels (device time from HIP events
+   * on the build's stream, summed over launches) */
+  uint32_t clip_wave_launches;  /* one-cell-per-wavefront clip launches */
+  uint32_t knn_launches;        /* neighbour-list launches (all variants) */
+  uint64_t clip_wave_cells;     /* cells the wavefront clip completed */
+  uint64_t knn_queries;         /* neighbour lists built */
+  uint64_t knn_entries;         /* list entries written */
+  double ms_clip_wave;          /* device time of those clip launches */
+  double ms_knn_kernels;        /* device time of the neighbour-list launches */
+  double ms_model;              /* features + proposal inference */
 } mv_stats;
 void mv_get_stats(const mv_tess *t, mv_stats *s);
 

@@ -71,7 +71,12 @@ class mv_stats(C.Structure):
                 ("cells_redone", C.c_uint64), ("exact_tests", C.c_uint64),
                 ("big_cells", C.c_uint64), ("ms_grid", C.c_double),
                 ("ms_knn", C.c_double), ("ms_clip", C.c_double),
-                ("ms_finalize", C.c_double), ("ms_total", C.c_double)]
+                ("ms_finalize", C.c_double), ("ms_total", C.c_double),
+                ("clip_wave_launches", C.c_uint32), ("knn_launches", C.c_uint32),
+                ("clip_wave_cells", C.c_uint64), ("knn_queries", C.c_uint64),
+                ("knn_entries", C.c_uint64),
+                ("ms_clip_wave", C.c_double), ("ms_knn_kernels", C.c_double),
+                ("ms_model", C.c_double)]
 
 
 class mv_device_view(C.Structure):

@@ -33,9 +33,18 @@ constexpr int WCH = 16;   // vertices per decomposition chunk
 struct WaveLds {
   double4 pa[WCAP];      // plane: n.xyz, d
   double4 pb[WCAP];      // plane: p.xyz, reference word
-  // per (tet, vertex) of a chunk of WCH vertices:
-  double4 fcf[6][WCH];   // area, area * (v0 + v1 + v2)
-  double4 fcv[6][WCH];   // volume, volume * (v0 + v1 + v2 + gen)
+  union {
+    // p and reference word of the 64 stream entries being consumed (lane
+    // t wrote entry t; read back when the entry's plane is kept)
+    struct {
+      double4 pr[64];
+    } e;
+    // decomposition: per (tet, vertex) of a chunk of WCH vertices
+    struct {
+      double4 fcf[6][WCH];  // area, area * (v0 + v1 + v2)
+      double4 fcv[6][WCH];  // volume, volume * (v0 + v1 + v2 + gen)
+    } t;
+  } u;
 };
 
 struct WaveArgs {
@@ -245,24 +254,27 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     // ---- ConvexCell::build (convex_cell.rs:316-356)
     bool stopped = false;
     for (uint32_t j0 = 1; state == CS_DONE && !stopped && j0 < cnt; j0 += 64) {
-      // prepare the planes of entries j0 .. j0+63, one per lane
+      // prepare the planes of entries j0 .. j0+63, one per lane, and stage
+      // them in LDS
       const uint32_t je = j0 + (uint32_t)lane;
-      uint64_t ppk = 0;
-      d3 pn = mk3(0, 0, 0), pp = mk3(0, 0, 0);
+      d3 pn = mk3(0, 0, 0);
       double pdist = 1., pd = 0., perrb = 0.;
+      lds_fence();
       if (je < cnt) {
-        ppk = list[je];
+        const uint64_t ppk = list[je];
         d3 ngb = xyz(A.spos[pk_slot(ppk)]);
         d3 s;
         if (image_shift(pk_img(ppk), B.width, &s)) ngb = add3(ngb, s);
         d3 dx = sub3(q, ngb);
         pdist = sqrt(dot3(dx, dx));
         pn = div3(dx, pdist);
-        pp = scl3(0.5, add3(q, ngb));
+        const d3 pp = scl3(0.5, add3(q, ngb));
         // HalfSpace::new (half_space.rs:24-34)
         perrb = 1e-13 * (1. + dot3(abs3(pn), abs3(pp)));
         pd = dot3(pn, pp);
+        S.u.e.pr[lane] = make_double4(pp.x, pp.y, pp.z, __longlong_as_double((long long)ppk));
       }
+      lds_fence();
       const int nb = (int)min(64u, cnt - j0);
       for (int t = 0; t < nb; t++) {
         const double dist = rl_d(pdist, t);
@@ -284,7 +296,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         const double cv = dot3(n, mk3(vx, vy, vz)) - d;
         const bool near = act && fabs(cv) < errb;
         bool clipped = act && !near && cv < 0.;
-        const uint64_t ref = rl_u64(ppk, t);
+        const uint64_t ref = (uint64_t)__double_as_longlong(S.u.e.pr[t].w);
         if (__ballot(near)) {
           d3 ngb = xyz(A.spos[pk_slot(ref)]);
           d3 sh;
@@ -302,15 +314,6 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         }
         uint64_t M = __ballot(clipped);
         if (!M) continue;
-        // kept vertices' radius maximum, and the NaN panic of
-        // update_safety_radius
-        const bool kept = act && !clipped;
-        if (__ballot(kept && isnan(vr))) {
-          state = CS_ERROR;
-          err = E_NAN;
-          break;
-        }
-        const double keepmax = wave_max_nonneg(kept ? vr : 0.);
         // replay of the swap-with-last scan on the mask: `src` = the slot
         // each position takes its vertex from
         int src = lane;
@@ -344,12 +347,9 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           break;
         }
         const int p_idx = np;
-        {
-          const d3 p = mk3(rl_d(pp.x, t), rl_d(pp.y, t), rl_d(pp.z, t));
-          if (lane == 0) {
-            S.pa[p_idx] = make_double4(n.x, n.y, n.z, d);
-            S.pb[p_idx] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)ref));
-          }
+        if (lane == 0) {
+          S.pa[p_idx] = make_double4(n.x, n.y, n.z, d);
+          S.pb[p_idx] = S.u.e.pr[t];
         }
         np += 1;
         // compute_boundary (convex_cell.rs:421-441) over positions
@@ -450,8 +450,9 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           break;
         }
         nv += clen;
-        // update_safety_radius (convex_cell.rs:443-453)
-        sr = 2. * sqrt(fmax(keepmax, wave_max_nonneg(newr)));
+        // update_safety_radius (convex_cell.rs:443-453): the maximum over
+        // every vertex (kept ones were checked for NaN when created)
+        sr = 2. * sqrt(wave_max_nonneg(lane < nv ? vr : 0.));
       }
     }
     if (state == CS_DONE && !stopped) {
@@ -467,6 +468,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
 
     // ---- decomposition and integrals (convex_cell.rs:131-187,
     // voronoi_cell.rs:40-100, integrals.rs:91-105, voronoi_face.rs:30-43)
+    lds_fence();  // the entry staging area becomes the tet staging area
     uint32_t nf = 0;
     uint64_t facemask = 0;
     double fa = 0., fx = 0., fy = 0., fz = 0.;
@@ -516,8 +518,8 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
               const d3 tc = scl3(tv, add3(add3(add3(v0, v1), vl), q));
               const double a = signed_area_tri(v0, v1, vl, q);
               const d3 sc = scl3(a, add3(add3(v0, v1), vl));
-              S.fcv[2 * k + h][sl] = make_double4(tv, tc.x, tc.y, tc.z);
-              S.fcf[2 * k + h][sl] = make_double4(a, sc.x, sc.y, sc.z);
+              S.u.t.fcv[2 * k + h][sl] = make_double4(tv, tc.x, tc.y, tc.z);
+              S.u.t.fcf[2 * k + h][sl] = make_double4(a, sc.x, sc.y, sc.z);
             }
             prev = pedge;
           }
@@ -532,7 +534,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           // VolumeCentroidIntegral::collect (integrals.rs:91-95), 6 tets
 #pragma unroll
           for (int t = 0; t < 6; t++) {
-            const double4 c = S.fcv[t][sl];
+            const double4 c = S.u.t.fcv[t][sl];
             vol += c.x;
             cx += c.y;
             cy += c.z;
@@ -544,7 +546,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
                         : ((du >> 8) & 0xFF) == lane ? 1
                         : ((du >> 16) & 0xFF) == lane ? 2 : -1;
           if (k >= 0) {
-            const double4 f0 = S.fcf[2 * k][sl], f1 = S.fcf[2 * k + 1][sl];
+            const double4 f0 = S.u.t.fcf[2 * k][sl], f1 = S.u.t.fcf[2 * k + 1][sl];
             fa += f0.x;
             fx += f0.y;
             fy += f0.z;

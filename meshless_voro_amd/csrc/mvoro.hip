@@ -643,6 +643,10 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   uint64_t redone = 0, bigcells = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
+  // per-kernel figures (reset with each attempt)
+  double ms_knn_k = 0, ms_cw = 0;
+  uint32_t n_knn = 0, n_cw = 0;
+  uint64_t q_knn = 0, cells_cw = 0, e_knn = 0;
   const uint32_t nwork0 = nwork;
 
   uint32_t *st_cell = nullptr;
@@ -675,6 +679,9 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     passes = 0;
     redone = 0;
     bigcells = 0;
+    ms_knn_k = ms_cw = 0;
+    n_knn = n_cw = 0;
+    q_knn = cells_cw = e_knn = 0;
     while (nw + nbig > 0) {
       passes++;
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
@@ -685,8 +692,13 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
+        Timer tkk(s);
         launch_knn(spos, cell_start, grid, box, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist,
                    big, TT, s);
+        ms_knn_k += tkk.stop();
+        n_knn++;
+        q_knn += cnt;
+        e_knn += total;
         ms_knn += tk.stop();
         Timer tc(s);
         if (!big && !exact_order) {
@@ -724,8 +736,11 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
           W.overflow = counters + 3;
           HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
           uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * 4u);
+          Timer tcw(s);
           hipLaunchKernelGGL(k_clip_wave, dim3(blocks), dim3(256), 0, s, W);
           HIPCHK(hipGetLastError());
+          ms_cw += tcw.stop();
+          n_cw++;
           ms_clip += tc.stop();
           return;
         }
@@ -833,6 +848,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         throw BuildError{MV_E_CAPACITY};
       bigcells += ncap;
       redone += nw_next + nb_next;
+      if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
       const uint32_t lb_main_old = lb_main, lmin_main_old = lmin_main;
       lb_main = std::min(2 * lb_main, kmax);
       lmin_main = std::min(2 * lmin_main, kmax);
@@ -864,6 +880,14 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   T->stats.big_cells = bigcells;
   T->stats.ms_knn = ms_knn + ms_model;
   T->stats.ms_clip = ms_clip;
+  T->stats.clip_wave_launches = n_cw;
+  T->stats.knn_launches = n_knn;
+  T->stats.clip_wave_cells = cells_cw;
+  T->stats.knn_queries = q_knn;
+  T->stats.knn_entries = e_knn;
+  T->stats.ms_clip_wave = ms_cw;
+  T->stats.ms_knn_kernels = ms_knn_k;
+  T->stats.ms_model = ms_model;
 
   // ---- faces and links
   Timer tf(s);

@@ -30,3 +30,32 @@ def test_status_strings_without_gpu():
     L = _native.lib()
     for s in range(10):
         assert L.mv_status_string(s)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Field offsets and sizes of the ctypes mirrors equal those the C
+    compiler gives the header's structs."""
+    import subprocess
+    structs = {"mv_options": _native.mv_options, "mv_stats": _native.mv_stats,
+               "mv_model": _native.mv_model, "mv_train_config": _native.mv_train_config,
+               "mv_device_view": _native.mv_device_view}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mvoro.h"', "int main(void){"]
+    for sn, cls in structs.items():
+        lines.append(f'printf("{sn} size %zu\\n", sizeof({sn}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{sn} {f} %zu\\n", offsetof({sn}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {}
+    for line in out.splitlines():
+        sn, f, v = line.split()
+        got[(sn, f)] = int(v)
+    for sn, cls in structs.items():
+        assert got[(sn, "size")] == C.sizeof(cls), sn
+        for f, _ in cls._fields_:
+            assert got[(sn, f)] == getattr(cls, f).offset, (sn, f)
