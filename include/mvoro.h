@@ -74,6 +74,12 @@ typedef struct mv_options {
  * connectivity with integrals equal up to rounding. */
 #define MV_FLAG_EXACT_ORDER 1u
 
+/* VoronoiIntegrator::build (src/voronoi.rs:549-621): also keep every active
+ * cell's half-spaces and vertices on the device, for custom cell and face
+ * integrals (mv_cell_integrals, mv_face_integrals, mv_tets and the device
+ * templates of mvoro_integrals.hpp). */
+#define MV_FLAG_KEEP_CELLS 2u
+
 /* Fill *opt with the defaults. */
 void mv_default_options(mv_options *opt);
 
@@ -178,6 +184,57 @@ void mv_get_stats(const mv_tess *t, mv_stats *s);
 const uint32_t *mv_cell_n_popped(mv_tess *t);
 
 const char *mv_status_string(mv_status s);
+
+/* ---------------------------------------------------------------------
+ * Custom integrals over kept cells (builds with MV_FLAG_KEEP_CELLS).
+ * Outputs follow VoronoiIntegrator (src/voronoi.rs:661-767): one entry per
+ * active cell in generator order; faces per active cell in half-space
+ * order. */
+
+/* Number of active (built) cells. */
+size_t mv_n_active(const mv_tess *t);
+
+/* compute_cell_integrals::<I> for the reference's example integrals
+ * (src/voronoi/integrals.rs:69-142): MV_CELL_VOLUME writes 1 double per
+ * active cell, MV_CELL_VOLUME_CENTROID 4 (volume, centroid x, y, z). */
+enum { MV_CELL_VOLUME = 0, MV_CELL_VOLUME_CENTROID = 1 };
+mv_status mv_cell_integrals(mv_tess *t, uint32_t kind, double *out);
+
+/* compute_face_integrals::<I> (symmetric = 0) and
+ * compute_face_integrals_sym::<I> (symmetric = 1) for AreaIntegral
+ * (MV_FACE_AREA: 1 value) and AreaCentroidIntegral (MV_FACE_AREA_CENTROID:
+ * area, centroid x, y, z), with FaceIntegrator's left / right (-1 = None) /
+ * shift and the face's half-space index within its cell.  Call with
+ * left == NULL to get *count; then pass arrays of that many entries (any of
+ * right, has_shift, shift, plane, values may be NULL). */
+enum { MV_FACE_AREA = 0, MV_FACE_AREA_CENTROID = 1 };
+mv_status mv_face_integrals(mv_tess *t, uint32_t kind, uint32_t symmetric,
+                            size_t *count, uint64_t *left, int64_t *right,
+                            uint8_t *has_shift, double *shift, int32_t *plane,
+                            double *values);
+
+/* The oriented tets that CellIntegral::collect receives, for every active
+ * cell in order (ConvexCell::decompose, src/voronoi/convex_cell.rs:131-187,
+ * 576-592): v012 holds count x 9 doubles (v0, v1, v2), plane the half-space
+ * index of each tet, cell_offset n_active + 1 tet offsets.  Call with
+ * v012 == NULL to get *count.  For host-side custom integrals. */
+mv_status mv_tets(mv_tess *t, size_t *count, double *v012, int32_t *plane,
+                  uint64_t *cell_offset);
+
+/* Device view of the kept cells, the input of the templates in
+ * mvoro_integrals.hpp (mv::CellGeom has this layout).  Pointers are device
+ * memory owned by the handle. */
+typedef struct mv_cell_geometry {
+  const double *pl_a;       /* 4 per half-space: n.xyz, d */
+  const double *pl_b;       /* 4 per half-space: p.xyz, reference word */
+  const double *vx;         /* 4 per vertex: loc.xyz, packed dual bits */
+  const uint64_t *pbase, *vbase;  /* per generator */
+  const uint32_t *np, *nv;        /* per generator (0: not built) */
+  const double *cell_loc;         /* per generator, 3 */
+  double width[3];
+  int32_t dim;
+} mv_cell_geometry;
+mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g);
 
 /* ---------------------------------------------------------------------
  * Learned neighbour-count proposal (no counterpart in the reference; it

@@ -15,7 +15,8 @@ _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libmvoro.so")
 SOURCES = [os.path.join(_PKG, "csrc", f) for f in (
     "mvoro.hip", "mv_common.hpp", "mv_geom.hpp", "mv_grid.hpp", "mv_clip.hpp",
-    "mv_clipw.hpp", "mv_finalize.hpp", "mv_learn.hpp")]
+    "mv_clipw.hpp", "mv_finalize.hpp", "mv_learn.hpp")] + [
+    os.path.join(_ROOT, "include", f) for f in ("mvoro.h", "mvoro_integrals.hpp")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -29,6 +30,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     """Compile libmvoro.so in-tree with hipcc for gfx950."""
     cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(_ROOT, "include"),
+           "-I", os.path.join(_PKG, "csrc"),
            "-o", LIB_PATH + ".tmp", os.path.join(_PKG, "csrc", "mvoro.hip")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -50,6 +52,15 @@ class mv_options(C.Structure):
 
 
 MV_MODEL_MAGIC = 0x4B4C564D
+MV_FLAG_EXACT_ORDER = 1
+MV_FLAG_KEEP_CELLS = 2
+
+
+class mv_cell_geometry(C.Structure):
+    _fields_ = [("pl_a", C.c_void_p), ("pl_b", C.c_void_p), ("vx", C.c_void_p),
+                ("pbase", C.c_void_p), ("vbase", C.c_void_p), ("np", C.c_void_p),
+                ("nv", C.c_void_p), ("cell_loc", C.c_void_p), ("width", C.c_double * 3),
+                ("dim", C.c_int32)]
 
 
 class mv_model(C.Structure):
@@ -96,7 +107,8 @@ EXPORTS = [
     "mv_cell_idx", "mv_face_left", "mv_face_right", "mv_face_area",
     "mv_face_centroid", "mv_face_normal", "mv_face_has_shift", "mv_face_shift",
     "mv_cell_face_connections", "mv_get_device_view", "mv_get_stats",
-    "mv_cell_n_popped", "mv_status_string", "mv_model_features", "mv_model_train",
+    "mv_cell_n_popped", "mv_status_string", "mv_n_active", "mv_cell_integrals",
+    "mv_face_integrals", "mv_tets", "mv_get_cell_geometry", "mv_model_features", "mv_model_train",
     "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact",
 ]
 
@@ -134,6 +146,16 @@ def lib():
     L.mv_get_stats.argtypes = [vp, C.POINTER(mv_stats)]
     L.mv_status_string.restype = C.c_char_p
     L.mv_status_string.argtypes = [C.c_int]
+    L.mv_n_active.restype = sz
+    L.mv_n_active.argtypes = [vp]
+    L.mv_cell_integrals.restype = C.c_int
+    L.mv_cell_integrals.argtypes = [vp, u32, vp]
+    L.mv_face_integrals.restype = C.c_int
+    L.mv_face_integrals.argtypes = [vp, u32, u32, C.POINTER(C.c_size_t), vp, vp, vp, vp, vp, vp]
+    L.mv_tets.restype = C.c_int
+    L.mv_tets.argtypes = [vp, C.POINTER(C.c_size_t), vp, vp, vp]
+    L.mv_get_cell_geometry.restype = C.c_int
+    L.mv_get_cell_geometry.argtypes = [vp, C.POINTER(mv_cell_geometry)]
     L.mv_model_features.restype = C.c_int
     L.mv_model_features.argtypes = [vp, sz, vp, vp, u32, u32, vp, vp]
     L.mv_model_train.restype = C.c_int

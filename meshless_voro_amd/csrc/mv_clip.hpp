@@ -54,7 +54,8 @@ struct ClipArgs {
   unsigned long long *st_counter;
   unsigned long long *err_word;  // min over (idx << 8 | code)
   unsigned long long *exact_counter;
-  unsigned long long *overflow;  // set when the staging area is too small
+  unsigned long long *overflow;  // bit 0: staging area full, bit 1: cell store full
+  CellStore cs;                  // kept cells (cs.pl_a == nullptr: none)
 };
 
 constexpr uint64_t BOX_REF = ~0ull;
@@ -477,7 +478,43 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   return CS_DONE;
 }
 
-template <int MAXP, int MAXV>
+// Append a completed cell's half-spaces and vertices to the cell store
+// (wave-aggregated cursors; every lane of the wavefront calls this).
+__device__ __forceinline__ void keep_cell(const Lane &S, const ClipArgs &A,
+                                          bool emit, const CellRegs &c) {
+  const uint32_t lane = __lane_id();
+  const uint32_t np = emit ? (uint32_t)c.np : 0u, nv = emit ? (uint32_t)c.nv : 0u;
+  uint32_t tp, tv;
+  const uint32_t ep = wave_excl_scan(np, &tp);
+  const uint32_t ev = wave_excl_scan(nv, &tv);
+  unsigned long long bp = 0, bv = 0;
+  if (lane == 0 && tp) {
+    bp = atomicAdd(A.cs.cur_pl, (unsigned long long)tp);
+    bv = atomicAdd(A.cs.cur_vx, (unsigned long long)tv);
+  }
+  bp = __shfl(bp, 0, 64) + ep;
+  bv = __shfl(bv, 0, 64) + ev;
+  if (!emit) return;
+  if (bp + np > A.cs.cap_pl || bv + nv > A.cs.cap_vx) {
+    atomicOr(A.overflow, 2ull);
+    return;
+  }
+  for (uint32_t k = 0; k < np; k++) {
+    A.cs.pl_a[bp + k] = S.plA[S.at((int)k)];
+    A.cs.pl_b[bp + k] = S.plB[S.at((int)k)];
+  }
+  for (uint32_t v = 0; v < nv; v++) {
+    const double4 x = S.vtx[S.at((int)v)];
+    A.cs.vx[bv + v] = make_double4(x.x, x.y, x.z,
+                                   __longlong_as_double((long long)S.vdual[S.at((int)v)]));
+  }
+  A.cs.pbase[c.idx] = bp;
+  A.cs.vbase[c.idx] = bv;
+  A.cs.np[c.idx] = np;
+  A.cs.nv[c.idx] = nv;
+}
+
+template <int MAXP, int MAXV, bool KEEP>
 __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
   const size_t gl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = __lane_id();
@@ -518,6 +555,7 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
         nf = 0;
       }
     }
+    if (KEEP) keep_cell(S, A, emit, c);
     uint32_t total;
     uint32_t excl = wave_excl_scan(nf, &total);
     unsigned long long base = 0;

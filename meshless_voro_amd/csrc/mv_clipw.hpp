@@ -73,7 +73,8 @@ struct WaveArgs {
   unsigned long long *st_counter;
   unsigned long long *err_word;
   unsigned long long *exact_counter;
-  unsigned long long *overflow;
+  unsigned long long *overflow;  // bit 0: staging area full, bit 1: cell store full
+  CellStore cs;                  // kept cells (cs.pl_a == nullptr: none)
 };
 
 __device__ __forceinline__ int rl_i(int v, int l) {
@@ -178,6 +179,9 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
   cyc = lane == at ? val : cyc;
 }
 
+// KEEP: also append every completed cell to the kept-cell store (a separate
+// instantiation, so that the plain kernel keeps its register budget)
+template <bool KEEP>
 __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
   __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
@@ -609,6 +613,32 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         A.st_norm[3 * o] = a.x;
         A.st_norm[3 * o + 1] = a.y;
         A.st_norm[3 * o + 2] = a.z;
+      }
+    }
+    if (KEEP && state == CS_DONE) {
+      // VoronoiIntegrator: keep the cell's half-spaces and vertices
+      unsigned long long bp = 0, bv = 0;
+      if (lane == 0) {
+        bp = atomicAdd(A.cs.cur_pl, (unsigned long long)np);
+        bv = atomicAdd(A.cs.cur_vx, (unsigned long long)nv);
+      }
+      bp = rl_u64(bp, 0);
+      bv = rl_u64(bv, 0);
+      if (bp + np > A.cs.cap_pl || bv + nv > A.cs.cap_vx) {
+        if (lane == 0) atomicOr(A.overflow, 2ull);
+      } else {
+        if (lane < np) {
+          A.cs.pl_a[bp + lane] = S.pa[lane];
+          A.cs.pl_b[bp + lane] = S.pb[lane];
+        }
+        if (lane < nv)
+          A.cs.vx[bv + lane] = make_double4(vx, vy, vz, __longlong_as_double((long long)vd));
+        if (lane == 0) {
+          A.cs.pbase[idx] = bp;
+          A.cs.vbase[idx] = bv;
+          A.cs.np[idx] = (uint32_t)np;
+          A.cs.nv[idx] = (uint32_t)nv;
+        }
       }
     }
     next = (uint32_t)rl_i((int)nx, 0);

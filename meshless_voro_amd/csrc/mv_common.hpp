@@ -96,4 +96,17 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v,
   return x - v;
 }
 
+// Kept cells (VoronoiIntegrator): every completed cell's half-spaces and
+// vertices, appended to two arenas by atomic cursors and indexed by
+// generator.  pl_a == nullptr: nothing is kept.
+struct CellStore {
+  double4 *pl_a;   // n.xyz, d
+  double4 *pl_b;   // p.xyz, reference word
+  double4 *vx;     // vertex loc.xyz, packed dual (bits) in w
+  uint64_t cap_pl, cap_vx;
+  unsigned long long *cur_pl, *cur_vx;
+  uint64_t *pbase, *vbase;  // by generator index
+  uint32_t *np, *nv;        // by generator index
+};
+
 }  // namespace mv

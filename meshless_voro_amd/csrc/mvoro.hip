@@ -24,6 +24,7 @@
 #include "mv_finalize.hpp"
 #include "mv_grid.hpp"
 #include "mv_learn.hpp"
+#include "mvoro_integrals.hpp"
 
 using namespace mv;
 
@@ -215,6 +216,12 @@ struct mv_tess {
          *d_face_shift = nullptr;
   uint8_t *d_face_has_shift = nullptr;
   uint64_t *d_links = nullptr;
+  // kept cells (MV_FLAG_KEEP_CELLS)
+  bool keep = false;
+  CellStore cs{};
+  uint64_t used_pl = 0, used_vx = 0, n_active = 0;
+  uint8_t *d_mask = nullptr;       // copy of the build's mask (symmetric faces)
+  uint64_t *d_active = nullptr;    // active cells in generator order
   // host copies
   std::mutex mu;
   bool have_host = false;
@@ -234,6 +241,13 @@ struct mv_tess {
     if (!p) throw BuildError{MV_E_OOM};
     dev.push_back(p);
     return p;
+  }
+  void release(void *p) {
+    auto it = std::find(dev.begin(), dev.end(), p);
+    if (it != dev.end()) {
+      dev.erase(it);
+      dfree(p);
+    }
   }
   ~mv_tess() {
     for (void *p : dev) dfree(p);
@@ -615,6 +629,40 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   uint64_t st_cap = std::max<uint64_t>(4096, 16ull * nwork);
   unsigned long long *counters = need(TT.get<unsigned long long>(4));
   // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow
+  // kept cells: arenas sized for ~48 half-spaces and ~64 vertices per cell,
+  // doubled when a pass runs out
+  T->keep = (opt.flags & MV_FLAG_KEEP_CELLS) != 0;
+  unsigned long long *cs_cur = need(TT.get<unsigned long long>(2));
+  uint64_t cap_pl = 0, cap_vx = 0;
+  auto alloc_store = [&]() {
+    if (T->cs.pl_a) {
+      T->release(T->cs.pl_a);
+      T->release(T->cs.pl_b);
+      T->release(T->cs.vx);
+    }
+    T->cs.pl_a = T->own<double4>(cap_pl);
+    T->cs.pl_b = T->own<double4>(cap_pl);
+    T->cs.vx = T->own<double4>(cap_vx);
+    T->cs.cap_pl = cap_pl;
+    T->cs.cap_vx = cap_vx;
+  };
+  if (T->keep) {
+    T->cs.pbase = T->own<uint64_t>(n);
+    T->cs.vbase = T->own<uint64_t>(n);
+    T->cs.np = T->own<uint32_t>(n);
+    T->cs.nv = T->own<uint32_t>(n);
+    HIPCHK(hipMemsetAsync(T->cs.np, 0, n * sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(T->cs.nv, 0, n * sizeof(uint32_t), s));
+    T->cs.cur_pl = cs_cur;
+    T->cs.cur_vx = cs_cur + 1;
+    cap_pl = 48ull * nwork + 4096;
+    cap_vx = 64ull * nwork + 4096;
+    alloc_store();
+    if (d_mask) {
+      T->d_mask = T->own<uint8_t>(n);
+      HIPCHK(hipMemcpyAsync(T->d_mask, d_mask, n, hipMemcpyDeviceToDevice, s));
+    }
+  }
   uint8_t *wstate = need(TT.get<uint8_t>(n));
   uint8_t *bstate = need(TT.get<uint8_t>(n));
   unsigned int *queue = need(TT.get<unsigned int>(1));
@@ -662,6 +710,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     st_cen = need(TT.get<double>(3 * st_cap));
     st_norm = need(TT.get<double>(3 * st_cap));
     HIPCHK(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
+    HIPCHK(hipMemsetAsync(cs_cur, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long err_init = ~0ull;
     HIPCHK(hipMemcpyAsync(counters + 1, &err_init, sizeof err_init, hipMemcpyHostToDevice, s));
 
@@ -676,6 +725,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     uint32_t lmin_main = std::min(lmin0, kmax), lmin_big = kmax;
     const float *Rmain = Rfirst;
     bool overflowed = false;
+    unsigned long long ovf_bits = 0;
     passes = 0;
     redone = 0;
     bigcells = 0;
@@ -734,10 +784,14 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
           W.err_word = counters + 1;
           W.exact_counter = counters + 2;
           W.overflow = counters + 3;
+          if (T->keep) W.cs = T->cs;
           HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
           uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * 4u);
           Timer tcw(s);
-          hipLaunchKernelGGL(k_clip_wave, dim3(blocks), dim3(256), 0, s, W);
+          if (T->keep)
+            hipLaunchKernelGGL(k_clip_wave<true>, dim3(blocks), dim3(256), 0, s, W);
+          else
+            hipLaunchKernelGGL(k_clip_wave<false>, dim3(blocks), dim3(256), 0, s, W);
           HIPCHK(hipGetLastError());
           ms_cw += tcw.stop();
           n_cw++;
@@ -795,10 +849,16 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         A.err_word = counters + 1;
         A.exact_counter = counters + 2;
         A.overflow = counters + 3;
-        if (!big)
-          hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN>), dim3(lanes / 256), dim3(256), 0, s, A);
+        if (T->keep) A.cs = T->cs;
+        const dim3 grid(lanes / 256);
+        if (!big && !T->keep)
+          hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, false>), grid, dim3(256), 0, s, A);
+        else if (!big)
+          hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, true>), grid, dim3(256), 0, s, A);
+        else if (!T->keep)
+          hipLaunchKernelGGL((k_clip<MAXP_BIG, MAXV_BIG, false>), grid, dim3(256), 0, s, A);
         else
-          hipLaunchKernelGGL((k_clip<MAXP_BIG, MAXV_BIG>), dim3(lanes / 256), dim3(256), 0, s, A);
+          hipLaunchKernelGGL((k_clip<MAXP_BIG, MAXV_BIG, true>), grid, dim3(256), 0, s, A);
         HIPCHK(hipGetLastError());
         ms_clip += tc.stop();
       };
@@ -816,6 +876,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         throw BuildError{code < 7 ? map[code] : MV_E_DEVICE};
       }
       if (cnts[3]) {
+        ovf_bits = cnts[3];
         overflowed = true;
         break;
       }
@@ -868,11 +929,23 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
       unsigned long long cnts[4];
       HIPCHK(hipMemcpy(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost));
       T->stats.exact_tests = cnts[2];
+      if (T->keep) {
+        unsigned long long used[2];
+        HIPCHK(hipMemcpy(used, cs_cur, sizeof used, hipMemcpyDeviceToHost));
+        T->used_pl = used[0];
+        T->used_vx = used[1];
+      }
       T->stats.k_last = lb_main;
       break;
     }
     if (attempt > 4) throw BuildError{MV_E_CAPACITY};
-    st_cap *= 2;
+    // grow what ran out (bit 0: face staging, bit 1: kept-cell store)
+    if (ovf_bits & 1) st_cap *= 2;
+    if (ovf_bits & 2) {
+      cap_pl *= 2;
+      cap_vx *= 2;
+      alloc_store();
+    }
     HIPCHK(hipMemsetAsync(cell_nf, 0, n * sizeof(uint32_t), s));
   }
   T->stats.passes = passes;
@@ -1121,6 +1194,158 @@ mv_status guarded(F &&f) {
     return e.st;
   }
   return MV_OK;
+}
+
+// ---------------------------------------------------------------------
+// kept cells: device view and the active-cell list
+// ---------------------------------------------------------------------
+CellGeom geom_of(const mv_tess *t) {
+  CellGeom g;
+  g.pl_a = t->cs.pl_a;
+  g.pl_b = t->cs.pl_b;
+  g.vx = t->cs.vx;
+  g.pbase = t->cs.pbase;
+  g.vbase = t->cs.vbase;
+  g.np = t->cs.np;
+  g.nv = t->cs.nv;
+  g.cell_loc = t->d_cell_loc;
+  for (int c = 0; c < 3; c++) g.width[c] = t->width[c];
+  g.dim = (int)t->dim;
+  return g;
+}
+
+__global__ void k_flag_built(const uint32_t *__restrict__ np, size_t n,
+                             uint32_t *__restrict__ flag) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = np[i] ? 1u : 0u;
+}
+__global__ void k_compact_index(const uint32_t *__restrict__ flag,
+                                const uint64_t *__restrict__ pos, size_t n,
+                                uint64_t *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = i;
+}
+__global__ void k_gather_nv(const uint64_t *__restrict__ cells, uint64_t n,
+                            const uint32_t *__restrict__ nv, uint64_t *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 6ull * nv[cells[i]];
+}
+struct TetOut {
+  double *v;
+  int32_t *plane;
+  uint64_t k;
+  __device__ void operator()(d3 v0, d3 v1, d3 v2, int p) {
+    double *o = v + 9 * k;
+    o[0] = v0.x; o[1] = v0.y; o[2] = v0.z;
+    o[3] = v1.x; o[4] = v1.y; o[5] = v1.z;
+    o[6] = v2.x; o[7] = v2.y; o[8] = v2.z;
+    plane[k] = p;
+    k++;
+  }
+};
+__global__ void __launch_bounds__(256)
+    k_export_tets(CellGeom G, const uint64_t *__restrict__ cells, uint64_t n,
+                  const uint64_t *__restrict__ off, double *__restrict__ v,
+                  int32_t *__restrict__ plane, unsigned int *err) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TetOut f{v, plane, off[i]};
+  if (!for_each_tet(G, cells[i], f)) atomicOr(err, 1u);
+}
+
+// active cells (those the build kept) in generator order
+void ensure_active(mv_tess *t) {
+  if (t->d_active || t->n == 0) return;
+  hipStream_t s = 0;
+  Temps TT;
+  const size_t n = t->n;
+  uint32_t *flag = need(TT.get<uint32_t>(n));
+  uint64_t *pos = need(TT.get<uint64_t>(n));
+  hipLaunchKernelGGL(k_flag_built, dim3(nblk(n)), dim3(256), 0, s, t->cs.np, n, flag);
+  const uint64_t na = scan_excl<uint32_t, uint64_t>(flag, pos, n, s);
+  t->d_active = t->own<uint64_t>(na ? na : 1);
+  if (na)
+    hipLaunchKernelGGL(k_compact_index, dim3(nblk(n)), dim3(256), 0, s, flag, pos, n,
+                       t->d_active);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  t->n_active = na;
+}
+
+template <class I>
+void run_cell_integrals(mv_tess *t, double *out, int width) {
+  hipStream_t s = 0;
+  Temps TT;
+  ensure_active(t);
+  const uint64_t na = t->n_active;
+  if (!na) return;
+  I *d = need(TT.get<I>(na));
+  unsigned int *err = need(TT.get<unsigned int>(1));
+  HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  hipLaunchKernelGGL((cell_integrals<I>), dim3(nblk(na)), dim3(256), 0, s, geom_of(t),
+                     (const uint64_t *)t->d_active, na, (const void *)nullptr, d, err);
+  HIPCHK(hipGetLastError());
+  std::vector<I> h(na);
+  unsigned int herr = 0;
+  HIPCHK(hipMemcpyAsync(h.data(), d, na * sizeof(I), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (herr) throw BuildError{MV_E_PLANE_DET_ZERO};
+  static_assert(sizeof(I) % sizeof(double) == 0, "integral of doubles");
+  for (uint64_t i = 0; i < na; i++)
+    memcpy(out + (size_t)i * width, &h[i], width * sizeof(double));
+}
+
+template <class I>
+void run_face_integrals(mv_tess *t, int symmetric, size_t *count, uint64_t *left,
+                        int64_t *right, uint8_t *has_shift, double *shift, int32_t *plane,
+                        double *values, int width) {
+  hipStream_t s = 0;
+  Temps TT;
+  ensure_active(t);
+  const uint64_t na = t->n_active;
+  if (!na) {
+    *count = 0;
+    return;
+  }
+  int32_t *face_local = need(TT.get<int32_t>(t->used_pl ? t->used_pl : 1));
+  uint64_t *cnt = need(TT.get<uint64_t>(na));
+  uint64_t *off = need(TT.get<uint64_t>(na));
+  const CellGeom G = geom_of(t);
+  hipLaunchKernelGGL(face_counts, dim3(nblk(na)), dim3(256), 0, s, G,
+                     (const uint64_t *)t->d_active, na, symmetric,
+                     (const uint8_t *)t->d_mask, face_local, cnt);
+  HIPCHK(hipGetLastError());
+  const uint64_t F = scan_excl<uint64_t, uint64_t>(cnt, off, na, s);
+  if (!left) {
+    *count = F;
+    return;
+  }
+  if (*count < F) throw BuildError{MV_E_ARG};
+  *count = F;
+  if (!F) return;
+  FaceIntegrator<I> *d = need(TT.get<FaceIntegrator<I>>(F));
+  unsigned int *err = need(TT.get<unsigned int>(1));
+  HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+  hipLaunchKernelGGL((face_integrals<I>), dim3(nblk(na)), dim3(256), 0, s, G,
+                     (const uint64_t *)t->d_active, na, (const int32_t *)face_local,
+                     (const uint64_t *)off, (const void *)nullptr, d, err);
+  HIPCHK(hipGetLastError());
+  std::vector<FaceIntegrator<I>> h(F);
+  unsigned int herr = 0;
+  HIPCHK(hipMemcpyAsync(h.data(), d, F * sizeof(FaceIntegrator<I>), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (herr) throw BuildError{MV_E_PLANE_DET_ZERO};
+  for (uint64_t f = 0; f < F; f++) {
+    left[f] = h[f].left;
+    if (right) right[f] = h[f].right;
+    if (has_shift) has_shift[f] = (uint8_t)h[f].has_shift;
+    if (shift)
+      for (int c = 0; c < 3; c++) shift[3 * f + c] = h[f].shift[c];
+    if (plane) plane[f] = h[f].plane;
+    if (values) memcpy(values + (size_t)f * width, &h[f].integral, width * sizeof(double));
+  }
 }
 
 }  // namespace
@@ -1498,6 +1723,92 @@ mv_status mv_model_train(const float *features, const float *targets, size_t n,
       *final_loss = (float)(L / n);
     }
   });
+}
+
+size_t mv_n_active(const mv_tess *t) {
+  if (!t) return 0;
+  if (!t->keep) return 0;
+  (void)guarded([&] { ensure_active(const_cast<mv_tess *>(t)); });
+  return t->n_active;
+}
+
+mv_status mv_cell_integrals(mv_tess *t, uint32_t kind, double *out) {
+  if (!t || !t->keep || kind > MV_CELL_VOLUME_CENTROID) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  if (t->n && !out) return MV_E_ARG;
+  return guarded([&] {
+    if (kind == MV_CELL_VOLUME)
+      run_cell_integrals<VolumeIntegral>(t, out, 1);
+    else
+      run_cell_integrals<VolumeCentroidIntegral>(t, out, 4);
+  });
+}
+
+mv_status mv_face_integrals(mv_tess *t, uint32_t kind, uint32_t symmetric, size_t *count,
+                            uint64_t *left, int64_t *right, uint8_t *has_shift, double *shift,
+                            int32_t *plane, double *values) {
+  if (!t || !t->keep || !count || kind > MV_FACE_AREA_CENTROID) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  const int sym = symmetric ? 1 : 0;
+  return guarded([&] {
+    if (kind == MV_FACE_AREA)
+      run_face_integrals<AreaIntegral>(t, sym, count, left, right, has_shift, shift, plane,
+                                       values, 1);
+    else
+      run_face_integrals<AreaCentroidIntegral>(t, sym, count, left, right, has_shift, shift,
+                                               plane, values, 4);
+  });
+}
+
+mv_status mv_tets(mv_tess *t, size_t *count, double *v012, int32_t *plane,
+                  uint64_t *cell_offset) {
+  if (!t || !t->keep || !count) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    ensure_active(t);
+    const uint64_t na = t->n_active;
+    if (!na) {
+      *count = 0;
+      if (v012 && cell_offset) cell_offset[0] = 0;
+      return;
+    }
+    uint64_t *cnt = need(TT.get<uint64_t>(na));
+    uint64_t *off = need(TT.get<uint64_t>(na));
+    hipLaunchKernelGGL(k_gather_nv, dim3(nblk(na)), dim3(256), 0, s,
+                       (const uint64_t *)t->d_active, na, (const uint32_t *)t->cs.nv, cnt);
+    const uint64_t K = scan_excl<uint64_t, uint64_t>(cnt, off, na, s);
+    if (!v012) {
+      *count = K;
+      return;
+    }
+    if (*count < K || !plane || !cell_offset) throw BuildError{MV_E_ARG};
+    *count = K;
+    double *dv = need(TT.get<double>(9 * K));
+    int32_t *dp = need(TT.get<int32_t>(K));
+    unsigned int *err = need(TT.get<unsigned int>(1));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(k_export_tets, dim3(nblk(na)), dim3(256), 0, s, geom_of(t),
+                       (const uint64_t *)t->d_active, na, (const uint64_t *)off, dv, dp, err);
+    HIPCHK(hipGetLastError());
+    unsigned int herr = 0;
+    HIPCHK(hipMemcpyAsync(v012, dv, 9 * K * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane, dp, K * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cell_offset, off, na * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    cell_offset[na] = K;
+    if (herr) throw BuildError{MV_E_PLANE_DET_ZERO};
+  });
+}
+
+mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g) {
+  if (!t || !g || !t->keep) return MV_E_ARG;
+  const CellGeom G = geom_of(t);
+  static_assert(sizeof(CellGeom) == sizeof(mv_cell_geometry), "mv_cell_geometry layout");
+  memcpy(g, &G, sizeof G);
+  return MV_OK;
 }
 
 mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,

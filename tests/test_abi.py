@@ -38,7 +38,8 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"mv_options": _native.mv_options, "mv_stats": _native.mv_stats,
                "mv_model": _native.mv_model, "mv_train_config": _native.mv_train_config,
-               "mv_device_view": _native.mv_device_view}
+               "mv_device_view": _native.mv_device_view,
+               "mv_cell_geometry": _native.mv_cell_geometry}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mvoro.h"', "int main(void){"]
     for sn, cls in structs.items():
         lines.append(f'printf("{sn} size %zu\\n", sizeof({sn}));')
