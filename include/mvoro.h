@@ -104,6 +104,30 @@ mv_status mv_build_device(const double *d_xyz, size_t n,
                           uint32_t periodic, const mv_options *opt,
                           mv_tess **out);
 
+/* Many independent tessellations in one pass (the batched form of
+ * Voronoi::build / build_partial; no counterpart in the reference, which
+ * builds one at a time).  Problem p owns generators [offsets[p],
+ * offsets[p+1]) of xyz (and of mask), its box anchors[3p..3p+2] /
+ * widths[3p..3p+2] and periodic[p] (NULL: none periodic); offsets has P + 1
+ * entries, offsets[0] = 0.  out[p] receives problem p's handle, equal to
+ * what mv_build of that problem alone returns; each handle is freed with
+ * mv_free (the device memory is released with the last one).  All problems
+ * run in the same kernel launches: one grid build, one neighbour-list and
+ * one clip launch per pass over every cell of every problem. */
+mv_status mv_build_batch(const double *xyz, const uint64_t *offsets, uint32_t P,
+                         const uint8_t *mask, const double *anchors,
+                         const double *widths, const uint32_t *periodic,
+                         uint32_t dimensionality, const mv_options *opt,
+                         mv_tess **out);
+/* Same with device-resident xyz and mask (offsets, anchors, widths and
+ * periodic are host arrays); results stay on the device. */
+mv_status mv_build_batch_device(const double *d_xyz, const uint64_t *offsets,
+                                uint32_t P, const uint8_t *d_mask,
+                                const double *anchors, const double *widths,
+                                const uint32_t *periodic,
+                                uint32_t dimensionality,
+                                const mv_options *opt, mv_tess **out);
+
 void mv_free(mv_tess *t);
 
 /* Voronoi::anchor / width / dimensionality / periodic
@@ -233,6 +257,8 @@ typedef struct mv_cell_geometry {
   const double *cell_loc;         /* per generator, 3 */
   double width[3];
   int32_t dim;
+  uint64_t idx_base;  /* subtracted from the generator index of reference
+                         words (non-zero for the problems of a batch) */
 } mv_cell_geometry;
 mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g);
 

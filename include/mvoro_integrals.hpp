@@ -43,6 +43,8 @@ struct CellGeom {
   const double *cell_loc;  // generator position per cell (3 per cell)
   double width[3];         // periodic box width (image shifts)
   int dim;                 // dimensionality 1, 2, 3
+  uint64_t idx_base;       // generator index of the problem's first cell
+                           // in the reference words (batched builds)
 };
 
 constexpr uint64_t KEPT_BOX_REF = ~0ull;
@@ -66,7 +68,7 @@ __device__ __forceinline__ int64_t kept_right(const CellGeom &G, uint64_t c, int
     *shift = mk3(-((double)i * G.width[0]), -((double)j * G.width[1]),
                  -((double)k * G.width[2]));
   }
-  return (int64_t)(ref >> 32);
+  return (int64_t)((ref >> 32) - G.idx_base);
 }
 
 // Dimensionality::vector_is_valid of a half-space normal

@@ -60,7 +60,7 @@ class mv_cell_geometry(C.Structure):
     _fields_ = [("pl_a", C.c_void_p), ("pl_b", C.c_void_p), ("vx", C.c_void_p),
                 ("pbase", C.c_void_p), ("vbase", C.c_void_p), ("np", C.c_void_p),
                 ("nv", C.c_void_p), ("cell_loc", C.c_void_p), ("width", C.c_double * 3),
-                ("dim", C.c_int32)]
+                ("dim", C.c_int32), ("idx_base", C.c_uint64)]
 
 
 class mv_model(C.Structure):
@@ -100,7 +100,8 @@ class mv_device_view(C.Structure):
 
 # every symbol declared in include/mvoro.h
 EXPORTS = [
-    "mv_default_options", "mv_build", "mv_build_device", "mv_free", "mv_anchor",
+    "mv_default_options", "mv_build", "mv_build_device", "mv_build_batch",
+    "mv_build_batch_device", "mv_free", "mv_anchor",
     "mv_width", "mv_dimensionality", "mv_periodic", "mv_n_cells", "mv_n_faces",
     "mv_n_links", "mv_cell_loc", "mv_cell_centroid", "mv_cell_volume",
     "mv_cell_safety_radius", "mv_cell_face_offset", "mv_cell_face_count",
@@ -129,6 +130,11 @@ def lib():
     L.mv_build.argtypes = [vp, sz, vp, vp, vp, u32, u32, C.POINTER(mv_options), C.POINTER(vp)]
     L.mv_build_device.restype = C.c_int
     L.mv_build_device.argtypes = [vp, sz, vp, vp, vp, u32, u32, C.POINTER(mv_options), C.POINTER(vp)]
+    L.mv_build_batch.restype = C.c_int
+    L.mv_build_batch.argtypes = [vp, vp, u32, vp, vp, vp, vp, u32, C.POINTER(mv_options), vp]
+    L.mv_build_batch_device.restype = C.c_int
+    L.mv_build_batch_device.argtypes = [vp, vp, u32, vp, vp, vp, vp, u32,
+                                        C.POINTER(mv_options), vp]
     L.mv_free.argtypes = [vp]
     L.mv_anchor.argtypes = [vp, vp]
     L.mv_width.argtypes = [vp, vp]

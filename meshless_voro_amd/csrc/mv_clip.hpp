@@ -18,8 +18,7 @@ namespace mv {
 struct ClipArgs {
   const double4 *spos;   // sorted generators, w = generator index bits
   const uint8_t *mask;   // by generator index, nullptr = all active
-  BoxParams box;
-  uint64_t total_stream;
+  Problems PR;           // boxes, grids (total stream) per problem
   const uint32_t *work;  // slots to build
   uint32_t nwork;
   const uint64_t *kpk;   // neighbour lists, item-major
@@ -86,6 +85,8 @@ __device__ __forceinline__ int dual_at(uint32_t d, int k) {
 }
 
 struct CellRegs {
+  const BoxParams *B;    // the cell's problem
+  uint64_t total_stream;
   d3 q;
   uint64_t idx;
   int np, nv;
@@ -140,7 +141,7 @@ __device__ __forceinline__ d3 right_loc(const Lane &S, const CellRegs &c,
   double4 g = A.spos[pk_slot(ref)];
   d3 l = xyz(g);
   d3 s;
-  if (image_shift(pk_img(ref), A.box.width, &s)) l = add3(l, s);
+  if (image_shift(pk_img(ref), c.B->width, &s)) l = add3(l, s);
   return l;
 }
 
@@ -210,11 +211,11 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
       // in-sphere test on snapped coordinates (convex_cell.rs:370-382)
       uint32_t du = S.vdual[S.at(i)];
       int64_t ia[3], ib[3], ic[3], id[3], iv[3];
-      iloc(A.box, c.q, ia);
-      iloc(A.box, right_loc(S, c, A, dual_at(du, 0)), ib);
-      iloc(A.box, right_loc(S, c, A, dual_at(du, 1)), ic);
-      iloc(A.box, right_loc(S, c, A, dual_at(du, 2)), id);
-      iloc(A.box, ngb, iv);
+      iloc(*c.B, c.q, ia);
+      iloc(*c.B, right_loc(S, c, A, dual_at(du, 0)), ib);
+      iloc(*c.B, right_loc(S, c, A, dual_at(du, 1)), ic);
+      iloc(*c.B, right_loc(S, c, A, dual_at(du, 2)), id);
+      iloc(*c.B, ngb, iv);
       clipped = in_sphere_exact(ia, ib, ic, id, iv) < 0;
       c.nexact++;
     } else {
@@ -290,7 +291,7 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
   for (int t = 0; t < c.clen; t++) {
     int next = S.cyc[S.at(cur)];
     double4 nvtx;
-    if (!make_vertex(S, c, A.box.dim, cur, next, p_idx, &nvtx))
+    if (!make_vertex(S, c, c.B->dim, cur, next, p_idx, &nvtx))
       return E_DET_ZERO;
     if (isnan(nvtx.w)) return E_NAN;
     newmax = fmax(newmax, nvtx.w);
@@ -310,7 +311,7 @@ __device__ __forceinline__ bool should_construct(const Lane &S,
                                                  const CellRegs &c,
                                                  const ClipArgs &A, int j) {
   double4 a = S.plA[S.at(j)];
-  int dim = A.box.dim;
+  int dim = c.B->dim;
   if (dim == 1 && !(a.y == 0. && a.z == 0.)) return false;
   if (dim == 2 && !(a.z == 0.)) return false;
   uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(j)].w);
@@ -326,7 +327,7 @@ template <int MAXP, int MAXV>
 __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
                               uint32_t w, uint32_t *nf, uint32_t *npop,
                               uint32_t *err) {
-  const BoxParams &B = A.box;
+  const BoxParams &B = *c.B;
   *nf = 0;
   *npop = 0;
   // ConvexCell::init (convex_cell.rs:294-312)
@@ -395,7 +396,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
       return CS_ERROR;
     }
   }
-  if (!stopped && (uint64_t)cnt < A.total_stream) {
+  if (!stopped && (uint64_t)cnt < c.total_stream) {
     // list exhausted: the stop test on the first entry left out
     if (!(c.sr < A.kdist[w])) return CS_NEED_MORE;
     stopped = true;
@@ -538,6 +539,9 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
     CellRegs c;
     if (w < A.nwork) {
       const uint32_t slot = A.work[w];
+      const uint32_t pr = problem_of(A.PR, slot);
+      c.B = &A.PR.boxes[pr];
+      c.total_stream = A.PR.grids[pr].total_stream;
       const double4 qq = A.spos[slot];
       c.q = xyz(qq);
       c.idx = (uint64_t)__double_as_longlong(qq.w);

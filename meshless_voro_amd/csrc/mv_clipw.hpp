@@ -50,8 +50,7 @@ struct WaveLds {
 struct WaveArgs {
   const double4 *spos;
   const uint8_t *mask;
-  BoxParams box;
-  uint64_t total_stream;
+  Problems PR;            // boxes, grids (total stream) per problem
   const uint32_t *work;
   uint32_t nwork;
   const uint64_t *kpk;    // neighbour lists
@@ -123,7 +122,7 @@ __device__ __forceinline__ void lds_fence() {
 
 // HalfSpace::right_loc of a stored plane (half_space.rs:53-65)
 __device__ __forceinline__ d3 w_right_loc(const WaveLds &S, const WaveArgs &A,
-                                          d3 q, int j) {
+                                          const BoxParams &B, d3 q, int j) {
   double4 b = S.pb[j];
   uint64_t ref = (uint64_t)__double_as_longlong(b.w);
   if (ref == BOX_REF) {
@@ -133,7 +132,7 @@ __device__ __forceinline__ d3 w_right_loc(const WaveLds &S, const WaveArgs &A,
   }
   d3 l = xyz(A.spos[pk_slot(ref)]);
   d3 s;
-  if (image_shift(pk_img(ref), A.box.width, &s)) l = add3(l, s);
+  if (image_shift(pk_img(ref), B.width, &s)) l = add3(l, s);
   return l;
 }
 
@@ -186,8 +185,6 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
   __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
   const int lane = (int)__lane_id();
-  const BoxParams &B = A.box;
-  const int dim = B.dim;
   unsigned long long my_exact = 0;
 
   uint32_t next = 0;
@@ -200,7 +197,12 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     uint32_t nx = 0;
     if (lane == 0) nx = atomicAdd(A.queue, 1u);
 
-    const double4 qq = A.spos[A.work[w]];
+    const uint32_t wslot = __builtin_amdgcn_readfirstlane(A.work[w]);
+    const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(A.PR, wslot));
+    const BoxParams &B = A.PR.boxes[pr];
+    const int dim = B.dim;
+    const uint64_t total_stream = A.PR.grids[pr].total_stream;
+    const double4 qq = A.spos[wslot];
     const d3 q = xyz(qq);
     const uint64_t idx = (uint64_t)__double_as_longlong(qq.w);
     const uint32_t cnt = A.kcnt[w];
@@ -308,9 +310,9 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           if (near) {
             int64_t ia[3], ib[3], ic[3], id[3], iv[3];
             iloc(B, q, ia);
-            iloc(B, w_right_loc(S, A, q, vd & 0xFF), ib);
-            iloc(B, w_right_loc(S, A, q, (vd >> 8) & 0xFF), ic);
-            iloc(B, w_right_loc(S, A, q, (vd >> 16) & 0xFF), id);
+            iloc(B, w_right_loc(S, A, B, q, vd & 0xFF), ib);
+            iloc(B, w_right_loc(S, A, B, q, (vd >> 8) & 0xFF), ic);
+            iloc(B, w_right_loc(S, A, B, q, (vd >> 16) & 0xFF), id);
             iloc(B, ngb, iv);
             clipped = in_sphere_exact(ia, ib, ic, id, iv) < 0;
             nexact++;
@@ -462,7 +464,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     if (state == CS_DONE && !stopped) {
       // list exhausted: the first entry left out decides (its distance
       // comes with the list), else the whole stream was used
-      if ((uint64_t)cnt >= A.total_stream)
+      if ((uint64_t)cnt >= total_stream)
         npop = cnt;
       else if (sr < A.kdist[w])
         npop = cnt + 1;

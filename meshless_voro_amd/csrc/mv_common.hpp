@@ -96,6 +96,19 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v,
   return x - v;
 }
 
+// Per-problem tables of a build over many independent tessellations (one
+// entry for a single build).  Slots and generator indices are global over
+// the concatenated problems; prob_of_slot == nullptr means one problem.
+struct Problems {
+  const BoxParams *boxes;
+  const GridParams *grids;
+  const uint32_t *coff;          // first grid cell of each problem
+  const uint32_t *prob_of_slot;  // problem of each sorted slot (or nullptr)
+};
+__device__ __forceinline__ uint32_t problem_of(const Problems &P, uint32_t slot) {
+  return P.prob_of_slot ? P.prob_of_slot[slot] : 0u;
+}
+
 // Kept cells (VoronoiIntegrator): every completed cell's half-spaces and
 // vertices, appended to two arenas by atomic cursors and indexed by
 // generator.  pl_a == nullptr: nothing is kept.

@@ -20,7 +20,8 @@ struct FinalArgs {
   uint64_t n_staged;
   const uint64_t *cell_fbase;
   const uint64_t *face_off;  // exclusive scan of cell_nf
-  double width[3];
+  const BoxParams *boxes;    // per problem (periodic width for the shifts)
+  const uint32_t *prob;      // problem of each generator (nullptr: one problem)
   // final faces
   uint64_t *face_left;
   int64_t *face_right;
@@ -44,7 +45,8 @@ __global__ void k_place_faces(FinalArgs A) {
     A.face_normal[3 * f + k] = A.st_norm[3 * o + k];
   }
   d3 s;
-  bool has = right >= 0 && image_shift(img, A.width, &s);
+  const double *width = A.boxes[A.prob ? A.prob[c] : 0u].width;
+  bool has = right >= 0 && image_shift(img, width, &s);
   if (!has) s = mk3(0, 0, 0);
   A.face_has_shift[f] = has ? 1 : 0;
   A.face_shift[3 * f] = s.x;
@@ -99,6 +101,8 @@ __global__ void k_sort_links(const uint64_t *__restrict__ link_off,
 // VoronoiCell records: loc and idx of active cells (Default for inactive)
 __global__ void k_cell_records(const double4 *__restrict__ gpos, size_t n,
                                const uint8_t *__restrict__ mask,
+                               const uint64_t *__restrict__ goff,
+                               const uint32_t *__restrict__ prob,
                                double *__restrict__ cell_loc,
                                uint64_t *__restrict__ cell_idx) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,7 +112,47 @@ __global__ void k_cell_records(const double4 *__restrict__ gpos, size_t n,
   cell_loc[3 * i] = act ? p.x : 0.;
   cell_loc[3 * i + 1] = act ? p.y : 0.;
   cell_loc[3 * i + 2] = act ? p.z : 0.;
-  cell_idx[i] = act ? (uint64_t)i : 0;
+  const uint64_t base = prob ? goff[prob[i]] : 0ull;
+  cell_idx[i] = act ? (uint64_t)i - base : 0;
+}
+
+// Many problems: every index above is global over the concatenated
+// problems; these turn each problem's faces and links into the ones a
+// build of that problem alone returns (generator indices, face indices and
+// link offsets relative to the problem's first generator, face and link).
+__global__ void k_problem_bases(const uint64_t *__restrict__ goff, uint32_t P,
+                                const uint64_t *__restrict__ face_off, uint64_t F,
+                                const uint64_t *__restrict__ link_off, uint64_t L,
+                                size_t n, uint64_t *__restrict__ foff,
+                                uint64_t *__restrict__ loff) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > P) return;
+  const uint64_t g = goff[p];
+  foff[p] = g < n ? face_off[g] : F;
+  loff[p] = g < n ? link_off[g] : L;
+}
+__global__ void k_localize_faces(uint64_t *__restrict__ face_left,
+                                 int64_t *__restrict__ face_right, uint64_t nf,
+                                 const uint64_t *__restrict__ goff,
+                                 const uint32_t *__restrict__ prob) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const uint64_t base = goff[prob[face_left[f]]];
+  face_left[f] -= base;
+  if (face_right[f] >= 0) face_right[f] -= (int64_t)base;
+}
+__global__ void k_localize_links(uint64_t *__restrict__ link_off,
+                                 const uint64_t *__restrict__ link_cnt, size_t n,
+                                 const uint32_t *__restrict__ prob,
+                                 const uint64_t *__restrict__ foff,
+                                 const uint64_t *__restrict__ loff,
+                                 uint64_t *__restrict__ links) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const uint32_t p = prob[c];
+  const uint64_t b = link_off[c], e = b + link_cnt[c];
+  for (uint64_t i = b; i < e; i++) links[i] -= foff[p];
+  link_off[c] = b - loff[p];
 }
 
 __global__ void k_exact_batch(const int64_t *__restrict__ t, size_t n,

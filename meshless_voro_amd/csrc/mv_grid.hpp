@@ -11,69 +11,75 @@
 namespace mv {
 
 // Generator::new (src/voronoi/generator.rs:13-24): copy each seed, zero the
-// unused components; per-block bounding-box partials.
+// unused components.
 __global__ void k_prepare(const double *__restrict__ xyz, size_t n, int dim,
-                          double4 *__restrict__ gpos,
-                          double *__restrict__ bbox_part) {
-  __shared__ double smin[3][256], smax[3][256];
+                          double4 *__restrict__ gpos) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double mn[3] = {INFINITY, INFINITY, INFINITY};
-  double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  if (i < n) {
-    double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-    if (dim == 1) {
-      y = 0.;
-      z = 0.;
-    } else if (dim == 2) {
-      z = 0.;
-    }
-    gpos[i] = make_double4(x, y, z, 0.);
-    mn[0] = mx[0] = x;
-    mn[1] = mx[1] = y;
-    mn[2] = mx[2] = z;
+  if (i >= n) return;
+  double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  if (dim == 1) {
+    y = 0.;
+    z = 0.;
+  } else if (dim == 2) {
+    z = 0.;
   }
-  for (int c = 0; c < 3; c++) {
-    smin[c][threadIdx.x] = mn[c];
-    smax[c][threadIdx.x] = mx[c];
+  gpos[i] = make_double4(x, y, z, 0.);
+}
+
+// problem of every generator (one block row per problem)
+__global__ void k_problem_of(const uint64_t *__restrict__ goff, uint32_t P,
+                             uint32_t *__restrict__ prob) {
+  const uint32_t p = blockIdx.y;
+  if (p >= P) return;
+  for (uint64_t i = goff[p] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < goff[p + 1];
+       i += (uint64_t)gridDim.x * blockDim.x)
+    prob[i] = p;
+}
+
+// bounding box of each problem: block (s, p) reduces chunk s of problem p,
+// then one block per problem reduces the chunks
+constexpr uint32_t BBOX_CHUNK = 65536;
+__global__ void __launch_bounds__(256)
+    k_bbox_part(const double4 *__restrict__ gpos, const uint64_t *__restrict__ goff,
+                uint32_t S, double *__restrict__ part) {
+  __shared__ double sm[6][256];
+  const uint32_t p = blockIdx.y, sidx = blockIdx.x;
+  const uint64_t b = goff[p] + (uint64_t)sidx * BBOX_CHUNK;
+  const uint64_t e = min(goff[p + 1], b + BBOX_CHUNK);
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const double4 g = gpos[i];
+    v[0] = fmin(v[0], g.x);
+    v[1] = fmin(v[1], g.y);
+    v[2] = fmin(v[2], g.z);
+    v[3] = fmax(v[3], g.x);
+    v[4] = fmax(v[4], g.y);
+    v[5] = fmax(v[5], g.z);
   }
+  for (int c = 0; c < 6; c++) sm[c][threadIdx.x] = v[c];
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < (unsigned)s)
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < (unsigned)st)
       for (int c = 0; c < 3; c++) {
-        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + s]);
-        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + s]);
+        sm[c][threadIdx.x] = fmin(sm[c][threadIdx.x], sm[c][threadIdx.x + st]);
+        sm[3 + c][threadIdx.x] = fmax(sm[3 + c][threadIdx.x], sm[3 + c][threadIdx.x + st]);
       }
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int c = 0; c < 3; c++) {
-      bbox_part[6 * blockIdx.x + c] = smin[c][0];
-      bbox_part[6 * blockIdx.x + 3 + c] = smax[c][0];
-    }
+    for (int c = 0; c < 6; c++) part[((size_t)p * S + sidx) * 6 + c] = sm[c][0];
 }
-
-__global__ void k_bbox_final(const double *__restrict__ part, int nb,
+__global__ void k_bbox_final(const double *__restrict__ part, uint32_t S, uint32_t P,
                              double *__restrict__ out) {
-  __shared__ double s[6][256];
-  int t = threadIdx.x;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int b = t; b < nb; b += blockDim.x)
+  for (uint32_t s2 = 0; s2 < S; s2++)
     for (int c = 0; c < 3; c++) {
-      v[c] = fmin(v[c], part[6 * b + c]);
-      v[3 + c] = fmax(v[3 + c], part[6 * b + 3 + c]);
+      v[c] = fmin(v[c], part[((size_t)p * S + s2) * 6 + c]);
+      v[3 + c] = fmax(v[3 + c], part[((size_t)p * S + s2) * 6 + 3 + c]);
     }
-  for (int c = 0; c < 6; c++) s[c][t] = v[c];
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (t < st)
-      for (int c = 0; c < 3; c++) {
-        s[c][t] = fmin(s[c][t], s[c][t + st]);
-        s[3 + c][t] = fmax(s[3 + c][t], s[3 + c][t + st]);
-      }
-    __syncthreads();
-  }
-  if (t == 0)
-    for (int c = 0; c < 6; c++) out[c] = s[c][0];
+  for (int c = 0; c < 6; c++) out[6 * p + c] = v[c];
 }
 
 __device__ __forceinline__ int grid_coord(const GridParams &g, int c,
@@ -85,14 +91,19 @@ __device__ __forceinline__ int grid_coord(const GridParams &g, int c,
 }
 
 __global__ void k_grid_count(const double4 *__restrict__ gpos, size_t n,
-                             GridParams g, uint32_t *__restrict__ cellid,
+                             const GridParams *__restrict__ grids,
+                             const uint32_t *__restrict__ coff,
+                             const uint32_t *__restrict__ prob,
+                             uint32_t *__restrict__ cellid,
                              uint32_t *__restrict__ counts) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const uint32_t pr = prob ? prob[i] : 0u;
+  const GridParams &g = grids[pr];
   double4 p = gpos[i];
   int a = grid_coord(g, 0, p.x), b = grid_coord(g, 1, p.y),
       c = grid_coord(g, 2, p.z);
-  uint32_t id = (uint32_t)(((size_t)a * g.G[1] + b) * g.G[2] + c);
+  uint32_t id = coff[pr] + (uint32_t)(((size_t)a * g.G[1] + b) * g.G[2] + c);
   cellid[i] = id;
   atomicAdd(&counts[id], 1u);
 }
@@ -129,8 +140,10 @@ __global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
 // sorted positions; w carries the generator index bits
 __global__ void k_grid_gather(const double4 *__restrict__ gpos,
                               const uint32_t *__restrict__ order, size_t n,
+                              const uint32_t *__restrict__ prob,
                               double4 *__restrict__ spos,
-                              uint32_t *__restrict__ slot_of) {
+                              uint32_t *__restrict__ slot_of,
+                              uint32_t *__restrict__ prob_of_slot) {
   size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   uint32_t i = order[s];
@@ -138,6 +151,7 @@ __global__ void k_grid_gather(const double4 *__restrict__ gpos,
   p.w = __longlong_as_double((long long)i);
   spos[s] = p;
   slot_of[i] = (uint32_t)s;
+  if (prob) prob_of_slot[s] = prob[i];
 }
 
 // ---------------------------------------------------------------------
@@ -425,8 +439,8 @@ struct KnnLds {
 template <int CAP, int WPB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
-          const uint32_t *__restrict__ cell_start, GridParams g,
-          BoxParams box, const uint32_t *__restrict__ work, uint32_t nwork,
+          const uint32_t *__restrict__ cell_start_all, Problems PR,
+          const uint32_t *__restrict__ work, uint32_t nwork,
           uint32_t L, const uint32_t *__restrict__ Lper,
           const float *__restrict__ Rper, const uint64_t *__restrict__ kofs,
           uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
@@ -437,7 +451,17 @@ __global__ void __launch_bounds__(256)
   KnnLds<CAP> &buf = lds[wib];
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + wib; w < nwork; w += nwaves) {
-    const double4 qq = spos[work[w]];
+    // wave-uniform: keep them in scalar registers
+    const uint32_t slot = __builtin_amdgcn_readfirstlane(work[w]);
+    const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(PR, slot));
+    // the problem's parameters in registers (wave-uniform scalar loads)
+    const GridParams g = PR.grids[pr];
+    const BoxParams &boxr = PR.boxes[pr];
+    struct {
+      double width[3];
+    } box = {{boxr.width[0], boxr.width[1], boxr.width[2]}};
+    const uint32_t *cell_start = cell_start_all + PR.coff[pr];
+    const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
     double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
@@ -472,8 +496,8 @@ __global__ void __launch_bounds__(256)
 // +inf keys) and writes the first L.
 __global__ void __launch_bounds__(256)
     k_knn_big(const double4 *__restrict__ spos,
-              const uint32_t *__restrict__ cell_start, GridParams g,
-              BoxParams box, const uint32_t *__restrict__ work,
+              const uint32_t *__restrict__ cell_start_all, Problems PR,
+              const uint32_t *__restrict__ work,
               uint32_t nwork, uint32_t L, const uint32_t *__restrict__ Lper,
               const uint64_t *__restrict__ kofs, uint64_t *__restrict__ kpk,
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
@@ -485,7 +509,17 @@ __global__ void __launch_bounds__(256)
   double *bkey = gkey + (size_t)wid * cap;
   uint64_t *bpk = gpk + (size_t)wid * cap;
   for (uint32_t w = wid; w < nwork; w += nwaves) {
-    const double4 qq = spos[work[w]];
+    // wave-uniform: keep them in scalar registers
+    const uint32_t slot = __builtin_amdgcn_readfirstlane(work[w]);
+    const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(PR, slot));
+    // the problem's parameters in registers (wave-uniform scalar loads)
+    const GridParams g = PR.grids[pr];
+    const BoxParams &boxr = PR.boxes[pr];
+    struct {
+      double width[3];
+    } box = {{boxr.width[0], boxr.width[1], boxr.width[2]}};
+    const uint32_t *cell_start = cell_start_all + PR.coff[pr];
+    const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
     double R = knn_start_radius(cell_start, g, q, Lw);

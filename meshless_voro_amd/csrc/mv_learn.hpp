@@ -53,13 +53,18 @@ struct FeatParams {
 // hloc[w] = local spacing (block volume / block count)^(1/nd).
 __global__ void __launch_bounds__(256)
     k_features(const double4 *__restrict__ spos,
-               const uint32_t *__restrict__ cell_start, GridParams g,
-               BoxParams box, FeatParams fp, const uint32_t *__restrict__ work,
+               const uint32_t *__restrict__ cell_start_all, Problems PR,
+               const FeatParams *__restrict__ fps, const uint32_t *__restrict__ work,
                uint32_t nwork, float *__restrict__ F,
                float *__restrict__ hloc) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork) return;
   const uint32_t slot = work[w];
+  const uint32_t pr = problem_of(PR, slot);
+  const GridParams &g = PR.grids[pr];
+  const BoxParams &box = PR.boxes[pr];
+  const FeatParams &fp = fps[pr];
+  const uint32_t *cell_start = cell_start_all + PR.coff[pr];
   const double4 qq = spos[slot];
   const double q[3] = {qq.x, qq.y, qq.z};
   int C[3], nd = 0;

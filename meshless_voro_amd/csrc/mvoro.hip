@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -200,11 +201,19 @@ constexpr int MAXP_BIG = 255, MAXV_BIG = 1024;
 // ---------------------------------------------------------------------
 // the handle
 // ---------------------------------------------------------------------
+// Device allocations of a build; shared by the handles of a batched build.
+struct DevStore {
+  std::vector<void *> dev;
+  ~DevStore() {
+    for (void *p : dev) dfree(p);
+  }
+};
+
 struct mv_tess {
   size_t n = 0, nf = 0, nl = 0;
   double anchor[3], width[3];
   uint32_t dim = 3, periodic = 0;
-  std::vector<void *> dev;  // device allocations owned by the handle
+  std::shared_ptr<DevStore> store = std::make_shared<DevStore>();
   double *d_cell_loc = nullptr, *d_cell_cen = nullptr, *d_cell_vol = nullptr,
          *d_cell_sr = nullptr;
   uint64_t *d_cell_foff = nullptr, *d_cell_fcnt = nullptr,
@@ -220,6 +229,7 @@ struct mv_tess {
   bool keep = false;
   CellStore cs{};
   uint64_t used_pl = 0, used_vx = 0, n_active = 0;
+  uint64_t idx_base = 0;           // first generator of this problem in the store
   uint8_t *d_mask = nullptr;       // copy of the build's mask (symmetric faces)
   uint64_t *d_active = nullptr;    // active cells in generator order
   // host copies
@@ -239,18 +249,15 @@ struct mv_tess {
   T *own(size_t n) {
     T *p = dnew<T>(n);
     if (!p) throw BuildError{MV_E_OOM};
-    dev.push_back(p);
+    store->dev.push_back(p);
     return p;
   }
   void release(void *p) {
-    auto it = std::find(dev.begin(), dev.end(), p);
-    if (it != dev.end()) {
-      dev.erase(it);
+    auto it = std::find(store->dev.begin(), store->dev.end(), p);
+    if (it != store->dev.end()) {
+      store->dev.erase(it);
       dfree(p);
     }
-  }
-  ~mv_tess() {
-    for (void *p : dev) dfree(p);
   }
 };
 
@@ -377,8 +384,7 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
 constexpr uint32_t BIG_KNN_CAP = 65536;
 constexpr uint32_t BIG_KNN_WAVES = 1024;
 
-void launch_knn(const double4 *spos, const uint32_t *cell_start,
-                const GridParams &grid, const BoxParams &box,
+void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems &PR,
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
                 uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
@@ -386,97 +392,178 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start,
   if (cnt == 0) return;
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else if (!big && L <= 1024) {
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
-                       cell_start, grid, box, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else {
     uint32_t waves = std::min<uint32_t>(cnt, BIG_KNN_WAVES);
     waves = (waves + 3) / 4 * 4;
     double *gkey = need(TT.get<double>((size_t)waves * BIG_KNN_CAP));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * BIG_KNN_CAP));
-    hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, grid,
-                       box, wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, BIG_KNN_CAP);
+    hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, PR,
+                       wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, BIG_KNN_CAP);
   }
   HIPCHK(hipGetLastError());
 }
 
-// The counting-sort grid of a build (device buffers owned by `TT`).
+// One problem of a build: its box after the 1D/2D normalisation
+// (src/voronoi.rs:200-210) and its generators [g0, g1) of the concatenation.
+struct ProblemSpec {
+  double anchor[3], width[3];
+  uint32_t periodic;
+  uint64_t g0, g1;
+};
+
+void normalise_box(const double a_in[3], const double w_in[3], uint32_t dim, double a[3],
+                   double w[3]) {
+  for (int c = 0; c < 3; c++) {
+    a[c] = a_in[c];
+    w[c] = w_in[c];
+  }
+  if (dim == 1) {
+    a[1] = -0.5;
+    w[1] = 1.;
+  }
+  if (dim == 1 || dim == 2) {
+    a[2] = -0.5;
+    w[2] = 1.;
+  }
+}
+
+// The counting-sort grids of all problems of a build (device buffers owned
+// by `TT`): problem p owns grid cells [coff[p], coff[p+1]) and therefore
+// sorted slots [g0, g1).
 struct GridData {
-  double anchor[3], width[3];  // after the 1D/2D normalisation
-  double bb[6];
-  BoxParams box;
-  GridParams grid;
+  uint32_t P = 1;
+  std::vector<BoxParams> boxes;
+  std::vector<GridParams> grids;
+  std::vector<uint32_t> coff;
   size_t ncell = 0;
   double4 *gpos = nullptr, *spos = nullptr;
   uint32_t *cell_start = nullptr, *order = nullptr, *slot_of = nullptr;
+  uint32_t *prob = nullptr, *prob_of_slot = nullptr;  // nullptr when P == 1
+  uint64_t *d_goff = nullptr;
+  BoxParams *d_boxes = nullptr;
+  GridParams *d_grids = nullptr;
+  uint32_t *d_coff = nullptr;
+  Problems PR() const { return Problems{d_boxes, d_grids, d_coff, prob_of_slot}; }
 };
 
 void build_grid(GridData &D, const double *d_xyz, size_t n,
-                const double anchor_in[3], const double width_in[3],
-                uint32_t dim, uint32_t periodic, double occ, Temps &TT,
+                const std::vector<ProblemSpec> &probs, uint32_t dim, double occ, Temps &TT,
                 hipStream_t s) {
-  // voronoi.rs:200-210
-  for (int c = 0; c < 3; c++) {
-    D.anchor[c] = anchor_in[c];
-    D.width[c] = width_in[c];
+  const uint32_t P = (uint32_t)probs.size();
+  D.P = P;
+  std::vector<uint64_t> goff(P + 1);
+  uint64_t maxn = 1;
+  for (uint32_t p = 0; p < P; p++) {
+    goff[p] = probs[p].g0;
+    maxn = std::max<uint64_t>(maxn, probs[p].g1 - probs[p].g0);
   }
-  if (dim == 1) {
-    D.anchor[1] = -0.5;
-    D.width[1] = 1.;
-  }
-  if (dim == 1 || dim == 2) {
-    D.anchor[2] = -0.5;
-    D.width[2] = 1.;
-  }
-  D.box = make_box(D.anchor, D.width, (int)periodic, (int)dim);
+  goff[P] = n;
+  D.d_goff = need(TT.get<uint64_t>(P + 1));
+  HIPCHK(hipMemcpyAsync(D.d_goff, goff.data(), (P + 1) * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, s));
   D.gpos = need(TT.get<double4>(n));
   const unsigned nb = nblk(n);
-  double *bbp = need(TT.get<double>(6 * (size_t)nb));
-  double *bbd = need(TT.get<double>(6));
-  hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, s, d_xyz, n, (int)dim, D.gpos, bbp);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, bbp, (int)nb, bbd);
+  hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, s, d_xyz, n, (int)dim, D.gpos);
+  if (P > 1) {
+    D.prob = need(TT.get<uint32_t>(n));
+    D.prob_of_slot = need(TT.get<uint32_t>(n));
+    const uint32_t gx = (uint32_t)std::min<uint64_t>((maxn + 255) / 256, 64);
+    hipLaunchKernelGGL(k_problem_of, dim3(gx, P), dim3(256), 0, s, (const uint64_t *)D.d_goff,
+                       P, D.prob);
+  }
+  // bounding boxes
+  const uint32_t S = (uint32_t)((maxn + BBOX_CHUNK - 1) / BBOX_CHUNK);
+  double *bbp = need(TT.get<double>(6 * (size_t)S * P));
+  double *bbd = need(TT.get<double>(6 * (size_t)P));
+  hipLaunchKernelGGL(k_bbox_part, dim3(S, P), dim3(256), 0, s, (const double4 *)D.gpos,
+                     (const uint64_t *)D.d_goff, S, bbp);
+  hipLaunchKernelGGL(k_bbox_final, dim3((P + 63) / 64), dim3(64), 0, s, (const double *)bbp, S,
+                     P, bbd);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(D.bb, bbd, sizeof D.bb, hipMemcpyDeviceToHost, s));
+  std::vector<double> bb(6 * (size_t)P);
+  HIPCHK(hipMemcpyAsync(bb.data(), bbd, bb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  for (int c = 0; c < 6; c++)
-    if (!std::isfinite(D.bb[c])) throw BuildError{MV_E_DEGENERATE_POINTS};
-  D.grid = make_grid(D.bb, n, D.width, (int)dim, (int)periodic, occ);
-  D.ncell = (size_t)D.grid.G[0] * D.grid.G[1] * D.grid.G[2];
+  D.boxes.resize(P);
+  D.grids.resize(P);
+  D.coff.resize(P + 1);
+  size_t ncell = 0;
+  for (uint32_t p = 0; p < P; p++) {
+    const uint64_t np_ = probs[p].g1 - probs[p].g0;
+    D.boxes[p] = make_box(probs[p].anchor, probs[p].width, (int)probs[p].periodic, (int)dim);
+    double b6[6];
+    for (int c = 0; c < 6; c++) {
+      b6[c] = bb[6 * p + c];
+      if (np_ && !std::isfinite(b6[c])) throw BuildError{MV_E_DEGENERATE_POINTS};
+      if (!np_) b6[c] = 0.;
+    }
+    D.grids[p] = make_grid(b6, np_ ? np_ : 1, probs[p].width, (int)dim, (int)probs[p].periodic,
+                           occ);
+    // total stream of this problem only
+    uint64_t nimg = (uint64_t)(D.grids[p].img_hi[0] - D.grids[p].img_lo[0] + 1) *
+                    (D.grids[p].img_hi[1] - D.grids[p].img_lo[1] + 1) *
+                    (D.grids[p].img_hi[2] - D.grids[p].img_lo[2] + 1);
+    D.grids[p].total_stream = np_ * nimg;
+    D.coff[p] = (uint32_t)ncell;
+    ncell += (size_t)D.grids[p].G[0] * D.grids[p].G[1] * D.grids[p].G[2];
+    if (ncell >= (1ull << 32) - 1) throw BuildError{MV_E_CAPACITY};
+  }
+  D.coff[P] = (uint32_t)ncell;
+  D.ncell = ncell;
+  D.d_boxes = need(TT.get<BoxParams>(P));
+  D.d_grids = need(TT.get<GridParams>(P));
+  D.d_coff = need(TT.get<uint32_t>(P + 1));
+  HIPCHK(hipMemcpyAsync(D.d_boxes, D.boxes.data(), P * sizeof(BoxParams), hipMemcpyHostToDevice,
+                        s));
+  HIPCHK(hipMemcpyAsync(D.d_grids, D.grids.data(), P * sizeof(GridParams),
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(D.d_coff, D.coff.data(), (P + 1) * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
   uint32_t *cellid = need(TT.get<uint32_t>(n));
-  uint32_t *counts = need(TT.get<uint32_t>(D.ncell + 1));
-  uint32_t *fill = need(TT.get<uint32_t>(D.ncell));
-  D.cell_start = need(TT.get<uint32_t>(D.ncell + 1));
+  uint32_t *counts = need(TT.get<uint32_t>(ncell + 1));
+  uint32_t *fill = need(TT.get<uint32_t>(ncell));
+  D.cell_start = need(TT.get<uint32_t>(ncell + 1));
   D.order = need(TT.get<uint32_t>(n));
   D.spos = need(TT.get<double4>(n));
   D.slot_of = need(TT.get<uint32_t>(n));
-  HIPCHK(hipMemsetAsync(counts, 0, (D.ncell + 1) * sizeof(uint32_t), s));
-  HIPCHK(hipMemsetAsync(fill, 0, D.ncell * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, s, D.gpos, n, D.grid, cellid, counts);
-  scan_excl<uint32_t, uint32_t>(counts, D.cell_start, D.ncell + 1, s);
+  HIPCHK(hipMemsetAsync(counts, 0, (ncell + 1) * sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(fill, 0, ncell * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos, n,
+                     (const GridParams *)D.d_grids, (const uint32_t *)D.d_coff,
+                     (const uint32_t *)D.prob, cellid, counts);
+  scan_excl<uint32_t, uint32_t>(counts, D.cell_start, ncell + 1, s);
   hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, s, cellid, n, D.cell_start, fill,
                      D.order);
-  hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(D.ncell)), dim3(256), 0, s, D.cell_start,
-                     D.ncell, D.order);
-  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, D.gpos, D.order, n, D.spos,
-                     D.slot_of);
+  hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(ncell)), dim3(256), 0, s, D.cell_start,
+                     ncell, D.order);
+  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos,
+                     (const uint32_t *)D.order, n, (const uint32_t *)D.prob, D.spos, D.slot_of,
+                     D.prob_of_slot);
   HIPCHK(hipGetLastError());
 }
 
-FeatParams feat_params(const GridData &D, size_t n, int dim) {
-  FeatParams fp;
-  double vol = 1.;
-  int nd = 0;
-  for (int c = 0; c < 3; c++) {
-    fp.box_lo[c] = D.anchor[c];
-    fp.box_hi[c] = D.anchor[c] + D.width[c];
-    if (c < dim) {
-      vol *= D.width[c];
-      nd++;
+// feature normalisation of each problem: its box and mean spacing
+std::vector<FeatParams> feat_params(const std::vector<ProblemSpec> &probs, int dim) {
+  std::vector<FeatParams> out(probs.size());
+  for (size_t p = 0; p < probs.size(); p++) {
+    FeatParams &fp = out[p];
+    double vol = 1.;
+    int nd = 0;
+    for (int c = 0; c < 3; c++) {
+      fp.box_lo[c] = probs[p].anchor[c];
+      fp.box_hi[c] = probs[p].anchor[c] + probs[p].width[c];
+      if (c < dim) {
+        vol *= probs[p].width[c];
+        nd++;
+      }
     }
+    const double np_ = (double)std::max<uint64_t>(probs[p].g1 - probs[p].g0, 1);
+    fp.h_global = std::pow(vol / np_, 1.0 / std::max(nd, 1));
   }
-  fp.h_global = std::pow(vol / (double)std::max<size_t>(n, 1), 1.0 / std::max(nd, 1));
-  return fp;
+  return out;
 }
 
 void upload_model(const mv_model *m, DevModel *d, hipStream_t s) {
@@ -497,14 +584,21 @@ void upload_model(const mv_model *m, DevModel *d, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
 }
 
-void build_impl(mv_tess *T, const double *d_xyz, size_t n,
-                const uint8_t *d_mask, const double anchor_in[3],
-                const double width_in[3], uint32_t dim, uint32_t periodic,
-                const mv_options &opt) {
+// One build over P independent problems (P = 1: Voronoi::build /
+// build_partial).  Ts[p] receives problem p; Ts[0] owns the device memory,
+// which the other handles share.  All kernels run once over the
+// concatenation: slots, generator indices and faces are global during the
+// build and made problem-relative at the end.
+void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n,
+                 const uint8_t *d_mask, const std::vector<ProblemSpec> &probs, uint32_t dim,
+                 const mv_options &opt) {
   auto t_start = std::chrono::steady_clock::now();
   hipStream_t s = 0;
+  mv_tess *T = Ts[0];
+  const uint32_t P = (uint32_t)probs.size();
+  for (uint32_t p = 1; p < P; p++) Ts[p]->store = T->store;
   T->dim = dim;
-  T->periodic = periodic;
+  T->periodic = probs[0].periodic;
   T->n = n;
   size_t nn = n ? n : 1;
   T->d_cell_loc = T->own<double>(3 * nn);
@@ -519,19 +613,9 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   HIPCHK(hipMemsetAsync(T->d_cell_vol, 0, nn * sizeof(double), s));
   HIPCHK(hipMemsetAsync(T->d_cell_sr, 0, nn * sizeof(double), s));
   HIPCHK(hipMemsetAsync(T->d_npop, 0, nn * sizeof(uint32_t), s));
-  {
-    double a[3] = {anchor_in[0], anchor_in[1], anchor_in[2]};
-    double w[3] = {width_in[0], width_in[1], width_in[2]};
-    if (dim == 1) {
-      a[1] = -0.5;
-      w[1] = 1.;
-    }
-    if (dim == 1 || dim == 2) {
-      a[2] = -0.5;
-      w[2] = 1.;
-    }
-    memcpy(T->anchor, a, sizeof a);
-    memcpy(T->width, w, sizeof w);
+  for (uint32_t p = 0; p < P; p++) {
+    memcpy(Ts[p]->anchor, probs[p].anchor, sizeof T->anchor);
+    memcpy(Ts[p]->width, probs[p].width, sizeof T->width);
   }
   if (n == 0) {
     T->d_face_left = T->own<uint64_t>(1);
@@ -543,6 +627,19 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     T->d_face_has_shift = T->own<uint8_t>(1);
     T->d_links = T->own<uint64_t>(1);
     HIPCHK(hipStreamSynchronize(s));
+    for (uint32_t p = 1; p < P; p++) {
+      mv_tess *U = Ts[p];
+      U->dim = dim;
+      U->periodic = probs[p].periodic;
+      U->d_cell_loc = T->d_cell_loc; U->d_cell_cen = T->d_cell_cen;
+      U->d_cell_vol = T->d_cell_vol; U->d_cell_sr = T->d_cell_sr;
+      U->d_cell_foff = T->d_cell_foff; U->d_cell_fcnt = T->d_cell_fcnt;
+      U->d_cell_idx = T->d_cell_idx; U->d_npop = T->d_npop;
+      U->d_face_left = T->d_face_left; U->d_face_right = T->d_face_right;
+      U->d_face_area = T->d_face_area; U->d_face_cen = T->d_face_cen;
+      U->d_face_norm = T->d_face_norm; U->d_face_shift = T->d_face_shift;
+      U->d_face_has_shift = T->d_face_has_shift; U->d_links = T->d_links;
+    }
     return;
   }
 
@@ -551,9 +648,8 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
   Timer tg(s);
   GridData D;
   const double occ = opt.grid_occupancy16 ? (double)opt.grid_occupancy16 / 16.0 : 2.0;
-  build_grid(D, d_xyz, n, anchor_in, width_in, dim, periodic, occ, TT, s);
-  const BoxParams &box = D.box;
-  const GridParams &grid = D.grid;
+  build_grid(D, d_xyz, n, probs, dim, occ, TT, s);
+  const Problems PR = D.PR();
   const double4 *spos = D.spos;
   const uint32_t *cell_start = D.cell_start;
   const unsigned nb = nblk(n);
@@ -590,8 +686,11 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     float *Y = need(TT.get<float>((size_t)nwork * NO));
     DevModel *dm = need(TT.get<DevModel>(1));
     upload_model(opt.model, dm, s);
-    hipLaunchKernelGGL(k_features, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start, grid,
-                       box, feat_params(D, n, (int)dim), lists[0], nwork, F, hl);
+    const std::vector<FeatParams> fph = feat_params(probs, (int)dim);
+    FeatParams *fpd = need(TT.get<FeatParams>(P));
+    HIPCHK(hipMemcpyAsync(fpd, fph.data(), P * sizeof(FeatParams), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_features, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start, PR,
+                       (const FeatParams *)fpd, lists[0], nwork, F, hl);
     const uint32_t tiles = (nwork + 15) / 16;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
                        0, s, dm, F, nwork, Y);
@@ -743,8 +842,8 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
         Timer tkk(s);
-        launch_knn(spos, cell_start, grid, box, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist,
-                   big, TT, s);
+        launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
+                   s);
         ms_knn_k += tkk.stop();
         n_knn++;
         q_knn += cnt;
@@ -756,8 +855,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
           memset(&W, 0, sizeof W);
           W.spos = spos;
           W.mask = d_mask;
-          W.box = box;
-          W.total_stream = grid.total_stream;
+          W.PR = PR;
           W.work = wl;
           W.nwork = cnt;
           W.kpk = kpk;
@@ -802,8 +900,7 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
         memset(&A, 0, sizeof A);
         A.spos = spos;
         A.mask = d_mask;
-        A.box = box;
-        A.total_stream = grid.total_stream;
+        A.PR = PR;
         A.work = wl;
         A.nwork = cnt;
         A.kpk = kpk;
@@ -988,7 +1085,8 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     FA.n_staged = F;
     FA.cell_fbase = cell_fbase;
     FA.face_off = face_off;
-    for (int c = 0; c < 3; c++) FA.width[c] = D.width[c];
+    FA.boxes = D.d_boxes;
+    FA.prob = D.prob;
     FA.face_left = T->d_face_left;
     FA.face_right = T->d_face_right;
     FA.face_area = T->d_face_area;
@@ -1013,13 +1111,87 @@ void build_impl(mv_tess *T, const double *d_xyz, size_t n,
     hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff,
                        T->d_cell_fcnt, n, T->d_links);
   }
-  hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, D.gpos, n, d_mask,
+  hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos, n,
+                     d_mask, (const uint64_t *)D.d_goff, (const uint32_t *)D.prob,
                      T->d_cell_loc, T->d_cell_idx);
   HIPCHK(hipGetLastError());
+  // many problems: per-problem face, link and generator numbering
+  std::vector<uint64_t> foff(P + 1, 0), loff(P + 1, 0);
+  if (P > 1) {
+    uint64_t *d_foff = need(TT.get<uint64_t>(P + 1));
+    uint64_t *d_loff = need(TT.get<uint64_t>(P + 1));
+    hipLaunchKernelGGL(k_problem_bases, dim3((P + 1 + 63) / 64), dim3(64), 0, s,
+                       (const uint64_t *)D.d_goff, P, (const uint64_t *)face_off, F,
+                       (const uint64_t *)T->d_cell_foff, NL, n, d_foff, d_loff);
+    HIPCHK(hipMemcpyAsync(foff.data(), d_foff, (P + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(loff.data(), d_loff, (P + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    if (F)
+      hipLaunchKernelGGL(k_localize_faces, dim3(nblk(F)), dim3(256), 0, s, T->d_face_left,
+                         T->d_face_right, F, (const uint64_t *)D.d_goff,
+                         (const uint32_t *)D.prob);
+    hipLaunchKernelGGL(k_localize_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff,
+                       (const uint64_t *)T->d_cell_fcnt, n, (const uint32_t *)D.prob,
+                       (const uint64_t *)d_foff, (const uint64_t *)d_loff, T->d_links);
+    HIPCHK(hipGetLastError());
+  } else {
+    foff[1] = F;
+    loff[1] = NL;
+  }
   T->stats.ms_finalize = tf.stop();
   HIPCHK(hipStreamSynchronize(s));
   auto t_end = std::chrono::steady_clock::now();
   T->stats.ms_total = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+  // the handles: problem p sees its slices
+  double *cl = T->d_cell_loc, *cc = T->d_cell_cen, *cv = T->d_cell_vol, *csr = T->d_cell_sr;
+  uint64_t *cfo = T->d_cell_foff, *cfc = T->d_cell_fcnt, *cid = T->d_cell_idx;
+  uint32_t *npp = T->d_npop;
+  uint64_t *fl = T->d_face_left;
+  int64_t *fr = T->d_face_right;
+  double *fa = T->d_face_area, *fc = T->d_face_cen, *fn = T->d_face_norm, *fs = T->d_face_shift;
+  uint8_t *fh = T->d_face_has_shift;
+  uint64_t *lk = T->d_links;
+  const CellStore cs = T->cs;
+  uint8_t *km = T->d_mask;
+  for (uint32_t p = 0; p < P; p++) {
+    mv_tess *U = Ts[p];
+    const uint64_t g0 = probs[p].g0;
+    U->dim = dim;
+    U->periodic = probs[p].periodic;
+    U->n = probs[p].g1 - g0;
+    U->nf = foff[p + 1] - foff[p];
+    U->nl = loff[p + 1] - loff[p];
+    U->d_cell_loc = cl + 3 * g0;
+    U->d_cell_cen = cc + 3 * g0;
+    U->d_cell_vol = cv + g0;
+    U->d_cell_sr = csr + g0;
+    U->d_cell_foff = cfo + g0;
+    U->d_cell_fcnt = cfc + g0;
+    U->d_cell_idx = cid + g0;
+    U->d_npop = npp + g0;
+    U->d_face_left = fl + foff[p];
+    U->d_face_right = fr + foff[p];
+    U->d_face_area = fa + foff[p];
+    U->d_face_cen = fc + 3 * foff[p];
+    U->d_face_norm = fn + 3 * foff[p];
+    U->d_face_shift = fs + 3 * foff[p];
+    U->d_face_has_shift = fh + foff[p];
+    U->d_links = lk + loff[p];
+    U->stats = T->stats;
+    U->keep = T->keep;
+    if (T->keep) {
+      U->cs = cs;
+      U->cs.pbase = cs.pbase + g0;
+      U->cs.vbase = cs.vbase + g0;
+      U->cs.np = cs.np + g0;
+      U->cs.nv = cs.nv + g0;
+      U->used_pl = T->used_pl;
+      U->used_vx = T->used_vx;
+      U->idx_base = g0;
+      U->d_mask = km ? km + g0 : nullptr;
+    }
+  }
 }
 
 bool valid_args(size_t n, uint32_t dim, const double *anchor,
@@ -1044,7 +1216,12 @@ mv_status run_build(const double *d_xyz, size_t n, const uint8_t *d_mask,
   mv_tess *T = new (std::nothrow) mv_tess();
   if (!T) return MV_E_OOM;
   try {
-    build_impl(T, d_xyz, n, d_mask, anchor, width, dim, periodic ? 1 : 0, o);
+    std::vector<ProblemSpec> probs(1);
+    normalise_box(anchor, width, dim, probs[0].anchor, probs[0].width);
+    probs[0].periodic = periodic ? 1 : 0;
+    probs[0].g0 = 0;
+    probs[0].g1 = n;
+    build_multi({T}, d_xyz, n, d_mask, probs, dim, o);
   } catch (const BuildError &e) {
     (void)hipDeviceSynchronize();
     (void)hipGetLastError();
@@ -1196,6 +1373,54 @@ mv_status guarded(F &&f) {
   return MV_OK;
 }
 
+// many problems in one build
+mv_status run_build_batch(const double *d_xyz, const uint64_t *offsets, uint32_t P,
+                          const uint8_t *d_mask, const double *anchors, const double *widths,
+                          const uint32_t *periodic, uint32_t dim, const mv_options *opt,
+                          mv_tess **out) {
+  mv_options o;
+  mv_default_options(&o);
+  if (opt) o = *opt;
+  if (o.mode != MV_MODE_PLAIN && o.mode != MV_MODE_LEARNED) return MV_E_ARG;
+  if (o.mode == MV_MODE_LEARNED &&
+      (!o.model || o.model->magic != MV_MODEL_MAGIC || o.model->version != 1))
+    return MV_E_ARG;
+  std::vector<mv_tess *> Ts(P, nullptr);
+  for (uint32_t p = 0; p < P; p++) {
+    Ts[p] = new (std::nothrow) mv_tess();
+    if (!Ts[p]) {
+      for (mv_tess *t : Ts) delete t;
+      return MV_E_OOM;
+    }
+  }
+  try {
+    std::vector<ProblemSpec> probs(P);
+    for (uint32_t p = 0; p < P; p++) {
+      normalise_box(anchors + 3 * p, widths + 3 * p, dim, probs[p].anchor, probs[p].width);
+      probs[p].periodic = periodic && periodic[p] ? 1u : 0u;
+      probs[p].g0 = offsets[p];
+      probs[p].g1 = offsets[p + 1];
+    }
+    build_multi(Ts, d_xyz, offsets[P], d_mask, probs, dim, o);
+  } catch (const BuildError &e) {
+    (void)hipDeviceSynchronize();
+    (void)hipGetLastError();
+    for (mv_tess *t : Ts) delete t;
+    return e.st;
+  }
+  for (uint32_t p = 0; p < P; p++) out[p] = Ts[p];
+  return MV_OK;
+}
+
+bool valid_batch(const uint64_t *offsets, uint32_t P, uint32_t dim, const double *anchors,
+                 const double *widths) {
+  if (!offsets || P == 0 || P > (1u << 24) || offsets[0] != 0) return false;
+  if (!anchors || !widths || dim < 1 || dim > 3) return false;
+  for (uint32_t p = 0; p < P; p++)
+    if (offsets[p + 1] < offsets[p]) return false;
+  return offsets[P] < (1ull << 27);
+}
+
 // ---------------------------------------------------------------------
 // kept cells: device view and the active-cell list
 // ---------------------------------------------------------------------
@@ -1211,6 +1436,7 @@ CellGeom geom_of(const mv_tess *t) {
   g.cell_loc = t->d_cell_loc;
   for (int c = 0; c < 3; c++) g.width[c] = t->width[c];
   g.dim = (int)t->dim;
+  g.idx_base = t->idx_base;
   return g;
 }
 
@@ -1382,6 +1608,61 @@ mv_status mv_build_device(const double *d_xyz, size_t n,
                    opt, out);
 }
 
+mv_status mv_build_batch_device(const double *d_xyz, const uint64_t *offsets, uint32_t P,
+                                const uint8_t *d_mask, const double *anchors,
+                                const double *widths, const uint32_t *periodic,
+                                uint32_t dimensionality, const mv_options *opt, mv_tess **out) {
+  if (!out || !valid_batch(offsets, P, dimensionality, anchors, widths)) return MV_E_ARG;
+  for (uint32_t p = 0; p < P; p++) out[p] = nullptr;
+  if (offsets[P] && !d_xyz) return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  return run_build_batch(d_xyz, offsets, P, d_mask, anchors, widths, periodic, dimensionality,
+                         opt, out);
+}
+
+mv_status mv_build_batch(const double *xyz, const uint64_t *offsets, uint32_t P,
+                         const uint8_t *mask, const double *anchors, const double *widths,
+                         const uint32_t *periodic, uint32_t dimensionality,
+                         const mv_options *opt, mv_tess **out) {
+  if (!out || !valid_batch(offsets, P, dimensionality, anchors, widths)) return MV_E_ARG;
+  for (uint32_t p = 0; p < P; p++) out[p] = nullptr;
+  const size_t n = offsets[P];
+  if (n && !xyz) return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  double *d_xyz = nullptr;
+  uint8_t *d_mask = nullptr;
+  mv_status st = MV_OK;
+  if (n) {
+    d_xyz = (double *)dalloc(3 * n * sizeof(double));
+    if (!d_xyz) return MV_E_OOM;
+    if (hipMemcpy(d_xyz, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      st = MV_E_DEVICE;
+    if (st == MV_OK && mask) {
+      d_mask = (uint8_t *)dalloc(n);
+      if (!d_mask)
+        st = MV_E_OOM;
+      else if (hipMemcpy(d_mask, mask, n, hipMemcpyHostToDevice) != hipSuccess)
+        st = MV_E_DEVICE;
+    }
+  }
+  if (st == MV_OK)
+    st = run_build_batch(d_xyz, offsets, P, d_mask, anchors, widths, periodic, dimensionality,
+                         opt, out);
+  dfree(d_xyz);
+  dfree(d_mask);
+  if (st == MV_OK) {
+    st = guarded([&] {
+      for (uint32_t p = 0; p < P; p++) download(out[p]);
+    });
+    if (st != MV_OK)
+      for (uint32_t p = 0; p < P; p++) {
+        delete out[p];
+        out[p] = nullptr;
+      }
+  }
+  return st;
+}
+
 mv_status mv_build(const double *xyz, size_t n, const uint8_t *mask,
                    const double anchor[3], const double width[3],
                    uint32_t dimensionality, uint32_t periodic,
@@ -1522,7 +1803,15 @@ mv_status mv_model_features(const double *xyz, size_t n, const double anchor[3],
     double *d_xyz = need(TT.get<double>(3 * n));
     HIPCHK(hipMemcpyAsync(d_xyz, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
     GridData D;
-    build_grid(D, d_xyz, n, anchor, width, dimensionality, periodic ? 1u : 0u, 2.0, TT, s);
+    std::vector<ProblemSpec> probs(1);
+    normalise_box(anchor, width, dimensionality, probs[0].anchor, probs[0].width);
+    probs[0].periodic = periodic ? 1u : 0u;
+    probs[0].g0 = 0;
+    probs[0].g1 = n;
+    build_grid(D, d_xyz, n, probs, dimensionality, 2.0, TT, s);
+    const std::vector<FeatParams> fph = feat_params(probs, (int)dimensionality);
+    FeatParams *fpd = need(TT.get<FeatParams>(1));
+    HIPCHK(hipMemcpyAsync(fpd, fph.data(), sizeof(FeatParams), hipMemcpyHostToDevice, s));
     uint32_t *flag = need(TT.get<uint32_t>(n));
     uint32_t *ident = need(TT.get<uint32_t>(n));
     hipLaunchKernelGGL(k_flag_active, dim3(nblk(n)), dim3(256), 0, s, D.spos, n,
@@ -1531,8 +1820,8 @@ mv_status mv_model_features(const double *xyz, size_t n, const double anchor[3],
     float *hl = need(TT.get<float>(n));
     float *Fo = need(TT.get<float>(n * NF));
     float *ho = need(TT.get<float>(n));
-    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start, D.grid,
-                       D.box, feat_params(D, n, (int)dimensionality), ident, (uint32_t)n, F, hl);
+    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start, D.PR(),
+                       (const FeatParams *)fpd, ident, (uint32_t)n, F, hl);
     hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, NF, F, Fo);
     hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, 1, hl, ho);
     HIPCHK(hipGetLastError());

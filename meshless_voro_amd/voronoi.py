@@ -176,6 +176,47 @@ class Voronoi:
         """Voronoi::build_partial (src/voronoi.rs:181-190)."""
         return cls._build(generators, mask, anchor, width, dimensionality, periodic, options)
 
+    @classmethod
+    def build_batch(cls, problems, dimensionality=Dimensionality.ThreeD, masks=None,
+                    options: Optional[BuildOptions] = None) -> "list[Voronoi]":
+        """Many independent builds in one pass (mv_build_batch): `problems` is
+        a sequence of (generators, anchor, width, periodic); `masks` (optional)
+        one mask or None per problem.  Returns one Voronoi per problem, equal
+        to Voronoi.build / build_partial of that problem alone."""
+        P = len(problems)
+        if P == 0:
+            return []
+        xs = [np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1, 3))
+              for g, _, _, _ in problems]
+        offsets = np.zeros(P + 1, np.uint64)
+        offsets[1:] = np.cumsum([len(x) for x in xs])
+        xyz = np.ascontiguousarray(np.concatenate(xs, 0)) if offsets[-1] else np.zeros((0, 3))
+        anchors = np.ascontiguousarray(np.stack([_vec3(a) for _, a, _, _ in problems]))
+        widths = np.ascontiguousarray(np.stack([_vec3(w) for _, _, w, _ in problems]))
+        per = np.ascontiguousarray(np.array([int(bool(p)) for _, _, _, p in problems], np.uint32))
+        m = None
+        if masks is not None and any(mk is not None for mk in masks):
+            parts = []
+            for x, mk in zip(xs, masks):
+                if mk is None:
+                    parts.append(np.ones(len(x), np.uint8))
+                else:
+                    mk = np.asarray(mk).astype(np.uint8).reshape(-1)
+                    if mk.shape[0] < len(x):
+                        raise VoronoiError(1)
+                    parts.append(mk[:len(x)])
+            m = np.ascontiguousarray(np.concatenate(parts)) if parts else None
+        L = _native.lib()
+        opt = (options or BuildOptions()).to_c()
+        hs = (C.c_void_p * P)()
+        st = L.mv_build_batch(xyz.ctypes.data if offsets[-1] else None, offsets.ctypes.data, P,
+                              m.ctypes.data if m is not None else None, anchors.ctypes.data,
+                              widths.ctypes.data, per.ctypes.data,
+                              int(Dimensionality(dimensionality)), C.byref(opt), hs)
+        if st != 0:
+            raise VoronoiError(st)
+        return [cls._from_handle(C.c_void_p(hs[p])) for p in range(P)]
+
     # --- accessors (src/voronoi.rs:317-359) ---
     def anchor(self) -> np.ndarray:
         return self._anchor.copy()
