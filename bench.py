@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--modes", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=524288)
+    ap.add_argument("--batched", action="store_true",
+                    help="also measure many problems in one build (512 problems, ~10.6 M "
+                         "seeds; the clustered problems take tens of seconds)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (elementwise kernel) so that "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE can be scaled")
@@ -71,6 +74,86 @@ def calibrate_copy():
     torch.add(x, 0.0, out=y)
     torch.cuda.synchronize()
     del x, y
+
+
+def batched_c5(L, mode, reps=3, only_uniform=False):
+    """512 independent problems (10^3..10^5 seeds each, ~10.6 M in total; even
+    problems uniform, odd problems 8 wrapped Gaussian clusters; periodic for
+    p % 4 < 2), device-resident: one batched build (mv_build_batch_device)
+    against one build per problem (mv_build_device), full and with a 10 %
+    mask on every problem (build_partial).  only_uniform keeps the 256
+    uniform problems."""
+    import numpy as np
+    import torch
+    from meshless_voro_amd import _native, workloads as W
+    from meshless_voro_amd.voronoi import BuildOptions, VoronoiError
+    probs = W.batched_problems(512, 5)
+    if only_uniform:
+        probs = probs[0::2]
+    P = len(probs)
+    sizes = [len(x) for x, _ in probs]
+    offsets = np.zeros(P + 1, np.uint64)
+    offsets[1:] = np.cumsum(sizes)
+    xyz = np.ascontiguousarray(np.concatenate([x for x, _ in probs]))
+    per = np.array([int(p) for _, p in probs], np.uint32)
+    anchors = np.zeros((P, 3))
+    widths = np.ones((P, 3))
+    mask = np.ascontiguousarray(np.concatenate(
+        [W.partial_mask(n, 0.1, 1000 + i) for i, n in enumerate(sizes)]).astype(np.uint8))
+    xt = torch.from_numpy(xyz).to("cuda:0")
+    mt = torch.from_numpy(mask).to("cuda:0")
+    opt = BuildOptions(mode=mode).to_c()
+    hs = (C.c_void_p * P)()
+
+    def batched(use_mask):
+        st = L.mv_build_batch_device(C.c_void_p(xt.data_ptr()), offsets.ctypes.data, P,
+                                     C.c_void_p(mt.data_ptr()) if use_mask else None,
+                                     anchors.ctypes.data, widths.ctypes.data, per.ctypes.data, 3,
+                                     C.byref(opt), hs)
+        if st != 0:
+            raise VoronoiError(st)
+        for p in range(P):
+            L.mv_free(hs[p])
+
+    def sequential(use_mask):
+        a, w = np.zeros(3), np.ones(3)
+        for p in range(P):
+            if p % 32 == 31:
+                print(f"bench:     problem {p + 1}/{P}", file=sys.stderr, flush=True)
+            o = int(offsets[p])
+            h = C.c_void_p()
+            st = L.mv_build_device(C.c_void_p(xt.data_ptr() + 24 * o), sizes[p],
+                                   C.c_void_p(mt.data_ptr() + o) if use_mask else None,
+                                   a.ctypes.data, w.ctypes.data, 3, int(per[p]), C.byref(opt),
+                                   C.byref(h))
+            if st != 0:
+                raise VoronoiError(st)
+            L.mv_free(h)
+
+    def timed(fn, *a):
+        print(f"bench:   {fn.__name__} mask={a[0]}", file=sys.stderr, flush=True)
+        if reps > 1:
+            fn(*a)
+            torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn(*a)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    n = int(offsets[-1])
+    nm = int(mask.sum())
+    out = {"problems": P, "seeds": n, "periodic_problems": int(per.sum())}
+    for name, use_mask, cells in (("full", False, n), ("partial_10pct", True, nm)):
+        tb = timed(batched, use_mask)
+        ts = timed(sequential, use_mask)
+        out[name] = {"cells": cells, "batched_ms": 1e3 * tb, "sequential_ms": 1e3 * ts,
+                     "batched_cells_per_s": cells / tb, "sequential_cells_per_s": cells / ts,
+                     "batched_speedup": ts / tb}
+        print(f"bench:   {name}: {json.dumps(out[name])}", file=sys.stderr, flush=True)
+    return out
 
 
 def traffic_from_profile(kernel):
@@ -173,6 +256,7 @@ def main():
     results = {}
     digests = {}
     for name, mode in modes.items():
+        print(f"bench: mode {name}", file=sys.stderr, flush=True)
         for _ in range(args.warmup):
             h, _ = build_device(L, xt, n, mode)
             L.mv_free(h)
@@ -257,9 +341,18 @@ def main():
     }
     if learned_note:
         out["note"] = learned_note
+    if args.batched:
+        bm = MODE_LEARNED if "learned" in modes else MODE_PLAIN
+        print("bench: many-problem builds (uniform half)", file=sys.stderr, flush=True)
+        out["batched_c5_uniform"] = batched_c5(L, bm, reps=3, only_uniform=True)
+        print("bench: many-problem builds (all 512)", file=sys.stderr, flush=True)
+        out["batched_c5"] = batched_c5(L, bm, reps=1)
+        out["batched_c5"]["mode"] = out["batched_c5_uniform"]["mode"] = (
+            "learned" if bm == MODE_LEARNED else "plain")
     if args.calibrate_copy:
         calibrate_copy()
     if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)
 

@@ -60,7 +60,7 @@ typedef struct mv_options {
   uint32_t k0;       /* neighbours in the first pass, self included
                         (plain mode); default 27 */
   uint32_t k_max;    /* largest neighbour list tried; default and
-                        maximum 32768 */
+                        maximum 4194304 (MV_K_MAX) */
   uint32_t lanes;    /* thread-per-cell lanes in flight; 0 = auto */
   uint32_t grid_occupancy16; /* generators per grid cell x 16; 0 = 32 */
   uint32_t flags;    /* MV_FLAG_* */
@@ -73,6 +73,11 @@ typedef struct mv_options {
  * areas); the default one-cell-per-wavefront kernel gives the same
  * connectivity with integrals equal up to rounding. */
 #define MV_FLAG_EXACT_ORDER 1u
+
+/* Largest neighbour list (entries of the neighbour stream one cell may
+ * need: all generators within its security radius, every periodic image
+ * counted). */
+#define MV_K_MAX 4194304u
 
 /* VoronoiIntegrator::build (src/voronoi.rs:549-621): also keep every active
  * cell's half-spaces and vertices on the device, for custom cell and face

@@ -380,9 +380,12 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
 }
 
 // neighbour lists: the LDS gather buffer must hold more than L entries;
-// long lists (and cells that did not fit) use the global-memory variant
+// long lists (and cells that did not fit) use the global-memory variant,
+// whose per-wavefront buffer holds at least 2 L entries (a power of two, at
+// least 65,536); at most 8 GiB of such buffers are in flight
 constexpr uint32_t BIG_KNN_CAP = 65536;
 constexpr uint32_t BIG_KNN_WAVES = 1024;
+constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
 
 void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems &PR,
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
@@ -397,12 +400,17 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else {
-    uint32_t waves = std::min<uint32_t>(cnt, BIG_KNN_WAVES);
+    uint32_t cap = BIG_KNN_CAP;
+    while (cap < 2 * (uint64_t)L && cap < (1u << 24)) cap <<= 1;
+    const uint64_t per_wave = (uint64_t)cap * (sizeof(double) + sizeof(uint64_t));
+    const uint32_t max_waves =
+        (uint32_t)std::max<uint64_t>(4, std::min<uint64_t>(BIG_KNN_WAVES, BIG_KNN_BYTES / per_wave));
+    uint32_t waves = std::min<uint32_t>(cnt, max_waves);
     waves = (waves + 3) / 4 * 4;
-    double *gkey = need(TT.get<double>((size_t)waves * BIG_KNN_CAP));
-    uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * BIG_KNN_CAP));
+    double *gkey = need(TT.get<double>((size_t)waves * cap));
+    uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * cap));
     hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, PR,
-                       wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, BIG_KNN_CAP);
+                       wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, cap);
   }
   HIPCHK(hipGetLastError());
 }
@@ -674,7 +682,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->stats.ms_grid = tg.stop();
 
   // ---- list lengths of the first pass
-  const uint32_t kmax = std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : 32768, 32768), 2);
+  const uint32_t kmax =
+      std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : MV_K_MAX, MV_K_MAX), 2);
   const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
   float *Rfirst = nullptr;
   uint32_t lbound = k0;  // upper bound of the current main list's lengths
@@ -1586,7 +1595,7 @@ void mv_default_options(mv_options *opt) {
   memset(opt, 0, sizeof *opt);
   opt->mode = MV_MODE_PLAIN;
   opt->k0 = 27;
-  opt->k_max = 32768;
+  opt->k_max = MV_K_MAX;
   opt->lanes = 0;
 }
 

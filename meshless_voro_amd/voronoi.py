@@ -55,7 +55,7 @@ class VoronoiError(RuntimeError):
 class BuildOptions:
     """mv_options: construction mode and neighbour-list parameters."""
 
-    def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 32768,
+    def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 4194304,
                  lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
                  model=None):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes

@@ -237,3 +237,19 @@ def test_default_mode_deterministic():
     b = Voronoi.build(x, ZERO, ONE, 3, True)
     for f in ("cell_volume", "cell_centroid", "face_area", "face_centroid", "links"):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_clusters_in_empty_space_need_long_lists():
+    """Eight tight clusters with empty space between them (the odd problems
+    of the many-problem workload): cells on a cluster's rim reach across the
+    empty box, so their security radius holds nearly every seed (lists far
+    beyond 32,768 entries with the periodic images)."""
+    rng = np.random.default_rng(77)
+    n = 40000
+    c = rng.random((8, 3))
+    x = np.mod(c[rng.integers(0, 8, n)] + 0.03 * rng.normal(size=(n, 3)), 1.0)
+    x[x >= 1.0] = 0.0
+    g, o = both(x, periodic=True)
+    check(g, o)
+    assert np.array_equal(g.cell_n_popped, o.cell_n_popped)
+    assert o.cell_n_popped.max() > 32768
