@@ -171,9 +171,11 @@ def build_device(L, xt, n, mode, periodic=True):
     from meshless_voro_amd import _native
     from meshless_voro_amd.voronoi import BuildOptions, VoronoiError
     import numpy as np
+    from meshless_voro_amd.autotune import tuned_options
     a = np.zeros(3)
     w = np.ones(3)
-    opt = BuildOptions(mode=mode).to_c()
+    # launch and layout parameters from the autotuner's table (defaults if none)
+    opt = tuned_options(n, periodic, mode).to_c()
     h = C.c_void_p()
     st = L.mv_build_device(C.c_void_p(xt.data_ptr()), n, None, a.ctypes.data, w.ctypes.data,
                            3, int(periodic), C.byref(opt), C.byref(h))

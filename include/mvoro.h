@@ -65,7 +65,9 @@ typedef struct mv_options {
   uint32_t grid_occupancy16; /* generators per grid cell x 16; 0 = 32 */
   uint32_t flags;    /* MV_FLAG_* */
   const struct mv_model *model; /* MV_MODE_LEARNED: proposal weights */
-  uint32_t reserved[4];
+  uint32_t clip_blocks_per_cu;  /* one-cell-per-wavefront clip launch:
+                                   blocks of 4 wavefronts per CU; 0 = 4 */
+  uint32_t reserved[3];
 } mv_options;
 
 /* Run every cell on the thread-per-cell kernel, which sums each integral in
@@ -84,6 +86,10 @@ typedef struct mv_options {
  * integrals (mv_cell_integrals, mv_face_integrals, mv_tets and the device
  * templates of mvoro_integrals.hpp). */
 #define MV_FLAG_KEEP_CELLS 2u
+
+/* Learned mode: keep the cells in grid order instead of ordering them by
+ * proposed list length (longest first). */
+#define MV_FLAG_NO_LENGTH_ORDER 4u
 
 /* Fill *opt with the defaults. */
 void mv_default_options(mv_options *opt);

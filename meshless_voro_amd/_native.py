@@ -48,12 +48,13 @@ class mv_options(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("k0", C.c_uint32), ("k_max", C.c_uint32),
                 ("lanes", C.c_uint32), ("grid_occupancy16", C.c_uint32),
                 ("flags", C.c_uint32), ("model", C.c_void_p),
-                ("reserved", C.c_uint32 * 4)]
+                ("clip_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 MV_MODEL_MAGIC = 0x4B4C564D
 MV_FLAG_EXACT_ORDER = 1
 MV_FLAG_KEEP_CELLS = 2
+MV_FLAG_NO_LENGTH_ORDER = 4
 
 
 class mv_cell_geometry(C.Structure):

@@ -709,21 +709,26 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     const float rscale = opt.model->rscale > 0.f ? opt.model->rscale : 0.f;
     hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
                        std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
-    // longest lists first
+    // longest lists first (or grid order on request)
     const uint32_t NB = std::min<uint32_t>(lbound / 8 + 1, LEN_MAX_BUCKETS);
     uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
     uint32_t *bst = need(TT.get<uint32_t>(NB + 1));
     uint32_t *bfill = need(TT.get<uint32_t>(NB));
     HIPCHK(hipMemsetAsync(hist, 0, (NB + 1) * sizeof(uint32_t), s));
     HIPCHK(hipMemsetAsync(bfill, 0, NB * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_len_hist, dim3((nwork + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK), dim3(256), 0, s, lens[0], nwork, NB, hist);
-    scan_excl<uint32_t, uint32_t>(hist, bst, NB + 1, s);
-    Rfirst = need(TT.get<float>(nwork));
-    hipLaunchKernelGGL(k_len_scatter, dim3((nwork + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK), dim3(256), 0, s, lists[0], lens[0], R0,
-                       nwork, NB, bst, bfill, lists[1], lens[1], Rfirst);
-    HIPCHK(hipGetLastError());
-    std::swap(lists[0], lists[1]);
-    std::swap(lens[0], lens[1]);
+    if (opt.flags & MV_FLAG_NO_LENGTH_ORDER) {
+      Rfirst = R0;
+    } else {
+      const uint32_t lblocks = (nwork + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK;
+      hipLaunchKernelGGL(k_len_hist, dim3(lblocks), dim3(256), 0, s, lens[0], nwork, NB, hist);
+      scan_excl<uint32_t, uint32_t>(hist, bst, NB + 1, s);
+      Rfirst = need(TT.get<float>(nwork));
+      hipLaunchKernelGGL(k_len_scatter, dim3(lblocks), dim3(256), 0, s, lists[0], lens[0], R0,
+                         nwork, NB, bst, bfill, lists[1], lens[1], Rfirst);
+      HIPCHK(hipGetLastError());
+      std::swap(lists[0], lists[1]);
+      std::swap(lens[0], lens[1]);
+    }
     ms_model = tm.stop();
   } else if (nwork) {
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nwork)), dim3(256), 0, s, lens[0], nwork, k0);
@@ -893,7 +898,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.overflow = counters + 3;
           if (T->keep) W.cs = T->cs;
           HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
-          uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * 4u);
+          const uint32_t bpc = opt.clip_blocks_per_cu ? opt.clip_blocks_per_cu : 4u;
+          uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * bpc);
           Timer tcw(s);
           if (T->keep)
             hipLaunchKernelGGL(k_clip_wave<true>, dim3(blocks), dim3(256), 0, s, W);

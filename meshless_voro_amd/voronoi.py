@@ -57,10 +57,12 @@ class BuildOptions:
 
     def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 4194304,
                  lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
-                 model=None):
+                 model=None, clip_blocks_per_cu: int = 0, length_order: bool = True):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
         self.exact_order = exact_order
+        self.clip_blocks_per_cu = clip_blocks_per_cu
+        self.length_order = length_order
         # learned mode: a meshless_voro_amd.learned.Model (None: the
         # committed checkpoint)
         self.model = model
@@ -70,7 +72,9 @@ class BuildOptions:
         _native.lib().mv_default_options(C.byref(o))
         o.mode, o.k0, o.k_max, o.lanes = self.mode, self.k0, self.k_max, self.lanes
         o.grid_occupancy16 = self.occupancy16
-        o.flags = 1 if self.exact_order else 0
+        o.flags = (_native.MV_FLAG_EXACT_ORDER if self.exact_order else 0) | (
+            0 if self.length_order else _native.MV_FLAG_NO_LENGTH_ORDER)
+        o.clip_blocks_per_cu = self.clip_blocks_per_cu
         if self.mode == MODE_LEARNED:
             from . import learned
             m = self.model if self.model is not None else learned.default_model()
