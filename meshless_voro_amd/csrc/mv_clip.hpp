@@ -25,6 +25,7 @@ struct ClipArgs {
   const uint64_t *kofs;  // list offset per item (nullptr: w * L)
   const uint32_t *kcnt;
   const double *kdist;   // distance of the first entry left out (< 0: none)
+  double *need_r;        // safety radius of cells that need more entries
   uint32_t L;
   // lane state
   double4 *plA;   // plane: n.xyz, d
@@ -549,6 +550,7 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
       uint8_t st = build_cell<MAXP, MAXV>(S, c, A, (uint32_t)w, &nf, &npop, &err);
       my_exact += c.nexact;
       A.wstate[w] = st;
+      if (st == CS_NEED_MORE) A.need_r[w] = c.sr;
       if (st == CS_ERROR) {
         atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
       }

@@ -57,6 +57,7 @@ struct WaveArgs {
   const uint64_t *kofs;   // list offset per work item (nullptr: w * L)
   const uint32_t *kcnt;
   const double *kdist;    // distance of the first entry left out (< 0: none)
+  double *need_r;         // safety radius of cells that need more entries
   uint32_t L;
   unsigned int *queue;    // work-queue head
   uint8_t *wstate;
@@ -586,6 +587,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     }
     if (lane == 0) {
       A.wstate[w] = state;
+      if (state == CS_NEED_MORE) A.need_r[w] = sr;
       if (state == CS_ERROR)
         atomicMin(A.err_word, (unsigned long long)((idx << 8) | err));
     }

@@ -370,7 +370,7 @@ __device__ __forceinline__ uint32_t knn_gather(
 // (> cap when no radius fits).
 __device__ __forceinline__ uint32_t knn_fill(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
-    const GridParams &g, const double *width, const double q[3], double R,
+    const GridParams &g, const double *width, const double q[3], double &R,
     uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap) {
   double r_lo = 0., r_hi = 0.;
   uint32_t cnt = 0;
@@ -398,6 +398,17 @@ __device__ __forceinline__ uint32_t knn_fill(
     if (r_hi > 0. && r_lo > 0. && r_hi - r_lo <= 1e-15 * r_hi) break;
   }
   return cnt > cap ? cnt : cap + 1;
+}
+
+// Lower bound on the distance (as the clipper computes it) of every stream
+// entry outside a gathered ball of radius R around q: the ball test uses the
+// key ((q + s) - g)^2, the clipper |q - (g + s')|, which differ by rounding
+// on the scale of the coordinates (and the periodic width).
+__device__ __forceinline__ double ball_certificate(const double q[3], double R,
+                                                  const double *width) {
+  const double M = fmax(fabs(q[0]), fmax(fabs(q[1]), fabs(q[2]))) +
+                   fmax(width[0], fmax(width[1], width[2])) + R;
+  return R - 1e-13 * M;
 }
 
 __device__ __forceinline__ double knn_start_radius(
@@ -485,7 +496,8 @@ __global__ void __launch_bounds__(256)
     }
     if (lane == 0) {
       kcnt[w] = outn;
-      if (cnt == outn) kdist[w] = -1.;  // nothing left out: no certificate
+      // nothing left out: every entry outside lies beyond the ball
+      if (cnt == outn) kdist[w] = ball_certificate(q, R, box.width);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -499,6 +511,7 @@ __global__ void __launch_bounds__(256)
               const uint32_t *__restrict__ cell_start_all, Problems PR,
               const uint32_t *__restrict__ work,
               uint32_t nwork, uint32_t L, const uint32_t *__restrict__ Lper,
+              const float *__restrict__ Rper,
               const uint64_t *__restrict__ kofs, uint64_t *__restrict__ kpk,
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
@@ -522,7 +535,8 @@ __global__ void __launch_bounds__(256)
     const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
-    double R = knn_start_radius(cell_start, g, q, Lw);
+    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
+                                       : knn_start_radius(cell_start, g, q, Lw);
     uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lw, bkey, bpk, cap);
     if (cnt > cap) {
       if (lane == 0) kcnt[w] = 0xFFFFFFFFu;
@@ -559,9 +573,43 @@ __global__ void __launch_bounds__(256)
     for (uint32_t e = lane; e < outn; e += 64) out[e] = bpk[e];
     if (lane == 0) {
       kcnt[w] = outn;
-      kdist[w] = cnt > outn ? entry_dist(spos, box.width, q, bpk[outn]) : -1.;
+      kdist[w] = cnt > outn ? entry_dist(spos, box.width, q, bpk[outn])
+                            : ball_certificate(q, R, box.width);
     }
     __threadfence();
+  }
+}
+
+// Cells to redo get a longer list: doubled while it is short, quadrupled
+// once it holds count_from entries or more (cells whose security radius
+// spans a large part of the problem reach their length in fewer passes).
+// Their gather starts from the local density (a start at the safety radius
+// of a young cell would gather far more entries than the list holds).  A
+// cell already at kmax sets *cap_err.
+__global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
+                            float *__restrict__ Rout, uint32_t kmax, uint32_t count_from,
+                            unsigned int *__restrict__ cap_err) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwork) return;
+  const uint32_t Lp = Lio[w];
+  if (Lp >= kmax) atomicOr(cap_err, 1u);
+  const uint64_t mult = Lp < count_from ? 2 : 4;
+  Lio[w] = (uint32_t)min(mult * Lp, (uint64_t)kmax);
+  Rout[w] = 0.f;
+}
+
+__global__ void k_compact_redo(const uint32_t *__restrict__ in_w,
+                               const uint32_t *__restrict__ in_L,
+                               const double *__restrict__ in_r,
+                               const uint32_t *__restrict__ flag,
+                               const uint32_t *__restrict__ pos, uint32_t n,
+                               uint32_t *__restrict__ out_w, uint32_t *__restrict__ out_L,
+                               double *__restrict__ out_r) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) {
+    out_w[pos[i]] = in_w[i];
+    out_L[pos[i]] = in_L[i];
+    out_r[pos[i]] = in_r[i];
   }
 }
 

@@ -410,7 +410,7 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
     double *gkey = need(TT.get<double>((size_t)waves * cap));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)waves * cap));
     hipLaunchKernelGGL(k_knn_big, dim3(waves / 4), dim3(256), 0, s, spos, cell_start, PR,
-                       wl, cnt, L, Lper, kofs, kpk, kcnt, kdist, gkey, gpk, cap);
+                       wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, cap);
   }
   HIPCHK(hipGetLastError());
 }
@@ -814,7 +814,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   int64_t *st_right = nullptr;
   uint8_t *st_img = nullptr;
   double *st_area = nullptr, *st_cen = nullptr, *st_norm = nullptr;
-  const uint32_t lmin0 = opt.mode == MV_MODE_LEARNED ? std::min<uint32_t>(16, kmax) : k0;
+  unsigned int *cap_err = need(TT.get<unsigned int>(1));
   for (int attempt = 0;; attempt++) {
     st_cell = need(TT.get<uint32_t>(st_cap));
     st_right = need(TT.get<int64_t>(st_cap));
@@ -835,7 +835,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     // bounds on the list lengths of the main and big lists (all clamped to
     // kmax; a list whose lower bound is kmax cannot grow any more)
     uint32_t lb_main = std::min(lbound, kmax), lb_big = lb_main;
-    uint32_t lmin_main = std::min(lmin0, kmax), lmin_big = kmax;
+    const float *Rbig = nullptr;
+    HIPCHK(hipMemsetAsync(cap_err, 0, sizeof(unsigned int), s));
     const float *Rmain = Rfirst;
     bool overflowed = false;
     unsigned long long ovf_bits = 0;
@@ -847,8 +848,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     q_knn = cells_cw = e_knn = 0;
     while (nw + nbig > 0) {
       passes++;
+      double *nr_main = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
+      double *nr_big = need(TT.get<double>(std::max<uint32_t>(nbig, 1)));
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
                      uint8_t *state, bool big) {
+        double *need_r = big ? nr_big : nr_main;
         Timer tk(s);
         uint64_t *kofs = need(TT.get<uint64_t>(cnt));
         const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
@@ -876,6 +880,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.kofs = kofs;
           W.kcnt = kcnt;
           W.kdist = kdist;
+          W.need_r = need_r;
           W.L = 0;
           W.queue = queue;
           W.wstate = state;
@@ -922,6 +927,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.kofs = kofs;
         A.kcnt = kcnt;
         A.kdist = kdist;
+        A.need_r = need_r;
         A.L = 0;
         uint32_t lanes;
         if (!big) {
@@ -975,11 +981,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         ms_clip += tc.stop();
       };
       if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false);
-      if (nbig) run(cb, cbl, nbig, lb_big, nullptr, bstate, true);
-      Rmain = nullptr;
+      if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, true);
       unsigned long long cnts[4];
+      unsigned int cap_err_h = 0;
       HIPCHK(hipMemcpyAsync(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(&cap_err_h, cap_err, sizeof cap_err_h, hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
+      // a cell that reached kmax entries without its security radius
+      if (cap_err_h) throw BuildError{MV_E_CAPACITY};
       if (cnts[1] != ~0ull) {
         static const mv_status map[] = {MV_OK, MV_E_ARG, MV_E_DEGENERATE_POINTS,
                                         MV_E_SELF_NOT_FIRST, MV_E_PLANE_DET_ZERO,
@@ -992,42 +1001,66 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         overflowed = true;
         break;
       }
-      // next lists: the cells whose security radius was not reached get a
-      // doubled list; cells over the on-chip capacity move to the big list
+      // next lists: cells over the on-chip capacity move to the big list
+      // with their list; cells whose security radius was not reached get a
+      // longer list (k_redo_size: doubled, or quadrupled once long)
       auto take = [&](const uint32_t *w, const uint32_t *L, const uint8_t *st, uint32_t cnt,
-                      uint8_t want, uint32_t mult, uint32_t *ow, uint32_t *oL) -> uint32_t {
+                      uint8_t want, uint32_t *ow, uint32_t *oL, const double *nr,
+                      double *onr) -> uint32_t {
         if (cnt == 0) return 0;
         Temps T2;
         uint32_t *fl = need(T2.get<uint32_t>(cnt));
         uint32_t *ps = need(T2.get<uint32_t>(cnt));
         hipLaunchKernelGGL(k_flag_state, dim3(nblk(cnt)), dim3(256), 0, s, st, cnt, want, fl);
         uint32_t c = scan_excl<uint32_t, uint32_t>(fl, ps, cnt, s);
-        if (c)
+        if (c && nr)
+          hipLaunchKernelGGL(k_compact_redo, dim3(nblk(cnt)), dim3(256), 0, s, w, L, nr, fl, ps,
+                             cnt, ow, oL, onr);
+        else if (c)
           hipLaunchKernelGGL(k_compact_pairs, dim3(nblk(cnt)), dim3(256), 0, s, w, L, fl, ps,
-                             cnt, mult, kmax, ow, oL);
+                             cnt, 1u, kmax, ow, oL);
         HIPCHK(hipGetLastError());
         return c;
       };
       uint32_t *cw2 = lists[ping_m], *cl2 = lens[ping_m];
       uint32_t *cb2 = lists[ping_b], *cbl2 = lens[ping_b];
-      const uint32_t nw_next = take(cw, cl, wstate, nw, CS_NEED_MORE, 2, cw2, cl2);
-      const uint32_t nb_next = take(cb, cbl, bstate, nbig, CS_NEED_MORE, 2, cb2, cbl2);
-      const uint32_t ncap = take(cw, cl, wstate, nw, CS_CAPACITY, 1, cb2 + nb_next, cbl2 + nb_next);
-      const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, 1, cw2 + nw_next,
-                                  cl2 + nw_next);
+      double *rr_m = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
+      double *rr_b = need(TT.get<double>(std::max<uint32_t>(nw + nbig, 1)));
+      const uint32_t nw_next = take(cw, cl, wstate, nw, CS_NEED_MORE, cw2, cl2, nr_main, rr_m);
+      const uint32_t nb_next =
+          take(cb, cbl, bstate, nbig, CS_NEED_MORE, cb2, cbl2, nr_big, rr_b);
+      const uint32_t ncap = take(cw, cl, wstate, nw, CS_CAPACITY, cb2 + nb_next, cbl2 + nb_next,
+                                 nullptr, nullptr);
+      const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, cw2 + nw_next,
+                                  cl2 + nw_next, nullptr, nullptr);
       if (nbcap) throw BuildError{MV_E_CAPACITY};
-      // a cell that reached kmax entries without its security radius
-      if ((nw_next && lmin_main >= kmax) || (nb_next && lmin_big >= kmax))
-        throw BuildError{MV_E_CAPACITY};
+      float *Rm_next = need(TT.get<float>(std::max<uint32_t>(nw_next, 1)));
+      float *Rb_next = need(TT.get<float>(std::max<uint32_t>(nb_next + ncap, 1)));
+      // doubling throughout: quadrupling long lists (COUNT_FROM) made the
+      // clustered many-problem workload slower (its huge lists overshoot)
+      constexpr uint32_t GROW = 4, COUNT_FROM = 0xFFFFFFFFu;
+      if (nw_next)
+        hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
+                           Rm_next, kmax, COUNT_FROM, cap_err);
+      if (nb_next)
+        hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
+                           Rb_next, kmax, COUNT_FROM, cap_err);
+      if (ncap)  // moved cells keep their list; their gather starts from the density
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
+                           (uint32_t *)(Rb_next + nb_next), ncap, 0u);
+      HIPCHK(hipGetLastError());
       bigcells += ncap;
       redone += nw_next + nb_next;
       if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
-      const uint32_t lb_main_old = lb_main, lmin_main_old = lmin_main;
-      lb_main = std::min(2 * lb_main, kmax);
-      lmin_main = std::min(2 * lmin_main, kmax);
-      lb_big = std::min(std::max(2 * lb_big, lb_main_old), kmax);
-      lmin_big = std::min(nb_next ? std::min(2 * lmin_big, kmax) : kmax,
-                          ncap ? lmin_main_old : kmax);
+      const uint32_t lb_main_old = lb_main;
+      // bounds on the new lengths (k_redo_size doubles below COUNT_FROM)
+      auto grown = [&](uint32_t lb) {
+        return (uint32_t)std::min<uint64_t>((uint64_t)(lb < COUNT_FROM ? 2 : GROW) * lb, kmax);
+      };
+      lb_main = grown(lb_main);
+      lb_big = std::max(grown(lb_big), std::min(lb_main_old, kmax));
+      Rmain = Rm_next;
+      Rbig = Rb_next;
       nw = nw_next;
       nbig = nb_next + ncap;
       cw = cw2;

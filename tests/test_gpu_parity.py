@@ -153,12 +153,26 @@ def test_low_dimensions(dim):
         check(g, o)
 
 
-def test_small_k0_forces_doubling():
-    """k0 = 2 makes nearly every cell take several passes; output unchanged."""
+def test_small_k0_forces_redo_passes():
+    """k0 = 2 makes every cell run out of neighbours and be redone with a list
+    sized from its safety radius (several passes); output unchanged."""
     x = W.uniform(3000, 44)
     g, o = both(x, periodic=True, options=BuildOptions(k0=2))
     check(g, o)
-    assert g.stats["passes"] >= 5
+    assert g.stats["passes"] >= 3 and g.stats["cells_redone"] >= 3000
+    assert np.array_equal(g.cell_n_popped, o.cell_n_popped)
+
+
+@pytest.mark.parametrize("k0", [3, 8, 40])
+def test_small_k_max_reports_capacity_or_matches(k0):
+    """A cap on the list length below what cells need returns
+    MV_E_CAPACITY; a sufficient cap gives the oracle's result."""
+    x = W.uniform(2000, 45)
+    with pytest.raises(VoronoiError) as e:
+        Voronoi.build(x, ZERO, ONE, 3, True, options=BuildOptions(k0=k0, k_max=12))
+    assert e.value.status == 9
+    g, o = both(x, periodic=True, options=BuildOptions(k0=k0, k_max=4096))
+    check(g, o)
 
 
 def test_errors():
