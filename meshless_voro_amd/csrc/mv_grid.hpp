@@ -265,30 +265,30 @@ __device__ __forceinline__ uint32_t knn_gather(
   const double rr = R + g.margin;
   const double rr2 = rr * rr;
   uint32_t cnt = 0;
-  for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++)
-    for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++)
+  // images per axis, nested: an image that misses the grid on an outer axis
+  // skips its inner loops (most balls cross no periodic face)
+  auto axis = [&](int c, int sv, double &qs, int &lo, int &hi) -> bool {
+    qs = q[c] + (double)sv * width[c];
+    if (g.inv_h[c] == 0.) {
+      lo = 0;
+      hi = 0;
+      return true;
+    }
+    const double a = floor((qs - rr - g.lo[c]) * g.inv_h[c]);
+    const double b = floor((qs + rr - g.lo[c]) * g.inv_h[c]);
+    if (b < 0. || a > (double)(g.G[c] - 1)) return false;
+    lo = a < 0. ? 0 : (int)a;
+    hi = b > (double)(g.G[c] - 1) ? g.G[c] - 1 : (int)b;
+    return true;
+  };
+  for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++) {
+    double qs[3];
+    int lo[3], hi[3];
+    if (!axis(0, i, qs[0], lo[0], hi[0])) continue;
+    for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++) {
+      if (!axis(1, j, qs[1], lo[1], hi[1])) continue;
       for (int k = g.img_lo[2]; k <= g.img_hi[2]; k++) {
-        const int sv[3] = {i, j, k};
-        double qs[3];
-        int lo[3], hi[3];
-        bool skip = false;
-        for (int c = 0; c < 3; c++) {
-          qs[c] = q[c] + (double)sv[c] * width[c];
-          if (g.inv_h[c] == 0.) {
-            lo[c] = 0;
-            hi[c] = 0;
-            continue;
-          }
-          double a = floor((qs[c] - rr - g.lo[c]) * g.inv_h[c]);
-          double b = floor((qs[c] + rr - g.lo[c]) * g.inv_h[c]);
-          if (b < 0. || a > (double)(g.G[c] - 1)) {
-            skip = true;
-            break;
-          }
-          lo[c] = a < 0. ? 0 : (int)a;
-          hi[c] = b > (double)(g.G[c] - 1) ? g.G[c] - 1 : (int)b;
-        }
-        if (skip) continue;
+        if (!axis(2, k, qs[2], lo[2], hi[2])) continue;
         const uint32_t img = (uint32_t)((i + 1) * 9 + (j + 1) * 3 + (k + 1));
         const int nb_ = hi[1] - lo[1] + 1;
         const int nrows = (hi[0] - lo[0] + 1) * nb_;
@@ -361,6 +361,8 @@ __device__ __forceinline__ uint32_t knn_gather(
           }
         }
       }
+    }
+  }
   return cnt;
 }
 
@@ -447,6 +449,42 @@ struct KnnLds {
   uint64_t pk[CAP];
 };
 
+// Bitonic sort of cnt (key, pk) entries held in LDS by one wavefront,
+// padded with +inf keys to a power of two (the buffer must hold it).
+__device__ __forceinline__ void wave_bitonic_lds(double *key, uint64_t *pk, uint32_t cnt) {
+  const uint32_t lane = __lane_id();
+  uint32_t n2 = 64;
+  while (n2 < cnt) n2 <<= 1;
+  for (uint32_t e = cnt + lane; e < n2; e += 64) {
+    key[e] = INFINITY;
+    pk[e] = ~0ull;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t k = 2; k <= n2; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      // lane handles the pairs (i, i ^ j) with i ^ j > i: n2 / 2 pairs
+      for (uint32_t p = lane; p < (n2 >> 1); p += 64) {
+        const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+        const uint32_t l = i | j;
+        const double ki = key[i], kl = key[l];
+        const uint64_t pi = pk[i], pl = pk[l];
+        const bool up = (i & k) == 0;
+        const bool sw = up ? entry_less(kl, pl, ki, pi) : entry_less(ki, pi, kl, pl);
+        if (sw) {
+          key[i] = kl;
+          key[l] = ki;
+          pk[i] = pl;
+          pk[l] = pi;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 template <int CAP, int WPB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
@@ -485,14 +523,22 @@ __global__ void __launch_bounds__(256)
       continue;
     }
     const uint32_t outn = min(cnt, Lw);
-    for (uint32_t e = lane; e < cnt; e += 64) {
-      const double ke = buf.key[e];
-      const uint64_t pe = buf.pk[e];
-      uint32_t rank = 0;
-      for (uint32_t t = 0; t < cnt; t++)
-        rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
-      if (rank < outn) kpk[(kofs ? kofs[w] : (uint64_t)w * L) + rank] = pe;
-      if (rank == outn) kdist[w] = entry_dist(spos, box.width, q, pe);
+    uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
+    if (cnt <= 64) {
+      // rank sort: each lane counts the entries smaller than its own
+      for (uint32_t e = lane; e < cnt; e += 64) {
+        const double ke = buf.key[e];
+        const uint64_t pe = buf.pk[e];
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < cnt; t++)
+          rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
+        if (rank < outn) out[rank] = pe;
+        if (rank == outn) kdist[w] = entry_dist(spos, box.width, q, pe);
+      }
+    } else {
+      wave_bitonic_lds(buf.key, buf.pk, cnt);
+      for (uint32_t e = lane; e < outn; e += 64) out[e] = buf.pk[e];
+      if (lane == 0 && cnt > outn) kdist[w] = entry_dist(spos, box.width, q, buf.pk[outn]);
     }
     if (lane == 0) {
       kcnt[w] = outn;

@@ -396,6 +396,9 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+  } else if (!big && L <= 512) {
+    hipLaunchKernelGGL((k_knn<1024, 2>), dim3((cnt + 1) / 2), dim3(128), 0, s, spos,
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
   } else if (!big && L <= 1024) {
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
