@@ -164,12 +164,15 @@ uint32_t compact(const uint32_t *work, const uint8_t *st, uint32_t n,
 struct Timer {
   hipEvent_t a, b;
   hipStream_t s;
-  explicit Timer(hipStream_t st) : s(st) {
+  bool on;
+  explicit Timer(hipStream_t st, bool enabled = true) : s(st), on(enabled) {
+    if (!on) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, s);
   }
   double stop() {
+    if (!on) return 0.;
     (void)hipEventRecord(b, s);
     (void)hipEventSynchronize(b);
     float ms = 0;
@@ -655,6 +658,17 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   }
 
   Temps TT;
+  struct Stream2 {
+    hipStream_t h = nullptr;
+    hipEvent_t ev = nullptr;
+    ~Stream2() {
+      if (ev) (void)hipEventDestroy(ev);
+      if (h) (void)hipStreamDestroy(h);
+    }
+  } st2;
+  HIPCHK(hipStreamCreateWithFlags(&st2.h, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&st2.ev, hipEventDisableTiming));
+  hipStream_t s2 = st2.h;
   // ---- grid
   Timer tg(s);
   GridData D;
@@ -853,16 +867,17 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       passes++;
       double *nr_main = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *nr_big = need(TT.get<double>(std::max<uint32_t>(nbig, 1)));
+      // one list on one stream; untimed runs do not wait for their kernels
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
-                     uint8_t *state, bool big) {
+                     uint8_t *state, bool big, hipStream_t s, bool timed) {
         double *need_r = big ? nr_big : nr_main;
-        Timer tk(s);
+        Timer tk(s, timed);
         uint64_t *kofs = need(TT.get<uint64_t>(cnt));
         const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
         uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
-        Timer tkk(s);
+        Timer tkk(s, timed);
         launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
                    s);
         ms_knn_k += tkk.stop();
@@ -870,7 +885,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         q_knn += cnt;
         e_knn += total;
         ms_knn += tk.stop();
-        Timer tc(s);
+        Timer tc(s, timed);
         if (!big && !exact_order) {
           WaveArgs W;
           memset(&W, 0, sizeof W);
@@ -908,7 +923,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
           const uint32_t bpc = opt.clip_blocks_per_cu ? opt.clip_blocks_per_cu : 4u;
           uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * bpc);
-          Timer tcw(s);
+          Timer tcw(s, timed);
           if (T->keep)
             hipLaunchKernelGGL(k_clip_wave<true>, dim3(blocks), dim3(256), 0, s, W);
           else
@@ -983,8 +998,18 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         HIPCHK(hipGetLastError());
         ms_clip += tc.stop();
       };
-      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false);
-      if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, true);
+      // the big list (long lists, large cells: few items, long latency) runs
+      // on its own stream beside the main list
+      if (nbig) {
+        // s2 starts after everything queued on s (the lists it reads)
+        HIPCHK(hipEventRecord(st2.ev, s));
+        HIPCHK(hipStreamWaitEvent(s2, st2.ev, 0));
+      }
+      Timer tbig(s2, nbig > 0);
+      if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, true, s2, false);
+      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false, s, true);
+      if (nbig) ms_clip += tbig.stop();
+      HIPCHK(hipStreamSynchronize(s2));
       unsigned long long cnts[4];
       unsigned int cap_err_h = 0;
       HIPCHK(hipMemcpyAsync(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost, s));
