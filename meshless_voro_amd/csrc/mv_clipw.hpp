@@ -17,10 +17,10 @@
 // Cells that need more than 64 vertices or 64 planes report CS_CAPACITY and
 // are rebuilt by the thread-per-cell kernel with larger capacity.
 //
-// Integrals: each lane evaluates the six tets of its vertex and stages them
-// in LDS; every face and the cell then sum their tets in the reference's
-// order (vertex order, then tet order), so volumes, centroids and face
-// areas are the reference's bits.
+// Integrals: each lane evaluates the two tets of one (vertex, plane) pair
+// and stages them in LDS; every face and the cell then sum their tets in the
+// reference's order (vertex order, then tet order), so volumes, centroids
+// and face areas are the reference's bits.
 #pragma once
 #include "mv_clip.hpp"
 #include "mv_common.hpp"
@@ -28,7 +28,9 @@
 namespace mv {
 
 constexpr int WCAP = 64;  // vertices and planes per cell on this path
-constexpr int WCH = 16;   // vertices per decomposition chunk
+// decomposition: lane 3 i + k takes plane k of the chunk's vertex i, so a
+// chunk covers 21 vertices (63 lanes)
+constexpr int WCH = 21;
 
 struct WaveLds {
   double4 pa[WCAP];      // plane: n.xyz, d
@@ -39,7 +41,8 @@ struct WaveLds {
     struct {
       double4 pr[64];
     } e;
-    // decomposition: per (tet, vertex) of a chunk of WCH vertices
+    // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
+    // stays under 13 KiB: three blocks of four wavefronts fit a CU's LDS)
     struct {
       double4 fcf[6][WCH];  // area, area * (v0 + v1 + v2)
       double4 fcv[6][WCH];  // volume, volume * (v0 + v1 + v2 + gen)
@@ -183,7 +186,7 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
 // instantiation, so that the plain kernel keeps its register budget)
 template <bool KEEP>
 __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
-  __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
+  __shared__ WaveLds lds_all[4];  // 4 x 11.9 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
   const int lane = (int)__lane_id();
   unsigned long long my_exact = 0;
@@ -493,67 +496,69 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           cons = (uint64_t)right > idx || (A.mask != nullptr && !A.mask[right]);
         }
       }
-      // vertices in chunks of WCH: each lane evaluates the six tets of its
-      // vertex and stages them in LDS; faces (lane p = plane p) and the cell
-      // (every lane, broadcast reads) then accumulate them in the
-      // reference's order: vertex order, tet order within a vertex
-      double vol = 0., cx = 0., cy = 0., cz = 0.;
+      // vertices in chunks of WCH: lane 3 i + k evaluates the two tets on
+      // plane k of the chunk's vertex i and stages them in LDS; the cell
+      // (lane c sums component c) and the faces (lane p = plane p) then
+      // accumulate them in the reference's order: vertex order, tet order
+      // within a vertex
+      double cacc = 0.;
+      const double *cterm =
+          reinterpret_cast<const double *>(&S.u.t.fcv[0][0]) + (lane & 3);
+      const int sl = lane / 3, kk = lane - 3 * sl;
       bool ok = true, has = false;
       for (int c0 = 0; c0 < nv && ok; c0 += WCH) {
         const int c1 = min(nv, c0 + WCH);
+        const bool mine = c0 + sl < c1;
+        // the vertex of this lane, from the lane that holds it
+        const int vsrc = min(c0 + sl, 63);
+        const d3 vl = mk3(__shfl(vx, vsrc, 64), __shfl(vy, vsrc, 64), __shfl(vz, vsrc, 64));
+        const int du = __shfl(vd, vsrc, 64);
+        const int pk = (du >> (8 * kk)) & 0xFF;
+        const int pn2 = (du >> (kk == 2 ? 0 : 8 * kk + 8)) & 0xFF;
+        // edge point of planes (k+1, k); the one of (k, k-1) comes from the
+        // lane of plane k-1 of the same vertex (the reference evaluates
+        // both with the same arguments)
+        d3 pedge = mk3(0., 0., 0.);
         bool lok = true;
-        if (lane >= c0 && lane < c1) {
-          const int sl = lane - c0;
-          const d3 vl = mk3(vx, vy, vz);
-          const int pl[3] = {vd & 0xFF, (vd >> 8) & 0xFF, (vd >> 16) & 0xFF};
-          d3 prev;
-          {
-            const double4 a0 = S.pa[pl[0]], a2 = S.pa[pl[2]];
-            lok = project_onto_intersection(xyz(a0), a0.w, xyz(a2), a2.w, q, &prev);
-          }
-          for (int k = 0; k < 3 && lok; k++) {
-            const int pi = pl[k], pn2 = pl[k == 2 ? 0 : k + 1];
-            const double4 ak = S.pa[pi], bk = S.pb[pi], an = S.pa[pn2];
-            const d3 pface = plane_project_onto(xyz(ak), xyz(bk), q);
-            d3 pedge;
-            lok = project_onto_intersection(xyz(an), an.w, xyz(ak), ak.w, q, &pedge);
-            if (!lok) break;
-            for (int h = 0; h < 2; h++) {
-              const d3 v0 = h == 0 ? pface : pedge;
-              const d3 v1 = h == 0 ? prev : pface;
-              const double tv = signed_volume_tet(v0, v1, vl, q);
-              const d3 tc = scl3(tv, add3(add3(add3(v0, v1), vl), q));
-              const double a = signed_area_tri(v0, v1, vl, q);
-              const d3 sc = scl3(a, add3(add3(v0, v1), vl));
-              S.u.t.fcv[2 * k + h][sl] = make_double4(tv, tc.x, tc.y, tc.z);
-              S.u.t.fcf[2 * k + h][sl] = make_double4(a, sc.x, sc.y, sc.z);
-            }
-            prev = pedge;
-          }
+        if (mine) {
+          const double4 ak = S.pa[pk], an = S.pa[pn2];
+          lok = project_onto_intersection(xyz(an), an.w, xyz(ak), ak.w, q, &pedge);
         }
         if (__ballot(!lok)) {
           ok = false;
           break;
         }
+        const int psrc = min(kk == 0 ? lane + 2 : lane - 1, 63);
+        const d3 prev = mk3(__shfl(pedge.x, psrc, 64), __shfl(pedge.y, psrc, 64),
+                            __shfl(pedge.z, psrc, 64));
+        if (mine) {
+          const double4 ak = S.pa[pk], bk = S.pb[pk];
+          const d3 pface = plane_project_onto(xyz(ak), xyz(bk), q);
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const d3 v0 = h == 0 ? pface : pedge;
+            const d3 v1 = h == 0 ? prev : pface;
+            const double tv = signed_volume_tet(v0, v1, vl, q);
+            const d3 tc = scl3(tv, add3(add3(add3(v0, v1), vl), q));
+            const double a = signed_area_tri(v0, v1, vl, q);
+            const d3 sc = scl3(a, add3(add3(v0, v1), vl));
+            S.u.t.fcv[2 * kk + h][sl] = make_double4(tv, tc.x, tc.y, tc.z);
+            S.u.t.fcf[2 * kk + h][sl] = make_double4(a, sc.x, sc.y, sc.z);
+          }
+        }
         lds_fence();
         for (int s2 = c0; s2 < c1; s2++) {
-          const int sl = s2 - c0;
+          const int sv = s2 - c0;
           // VolumeCentroidIntegral::collect (integrals.rs:91-95), 6 tets
 #pragma unroll
-          for (int t = 0; t < 6; t++) {
-            const double4 c = S.u.t.fcv[t][sl];
-            vol += c.x;
-            cx += c.y;
-            cy += c.z;
-            cz += c.w;
-          }
+          for (int t = 0; t < 6; t++) cacc += cterm[(t * WCH + sv) * 4];
           // VoronoiFaceIntegral::collect for the two tets on plane `lane`
-          const int du = rl_i(vd, s2);
-          const int k = (du & 0xFF) == lane ? 0
-                        : ((du >> 8) & 0xFF) == lane ? 1
-                        : ((du >> 16) & 0xFF) == lane ? 2 : -1;
+          const int dv = rl_i(vd, s2);
+          const int k = (dv & 0xFF) == lane ? 0
+                        : ((dv >> 8) & 0xFF) == lane ? 1
+                        : ((dv >> 16) & 0xFF) == lane ? 2 : -1;
           if (k >= 0) {
-            const double4 f0 = S.u.t.fcf[2 * k][sl], f1 = S.u.t.fcf[2 * k + 1][sl];
+            const double4 f0 = S.u.t.fcf[2 * k][sv], f1 = S.u.t.fcf[2 * k + 1][sv];
             fa += f0.x;
             fx += f0.y;
             fy += f0.z;
@@ -567,6 +572,8 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         }
         lds_fence();
       }
+      const double vol = rl_d(cacc, 0), cx = rl_d(cacc, 1), cy = rl_d(cacc, 2),
+                   cz = rl_d(cacc, 3);
       if (!ok) {
         state = CS_ERROR;
         err = E_DET_ZERO;
