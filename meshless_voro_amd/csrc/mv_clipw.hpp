@@ -36,10 +36,13 @@ struct WaveLds {
   double4 pa[WCAP];      // plane: n.xyz, d
   double4 pb[WCAP];      // plane: p.xyz, reference word
   union {
-    // p and reference word of the 64 stream entries being consumed (lane
-    // t wrote entry t; read back when the entry's plane is kept)
+    // the 64 stream entries being consumed (lane t wrote entry t): plane
+    // p and reference word, plane n and d, neighbour distance and the
+    // plane's rounding bound
     struct {
       double4 pr[64];
+      double4 nd[64];
+      double2 de[64];
     } e;
     // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
     // stays under 13 KiB: three blocks of four wavefronts fit a CU's LDS)
@@ -223,7 +226,10 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     lds_fence();
     int np = 6, nv = 8, cstart = 0, clen = 0;
     int cyc = lane;  // ptrs[p] of SimpleCycle::new
-    double vx = 0., vy = 0., vz = 0., vr = -INFINITY;
+    // vs = 2 |v - q| as the reference rounds it (2 sqrt(r2)): the safety
+    // radius 2 sqrt(max r2) is the maximum of vs over the vertices, sqrt
+    // being correctly rounded and so monotone
+    double vx = 0., vy = 0., vz = 0., vs = -INFINITY;
     int vd = 0;
     {
       // init vertex duals (2,5,0) (5,3,0) (1,5,2) (5,1,3) (4,2,0) (4,0,3)
@@ -239,7 +245,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         vx = l.x;
         vy = l.y;
         vz = l.z;
-        vr = r2;
+        vs = 2. * sqrt(r2);
         vd = (int)du;
       }
       if (__ballot(!ok)) {
@@ -247,13 +253,9 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         err = E_DET_ZERO;
       }
     }
-    double sr = 0.;
-    if (state == CS_DONE) {
-      if (__ballot(lane < nv && isnan(vr))) {
-        state = CS_ERROR;
-        err = E_NAN;
-      }
-      sr = 2. * sqrt(wave_max_nonneg(lane < nv ? vr : 0.));
+    if (state == CS_DONE && __ballot(lane < nv && isnan(vs))) {
+      state = CS_ERROR;
+      err = E_NAN;
     }
     if (state == CS_DONE && cnt == 0xFFFFFFFFu) state = CS_CAPACITY;
     if (state == CS_DONE && (cnt == 0 || (uint64_t)pk_idx(list[0]) != idx)) {
@@ -283,24 +285,36 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         perrb = 1e-13 * (1. + dot3(abs3(pn), abs3(pp)));
         pd = dot3(pn, pp);
         S.u.e.pr[lane] = make_double4(pp.x, pp.y, pp.z, __longlong_as_double((long long)ppk));
+        S.u.e.nd[lane] = make_double4(pn.x, pn.y, pn.z, pd);
+        S.u.e.de[lane] = make_double2(pdist, perrb);
       }
+      // the first entry whose distance is not finite and positive
+      const uint64_t bad_m = __ballot(je < cnt && !(isfinite(pdist) && pdist > 0.0));
+      const int tbad = bad_m ? (int)__builtin_ctzll(bad_m) : 64;
       lds_fence();
       const int nb = (int)min(64u, cnt - j0);
+      // entry t's plane is read from LDS one entry ahead
+      double4 nd_next = S.u.e.nd[0];
+      double2 de_next = S.u.e.de[0];
       for (int t = 0; t < nb; t++) {
-        const double dist = rl_d(pdist, t);
-        if (!(isfinite(dist) && dist > 0.0)) {
+        if (t == tbad) {
           state = CS_ERROR;
           err = E_DEGENERATE;
           break;
         }
-        if (sr < dist) {
+        const double4 nd = nd_next;
+        const double dist = de_next.x, errb = de_next.y;
+        nd_next = S.u.e.nd[min(t + 1, 63)];
+        de_next = S.u.e.de[min(t + 1, 63)];
+        // sr < dist: every vertex has vs < dist
+        const uint64_t act_m = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+        if (!(__ballot(!(vs < dist)) & act_m)) {
           stopped = true;
           npop = j0 + (uint32_t)t + 1;
           break;
         }
-        const d3 n = mk3(rl_d(pn.x, t), rl_d(pn.y, t), rl_d(pn.z, t));
-        const double d = rl_d(pd, t);
-        const double errb = rl_d(perrb, t);
+        const d3 n = xyz(nd);
+        const double d = nd.w;
         // classify this lane's vertex (clip_by_plane, convex_cell.rs:368-390)
         const bool act = lane < nv;
         const double cv = dot3(n, mk3(vx, vy, vz)) - d;
@@ -348,7 +362,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         vx = __shfl(vx, src, 64);
         vy = __shfl(vy, src, 64);
         vz = __shfl(vz, src, 64);
-        vr = __shfl(vr, src, 64);
+        vs = __shfl(vs, src, 64);
         vd = __shfl(vd, src, 64);
         // push the plane (index np); SimpleCycle::grow is the identity
         // pointer already held by lane np
@@ -358,7 +372,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         }
         const int p_idx = np;
         if (lane == 0) {
-          S.pa[p_idx] = make_double4(n.x, n.y, n.z, d);
+          S.pa[p_idx] = nd;
           S.pb[p_idx] = S.u.e.pr[t];
         }
         np += 1;
@@ -445,7 +459,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           vx = l.x;
           vy = l.y;
           vz = l.z;
-          vr = r2;
+          vs = 2. * sqrt(r2);
           vd = (int)pack_dual(vc, vn, p_idx);
           newr = r2;
         }
@@ -460,11 +474,12 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           break;
         }
         nv += clen;
-        // update_safety_radius (convex_cell.rs:443-453): the maximum over
-        // every vertex (kept ones were checked for NaN when created)
-        sr = 2. * sqrt(wave_max_nonneg(lane < nv ? vr : 0.));
+        // update_safety_radius (convex_cell.rs:443-453) is the maximum of
+        // vs over every vertex (kept ones were checked for NaN when
+        // created); the stop test above reads the vs directly
       }
     }
+    const double sr = wave_max_nonneg(lane < nv ? vs : 0.);
     if (state == CS_DONE && !stopped) {
       // list exhausted: the first entry left out decides (its distance
       // comes with the list), else the whole stream was used
