@@ -39,10 +39,15 @@ struct WaveLds {
     // the 64 stream entries being consumed (lane t wrote entry t): plane
     // p and reference word, plane n and d, neighbour distance and the
     // plane's rounding bound
+    // and, while a clip's boundary is found, the directed edges of the
+    // removed vertices' dual triangles (bit y of emat[x]: edge x -> y) and
+    // each boundary plane's successor
     struct {
       double4 pr[64];
       double4 nd[64];
       double2 de[64];
+      uint64_t emat[64];
+      uint8_t succ[64];
     } e;
     // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
     // stays under 13 KiB: three blocks of four wavefronts fit a CU's LDS)
@@ -288,6 +293,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         S.u.e.nd[lane] = make_double4(pn.x, pn.y, pn.z, pd);
         S.u.e.de[lane] = make_double2(pdist, perrb);
       }
+      S.u.e.emat[lane] = 0ull;
       // the first entry whose distance is not finite and positive
       const uint64_t bad_m = __ballot(je < cnt && !(isfinite(pdist) && pdist > 0.0));
       const int tbad = bad_m ? (int)__builtin_ctzll(bad_m) : 64;
@@ -377,16 +383,77 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         }
         np += 1;
         // compute_boundary (convex_cell.rs:421-441) over positions
-        // num_v .. nv-1
+        // num_v .. nv-1.  The cycle it ends with is the boundary of the union
+        // of the removed vertices' dual triangles, and it starts at the
+        // first plane a0 of the triangle at num_v unless the walk enclosed
+        // that plane.  When the triangles form a disk (no directed edge
+        // twice, Euler characteristic 1) whose boundary is one cycle through
+        // a0, the lanes find the boundary edges at once; otherwise the
+        // reference's walk runs step by step.  Either way the new vertices
+        // (cur, next, p_idx) follow in cycle order from the start.
+        int vc = 0, vn = 0;
+        bool fast;
         {
+          const bool rem = lane >= num_v && lane < nv;
+          const int ta = vd & 0xFF, tb = (vd >> 8) & 0xFF, tc = (vd >> 16) & 0xFF;
+          bool dup = false;
+          if (rem) {
+            dup |= (atomicOr(&S.u.e.emat[ta], 1ull << tb) >> tb) & 1ull;
+            dup |= (atomicOr(&S.u.e.emat[tb], 1ull << tc) >> tc) & 1ull;
+            dup |= (atomicOr(&S.u.e.emat[tc], 1ull << ta) >> ta) & 1ull;
+          }
+          lds_fence();
+          // an edge is on the boundary when its reverse is in no removed
+          // triangle
+          bool bab = false, bbc = false, bca = false;
+          if (rem) {
+            bab = !((S.u.e.emat[tb] >> ta) & 1ull);
+            bbc = !((S.u.e.emat[tc] >> tb) & 1ull);
+            bca = !((S.u.e.emat[ta] >> tc) & 1ull);
+          }
+          const bool node = S.u.e.emat[lane] != 0ull;
+          lds_fence();
+          if (rem) {
+            S.u.e.emat[ta] = 0ull;
+            S.u.e.emat[tb] = 0ull;
+            S.u.e.emat[tc] = 0ull;
+          }
+          S.u.e.succ[lane] = 0xFF;
+          lds_fence();
+          if (rem) {
+            if (bab) S.u.e.succ[ta] = (uint8_t)tb;
+            if (bbc) S.u.e.succ[tb] = (uint8_t)tc;
+            if (bca) S.u.e.succ[tc] = (uint8_t)ta;
+          }
+          lds_fence();
+          const int nxt = S.u.e.succ[lane];
+          const int n_edge = __popcll(__ballot(bab)) + __popcll(__ballot(bbc)) +
+                             __popcll(__ballot(bca));
+          const int n_node = __popcll(__ballot(node));
+          // V - E + F = 1 with E = (3 F + n_edge) / 2
+          fast = !__ballot(dup) && 2 * n_node == (nv - num_v) + n_edge + 2;
+          if (fast) {
+            const int a0 = rl_i(vd, num_v) & 0xFF;
+            int cur = a0;
+            for (int r = 0; r < n_edge; r++) {
+              const int nx = rl_i(nxt, cur);
+              // a successor for every step, back at a0 after exactly n_edge
+              if (nx == 0xFF || (nx == a0) != (r == n_edge - 1)) {
+                fast = false;
+                break;
+              }
+              vc = lane == num_v + r ? cur : vc;
+              vn = lane == num_v + r ? nx : vn;
+              cur = nx;
+            }
+            clen = n_edge;
+          }
+        }
+        if (!fast) {
+          // the reference's walk; every pointer of `cyc` is the identity
+          // here
           const int d0 = rl_i(vd, num_v);
           const int a0 = d0 & 0xFF, b0 = (d0 >> 8) & 0xFF, c0 = (d0 >> 16) & 0xFF;
-          int cur = cstart;
-          for (int r = 0; r < clen; r++) {
-            const int nx = rl_i(cyc, cur);
-            wc_set(cyc, lane, cur, cur);
-            cur = nx;
-          }
           clen = 3;
           cstart = a0;
           wc_set(cyc, lane, a0, b0);
@@ -432,22 +499,19 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
             }
           }
           if (state != CS_DONE) break;
+          int cur = cstart;
+          for (int r = 0; r < clen; r++) {
+            const int nx = rl_i(cyc, cur);
+            vc = lane == num_v + r ? cur : vc;
+            vn = lane == num_v + r ? nx : vn;
+            cur = nx;
+          }
+          cyc = lane;  // SimpleCycle starts the next clip from the identity
         }
         nv = num_v;
         if (nv + clen > WCAP) {
           state = CS_CAPACITY;
           break;
-        }
-        // new vertices (cur, next, p_idx) in cycle order from the start
-        int vc = 0, vn = 0;
-        {
-          int cur = cstart;
-          for (int r = 0; r < clen; r++) {
-            const int nx = rl_i(cyc, cur);
-            vc = lane == nv + r ? cur : vc;
-            vn = lane == nv + r ? nx : vn;
-            cur = nx;
-          }
         }
         const bool mk = lane >= nv && lane < nv + clen;
         bool ok = true;
