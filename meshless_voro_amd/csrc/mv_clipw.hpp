@@ -48,6 +48,7 @@ struct WaveLds {
       double2 de[64];
       uint64_t emat[64];
       uint8_t succ[64];
+      uint8_t tab_h[64], tab_b[64];  // slots of the holes and b's by rank
     } e;
     // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
     // stays under 13 KiB: three blocks of four wavefronts fit a CU's LDS)
@@ -101,6 +102,16 @@ __device__ __forceinline__ uint64_t rl_u64(uint64_t v, int l) {
   int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffull), l);
   int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
   return ((uint64_t)(unsigned int)hi << 32) | (uint64_t)(unsigned int)lo;
+}
+// forward permutation: lane `dst` receives this lane's value (ds_permute)
+__device__ __forceinline__ int perm_i(int v, int dst) {
+  return __builtin_amdgcn_ds_permute(dst << 2, v);
+}
+__device__ __forceinline__ double perm_d(double v, int dst) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_ds_permute(dst << 2, (int)(b & 0xffffffffll));
+  int hi = __builtin_amdgcn_ds_permute(dst << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 // DPP move of a double (both halves with the same control)
 template <int CTRL>
@@ -344,32 +355,37 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         }
         uint64_t M = __ballot(clipped);
         if (!M) continue;
-        // replay of the swap-with-last scan on the mask: `src` = the slot
-        // each position takes its vertex from
-        int src = lane;
-        int i = 0, num_v = nv;
-        uint64_t m = M;
-        for (;;) {
-          // next position >= i (below num_v) holding a clipped vertex
-          const uint64_t below = num_v >= 64 ? ~0ull : ((1ull << num_v) - 1ull);
-          const uint64_t cand = m & below & (~0ull << i);
-          if (!cand) break;
-          i = __builtin_ctzll(cand);
-          num_v -= 1;
-          if (num_v != i) {
-            const int a = rl_i(src, i), b = rl_i(src, num_v);
-            src = lane == i ? b : (lane == num_v ? a : src);
-            if (!((m >> num_v) & 1ull)) {
-              m &= ~(1ull << i);
-              m |= 1ull << num_v;
-            }
-          }
-        }
-        vx = __shfl(vx, src, 64);
-        vy = __shfl(vy, src, 64);
-        vz = __shfl(vz, src, 64);
-        vs = __shfl(vs, src, 64);
-        vd = __shfl(vd, src, 64);
+        // the swap-with-last scan of clip_by_plane (convex_cell.rs:368-390)
+        // in closed form.  With K kept vertices, holes h_1 < h_2 < ... (the
+        // clipped slots below K) and b_1 > b_2 > ... (the kept slots from K
+        // up): h_j takes the vertex of b_j, the vertex of h_j goes to
+        // b_{j-1} - 1 (b_0 = nv), a clipped vertex at K goes to b_m - 1
+        // (the lowest b, else nv), and every other clipped vertex at p > K
+        // moves to p - 1; kept vertices below K stay.
+        const uint64_t kept_m = act_m & ~M;
+        const int num_v = __popcll(kept_m);  // < nv: M is not empty
+        const uint64_t low_m = (1ull << num_v) - 1ull;
+        const uint64_t holes = M & low_m, backk = kept_m & ~low_m;
+        const uint64_t below_me = (1ull << lane) - 1ull;  // lane < 64
+        const bool is_h = (holes >> lane) & 1ull, is_b = (backk >> lane) & 1ull;
+        const int jh = __popcll(holes & below_me) + 1;  // rank of a hole
+        const int jb = __popcll(backk & ~below_me);     // rank of a b, from the top
+        if (is_h) S.u.e.tab_h[jh] = (uint8_t)lane;
+        if (is_b) S.u.e.tab_b[jb] = (uint8_t)lane;
+        if (lane == 0) S.u.e.tab_b[0] = (uint8_t)nv;
+        lds_fence();
+        int dst = lane;
+        if (is_b)
+          dst = S.u.e.tab_h[jb];
+        else if (is_h)
+          dst = (int)S.u.e.tab_b[jh - 1] - 1;
+        else if ((M >> lane) & 1ull)
+          dst = lane == num_v ? (backk ? (int)__builtin_ctzll(backk) : nv) - 1 : lane - 1;
+        vx = perm_d(vx, dst);
+        vy = perm_d(vy, dst);
+        vz = perm_d(vz, dst);
+        vs = perm_d(vs, dst);
+        vd = perm_i(vd, dst);
         // push the plane (index np); SimpleCycle::grow is the identity
         // pointer already held by lane np
         if (np >= WCAP) {
