@@ -29,8 +29,8 @@ namespace mv {
 
 constexpr int WCAP = 64;  // vertices and planes per cell on this path
 // decomposition: lane 3 i + k takes plane k of the chunk's vertex i, so a
-// chunk covers 21 vertices (63 lanes)
-constexpr int WCH = 21;
+// chunk covers 16 vertices (48 lanes)
+constexpr int WCH = 16;
 
 struct WaveLds {
   double4 pa[WCAP];      // plane: n.xyz, d
@@ -51,7 +51,7 @@ struct WaveLds {
       uint8_t tab_h[64], tab_b[64];  // slots of the holes and b's by rank
     } e;
     // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
-    // stays under 13 KiB: three blocks of four wavefronts fit a CU's LDS)
+    // is 10 KiB: four blocks of four wavefronts fit a CU's 160 KiB)
     struct {
       double4 fcf[6][WCH];  // area, area * (v0 + v1 + v2)
       double4 fcv[6][WCH];  // volume, volume * (v0 + v1 + v2 + gen)
@@ -65,6 +65,11 @@ struct WaveArgs {
   Problems PR;            // boxes, grids (total stream) per problem
   const uint32_t *work;
   uint32_t nwork;
+  const uint32_t *sub;       // items to build (nullptr: all nwork) ...
+  const unsigned int *nsub;  // ... and their count, read on the device
+  uint32_t *exact_list;      // items that met a near-degenerate test ...
+  unsigned int *exact_cnt;   // ... and their count (kernels without the
+                             // exact predicate)
   const uint64_t *kpk;    // neighbour lists
   const uint64_t *kofs;   // list offset per work item (nullptr: w * L)
   const uint32_t *kcnt;
@@ -201,21 +206,25 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
   cyc = lane == at ? val : cyc;
 }
 
-// KEEP: also append every completed cell to the kept-cell store (a separate
-// instantiation, so that the plain kernel keeps its register budget)
-template <bool KEEP>
+// KEEP: also append every completed cell to the kept-cell store.  EXACT:
+// decide near-degenerate tests with the exact in-sphere predicate; without
+// it (its call holds ~30 VGPRs) the kernel fits 128 VGPRs, i.e. four
+// wavefronts per SIMD, and hands such cells (rare outside lattices) to the
+// EXACT instantiation through A.exact_list.
+template <bool KEEP, bool EXACT>
 __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
-  __shared__ WaveLds lds_all[4];  // 4 x 11.9 KiB
+  __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
   const int lane = (int)__lane_id();
   unsigned long long my_exact = 0;
 
+  const uint32_t nwork = A.nsub ? *A.nsub : A.nwork;
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(A.queue, 1u);
   next = (uint32_t)rl_i((int)next, 0);
   for (;;) {
-    const uint32_t w = next;
-    if (w >= A.nwork) break;
+    if (next >= nwork) break;
+    const uint32_t w = A.sub ? A.sub[next] : next;
     // claim the following item now; its latency overlaps this cell
     uint32_t nx = 0;
     if (lane == 0) nx = atomicAdd(A.queue, 1u);
@@ -338,7 +347,11 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         const bool near = act && fabs(cv) < errb;
         bool clipped = act && !near && cv < 0.;
         const uint64_t ref = (uint64_t)__double_as_longlong(S.u.e.pr[t].w);
-        if (__ballot(near)) {
+        if (!EXACT && __ballot(near)) {
+          state = CS_EXACT;
+          break;
+        }
+        if (EXACT && __ballot(near)) {
           d3 ngb = xyz(A.spos[pk_slot(ref)]);
           d3 sh;
           if (image_shift(pk_img(ref), B.width, &sh)) ngb = add3(ngb, sh);
@@ -690,6 +703,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
     if (lane == 0) {
       A.wstate[w] = state;
       if (state == CS_NEED_MORE) A.need_r[w] = sr;
+      if (!EXACT && state == CS_EXACT) A.exact_list[atomicAdd(A.exact_cnt, 1u)] = w;
       if (state == CS_ERROR)
         atomicMin(A.err_word, (unsigned long long)((idx << 8) | err));
     }

@@ -12,7 +12,9 @@ enum CellState : uint8_t {
   CS_DONE = 0,
   CS_NEED_MORE = 1,  // the security radius was not reached within the list
   CS_CAPACITY = 2,   // planes or vertices exceeded this kernel's capacity
-  CS_ERROR = 3       // reference would panic; code in the error word
+  CS_ERROR = 3,      // reference would panic; code in the error word
+  CS_EXACT = 4       // met a near-degenerate test; this kernel has no exact
+                     // predicate (another one rebuilds the cell)
 };
 
 // The simulation box of SimulationBoundary::cuboid (src/voronoi/boundary.rs:

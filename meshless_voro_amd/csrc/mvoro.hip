@@ -795,7 +795,9 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   }
   uint8_t *wstate = need(TT.get<uint8_t>(n));
   uint8_t *bstate = need(TT.get<uint8_t>(n));
-  unsigned int *queue = need(TT.get<unsigned int>(1));
+  // work-queue heads of the two wave clip launches and the count of cells
+  // handed from the first to the second
+  unsigned int *queue = need(TT.get<unsigned int>(3));
   const bool exact_order = (opt.flags & MV_FLAG_EXACT_ORDER) != 0;
   const uint32_t lanes_main = opt.lanes ? opt.lanes : (uint32_t)device_cus() * 512u;
   const uint32_t lanes_big = 8192;
@@ -920,17 +922,32 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.exact_counter = counters + 2;
           W.overflow = counters + 3;
           if (T->keep) W.cs = T->cs;
-          HIPCHK(hipMemsetAsync(queue, 0, sizeof(unsigned int), s));
+          HIPCHK(hipMemsetAsync(queue, 0, 3 * sizeof(unsigned int), s));
           const uint32_t bpc = opt.clip_blocks_per_cu ? opt.clip_blocks_per_cu : 4u;
           uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * bpc);
+          if (!T->keep) {
+            W.exact_list = need(TT.get<uint32_t>(cnt));
+            W.exact_cnt = queue + 1;
+          }
           Timer tcw(s, timed);
           if (T->keep)
-            hipLaunchKernelGGL(k_clip_wave<true>, dim3(blocks), dim3(256), 0, s, W);
+            hipLaunchKernelGGL((k_clip_wave<true, true>), dim3(blocks), dim3(256), 0, s, W);
           else
-            hipLaunchKernelGGL(k_clip_wave<false>, dim3(blocks), dim3(256), 0, s, W);
+            hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, W);
           HIPCHK(hipGetLastError());
           ms_cw += tcw.stop();
           n_cw++;
+          if (!T->keep) {
+            // the cells that met a near-degenerate test, with the exact
+            // predicate (the count stays on the device: an empty list costs
+            // one queue claim per wavefront)
+            WaveArgs X = W;
+            X.sub = W.exact_list;
+            X.nsub = W.exact_cnt;
+            X.queue = queue + 2;
+            hipLaunchKernelGGL((k_clip_wave<false, true>), dim3(blocks), dim3(256), 0, s, X);
+            HIPCHK(hipGetLastError());
+          }
           ms_clip += tc.stop();
           return;
         }
