@@ -40,15 +40,13 @@ struct WaveLds {
     // p and reference word, plane n and d, neighbour distance and the
     // plane's rounding bound
     // and, while a clip's boundary is found, the directed edges of the
-    // removed vertices' dual triangles (bit y of emat[x]: edge x -> y) and
-    // each boundary plane's successor
+    // removed vertices' dual triangles (bit y of emat[x]: edge x -> y)
     struct {
       double4 pr[64];
       double4 nd[64];
       double2 de[64];
       uint64_t emat[64];
-      uint8_t succ[64];
-      uint8_t tab_h[64], tab_b[64];  // slots of the holes and b's by rank
+      uint64_t rmat[64];  // bit x of rmat[y]: edge x -> y
     } e;
     // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
     // is 10 KiB: four blocks of four wavefronts fit a CU's 160 KiB)
@@ -314,6 +312,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         S.u.e.de[lane] = make_double2(pdist, perrb);
       }
       S.u.e.emat[lane] = 0ull;
+      S.u.e.rmat[lane] = 0ull;
       // the first entry whose distance is not finite and positive
       const uint64_t bad_m = __ballot(je < cnt && !(isfinite(pdist) && pdist > 0.0));
       const int tbad = bad_m ? (int)__builtin_ctzll(bad_m) : 64;
@@ -383,15 +382,17 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         const bool is_h = (holes >> lane) & 1ull, is_b = (backk >> lane) & 1ull;
         const int jh = __popcll(holes & below_me) + 1;  // rank of a hole
         const int jb = __popcll(backk & ~below_me);     // rank of a b, from the top
-        if (is_h) S.u.e.tab_h[jh] = (uint8_t)lane;
-        if (is_b) S.u.e.tab_b[jb] = (uint8_t)lane;
-        if (lane == 0) S.u.e.tab_b[0] = (uint8_t)nv;
-        lds_fence();
+        // lane j-1 receives b_j and h_j (other lanes send to lane 63, which
+        // no rank reaches: there are at most 32 pairs)
+        const int b_of = perm_i(lane, is_b ? jb - 1 : 63);
+        const int h_of = perm_i(lane, is_h ? jh - 1 : 63);
+        const int h_j = __shfl(h_of, max(jb - 1, 0), 64);
+        const int b_jm1 = __shfl(b_of, max(jh - 2, 0), 64);
         int dst = lane;
         if (is_b)
-          dst = S.u.e.tab_h[jb];
+          dst = h_j;
         else if (is_h)
-          dst = (int)S.u.e.tab_b[jh - 1] - 1;
+          dst = (jh == 1 ? nv : b_jm1) - 1;
         else if ((M >> lane) & 1ull)
           dst = lane == num_v ? (backk ? (int)__builtin_ctzll(backk) : nv) - 1 : lane - 1;
         vx = perm_d(vx, dst);
@@ -425,42 +426,34 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         {
           const bool rem = lane >= num_v && lane < nv;
           const int ta = vd & 0xFF, tb = (vd >> 8) & 0xFF, tc = (vd >> 16) & 0xFF;
-          bool dup = false;
           if (rem) {
-            dup |= (atomicOr(&S.u.e.emat[ta], 1ull << tb) >> tb) & 1ull;
-            dup |= (atomicOr(&S.u.e.emat[tb], 1ull << tc) >> tc) & 1ull;
-            dup |= (atomicOr(&S.u.e.emat[tc], 1ull << ta) >> ta) & 1ull;
+            atomicOr(&S.u.e.emat[ta], 1ull << tb);
+            atomicOr(&S.u.e.emat[tb], 1ull << tc);
+            atomicOr(&S.u.e.emat[tc], 1ull << ta);
+            atomicOr(&S.u.e.rmat[tb], 1ull << ta);
+            atomicOr(&S.u.e.rmat[tc], 1ull << tb);
+            atomicOr(&S.u.e.rmat[ta], 1ull << tc);
           }
           lds_fence();
-          // an edge is on the boundary when its reverse is in no removed
-          // triangle
-          bool bab = false, bbc = false, bca = false;
-          if (rem) {
-            bab = !((S.u.e.emat[tb] >> ta) & 1ull);
-            bbc = !((S.u.e.emat[tc] >> tb) & 1ull);
-            bca = !((S.u.e.emat[ta] >> tc) & 1ull);
-          }
-          const bool node = S.u.e.emat[lane] != 0ull;
-          lds_fence();
-          if (rem) {
-            S.u.e.emat[ta] = 0ull;
-            S.u.e.emat[tb] = 0ull;
-            S.u.e.emat[tc] = 0ull;
-          }
-          S.u.e.succ[lane] = 0xFF;
-          lds_fence();
-          if (rem) {
-            if (bab) S.u.e.succ[ta] = (uint8_t)tb;
-            if (bbc) S.u.e.succ[tb] = (uint8_t)tc;
-            if (bca) S.u.e.succ[tc] = (uint8_t)ta;
-          }
-          lds_fence();
-          const int nxt = S.u.e.succ[lane];
-          const int n_edge = __popcll(__ballot(bab)) + __popcll(__ballot(bbc)) +
-                             __popcll(__ballot(bca));
-          const int n_node = __popcll(__ballot(node));
-          // V - E + F = 1 with E = (3 F + n_edge) / 2
-          fast = !__ballot(dup) && 2 * n_node == (nv - num_v) + n_edge + 2;
+          // plane `lane`: its edges out and in; an edge is on the boundary
+          // when its reverse is in no removed triangle
+          const uint64_t eo = S.u.e.emat[lane], ei = S.u.e.rmat[lane];
+          S.u.e.emat[lane] = 0ull;
+          S.u.e.rmat[lane] = 0ull;
+          const uint64_t bo = eo & ~ei;
+          const int nbo = __popcll(bo);
+          const int nxt = nbo ? (int)__builtin_ctzll(bo) : 0xFF;
+          const int n_edge = __popcll(__ballot(nbo != 0));
+          const int n_node = __popcll(__ballot(eo != 0ull));
+          // distinct directed edges (3 per triangle unless one repeats)
+          const int pc = __popcll(eo);
+          int n_dir = 0;
+#pragma unroll
+          for (int k = 0; k < 7; k++) n_dir += __popcll(__ballot((pc >> k) & 1)) << k;
+          const int nf_t = nv - num_v;
+          // one boundary edge out of each boundary plane, V - E + F = 1 with
+          // E = (3 F + n_edge) / 2
+          fast = !__ballot(nbo > 1) && n_dir == 3 * nf_t && 2 * n_node == nf_t + n_edge + 2;
           if (fast) {
             const int a0 = rl_i(vd, num_v) & 0xFF;
             int cur = a0;

@@ -87,22 +87,22 @@ def test_swap_rule_random_masks():
 
 def kernel_boundary_rule(tris):
     """The lanes' boundary: returns the cycle from the first plane of the
-    first triangle, or None when the kernel falls back to the walk."""
-    emat = [0] * 64
-    dup = False
+    first triangle, or None when the kernel falls back to the walk.  Plane x
+    holds its out-edges (emat) and in-edges (rmat); its boundary out-edges
+    are those whose reverse is absent."""
+    emat, rmat = [0] * 64, [0] * 64
     for a, b, c in tris:
         for x, y in ((a, b), (b, c), (c, a)):
-            dup |= bool(emat[x] >> y & 1)
             emat[x] |= 1 << y
-    succ = [None] * 64
-    n_edge = 0
-    for a, b, c in tris:
-        for x, y in ((a, b), (b, c), (c, a)):
-            if not emat[y] >> x & 1:
-                succ[x] = y
-                n_edge += 1
-    n_node = sum(1 for r in emat if r)
-    if dup or 2 * n_node != len(tris) + n_edge + 2:
+            rmat[y] |= 1 << x
+    bo = [emat[x] & ~rmat[x] for x in range(64)]
+    if any(bin(m).count("1") > 1 for m in bo):
+        return None
+    succ = [(m.bit_length() - 1) if m else None for m in bo]
+    n_edge = sum(1 for m in bo if m)
+    n_node = sum(1 for m in emat if m)
+    n_dir = sum(bin(m).count("1") for m in emat)
+    if n_dir != 3 * len(tris) or 2 * n_node != len(tris) + n_edge + 2:
         return None
     a0 = tris[0][0]
     cur, seq = a0, []
