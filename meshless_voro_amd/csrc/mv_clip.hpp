@@ -43,6 +43,7 @@ struct ClipArgs {
   uint32_t *cell_nf;
   uint64_t *cell_fbase;
   uint32_t *cell_npop;
+  uint32_t *link_in;     // faces linked to each cell from a lower one
   // face staging
   uint32_t *st_cell;
   int64_t *st_right;
@@ -583,6 +584,8 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
           double fnrm = fa.x > 0. ? 1. / (3. * fa.x) : 0.;
           A.st_cell[o] = (uint32_t)c.idx;
           A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+          // a face without periodic shift is also a link of its right cell
+          if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
           A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
           A.st_area[o] = fa.x;
           A.st_cen[3 * o] = fa.y * fnrm;

@@ -80,6 +80,7 @@ struct WaveArgs {
   uint32_t *cell_nf;
   uint64_t *cell_fbase;
   uint32_t *cell_npop;
+  uint32_t *link_in;      // faces linked to each cell from a lower one
   uint32_t *st_cell;
   int64_t *st_right;
   uint8_t *st_img;
@@ -457,17 +458,21 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           if (fast) {
             const int a0 = rl_i(vd, num_v) & 0xFF;
             int cur = a0;
+            // a successor at every step, back at a0 after exactly n_edge
+            bool closed = false;
             for (int r = 0; r < n_edge; r++) {
               const int nx = rl_i(nxt, cur);
-              // a successor for every step, back at a0 after exactly n_edge
-              if (nx == 0xFF || (nx == a0) != (r == n_edge - 1)) {
-                fast = false;
+              if (nx == 0xFF) break;
+              const bool at = lane == num_v + r;
+              vc = at ? cur : vc;
+              vn = at ? nx : vn;
+              if (nx == a0) {
+                closed = r + 1 == n_edge;
                 break;
               }
-              vc = lane == num_v + r ? cur : vc;
-              vn = lane == num_v + r ? nx : vn;
               cur = nx;
             }
+            fast = closed;
             clen = n_edge;
           }
         }
@@ -718,6 +723,8 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         const double fnrm = fa > 0. ? 1. / (3. * fa) : 0.;
         A.st_cell[o] = (uint32_t)idx;
         A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+        // a face without periodic shift is also a link of its right cell
+        if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
         A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
         A.st_area[o] = fa;
         A.st_cen[3 * o] = fx * fnrm;

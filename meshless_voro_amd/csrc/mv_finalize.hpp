@@ -6,6 +6,13 @@
 // the final offsets); each face is linked to its left cell and, when it has
 // a right cell and no periodic shift, to the right cell; every cell's links
 // are ascending in face index.
+//
+// A face without shift is built by the lower of its two cells (or by the
+// active one of a partial build), so a cell's links are the faces of lower
+// cells that name it (ascending in face index because ascending in left
+// cell), then its own faces.  The clip kernels count the first kind per
+// cell (link_in); k_place_faces puts a face's own link in place and claims a
+// slot for the other; k_sort_links orders the claimed slots.
 #pragma once
 #include "mv_common.hpp"
 
@@ -27,7 +34,10 @@ struct FinalArgs {
   int64_t *face_right;
   double *face_area, *face_centroid, *face_normal, *face_shift;
   uint8_t *face_has_shift;
-  uint32_t *link_in;  // incoming links per cell
+  const uint32_t *link_in;   // links per cell from lower cells' faces
+  const uint64_t *link_off;  // exclusive scan of cell_nf + link_in
+  uint32_t *link_fill;       // slots claimed per cell (zeroed)
+  uint64_t *links;
 };
 
 __global__ void k_place_faces(FinalArgs A) {
@@ -52,7 +62,9 @@ __global__ void k_place_faces(FinalArgs A) {
   A.face_shift[3 * f] = s.x;
   A.face_shift[3 * f + 1] = s.y;
   A.face_shift[3 * f + 2] = s.z;
-  if (right >= 0 && !has) atomicAdd(&A.link_in[right], 1u);
+  A.links[A.link_off[c] + A.link_in[c] + (o - A.cell_fbase[c])] = f;
+  if (right >= 0 && !has)
+    A.links[A.link_off[right] + atomicAdd(&A.link_fill[right], 1u)] = f;
 }
 
 __global__ void k_link_count(const uint32_t *__restrict__ cell_nf,
@@ -62,31 +74,14 @@ __global__ void k_link_count(const uint32_t *__restrict__ cell_nf,
   if (c < n) link_cnt[c] = (uint64_t)cell_nf[c] + link_in[c];
 }
 
-__global__ void k_fill_links(const uint64_t *__restrict__ face_left,
-                             const int64_t *__restrict__ face_right,
-                             const uint8_t *__restrict__ face_has_shift,
-                             size_t nf, const uint64_t *__restrict__ link_off,
-                             uint32_t *__restrict__ fill,
-                             uint64_t *__restrict__ links) {
-  size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  uint64_t l = face_left[f];
-  uint32_t p = atomicAdd(&fill[l], 1u);
-  links[link_off[l] + p] = f;
-  int64_t r = face_right[f];
-  if (r >= 0 && !face_has_shift[f]) {
-    uint32_t q = atomicAdd(&fill[r], 1u);
-    links[link_off[r] + q] = f;
-  }
-}
-
-// per-cell ascending order of the links
+// ascending order of each cell's links from lower cells (claimed in any
+// order)
 __global__ void k_sort_links(const uint64_t *__restrict__ link_off,
-                             const uint64_t *__restrict__ link_cnt, size_t n,
+                             const uint32_t *__restrict__ link_in, size_t n,
                              uint64_t *__restrict__ links) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  uint64_t b = link_off[c], e = b + link_cnt[c];
+  uint64_t b = link_off[c], e = b + link_in[c];
   for (uint64_t i = b + 1; i < e; i++) {
     uint64_t v = links[i];
     uint64_t j = i;

@@ -756,6 +756,9 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   uint64_t *cell_fbase = need(TT.get<uint64_t>(n));
   HIPCHK(hipMemsetAsync(cell_nf, 0, n * sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(cell_fbase, 0, n * sizeof(uint64_t), s));
+  // links from lower cells' faces, counted as the faces are emitted
+  uint32_t *link_in = need(TT.get<uint32_t>(n));
+  HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
   uint64_t st_cap = std::max<uint64_t>(4096, 16ull * nwork);
   unsigned long long *counters = need(TT.get<unsigned long long>(4));
   // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow
@@ -908,6 +911,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.cell_cen = T->d_cell_cen;
           W.cell_sr = T->d_cell_sr;
           W.cell_nf = cell_nf;
+          W.link_in = link_in;
           W.cell_fbase = cell_fbase;
           W.cell_npop = T->d_npop;
           W.st_cell = st_cell;
@@ -989,6 +993,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.cell_cen = T->d_cell_cen;
         A.cell_sr = T->d_cell_sr;
         A.cell_nf = cell_nf;
+        A.link_in = link_in;
         A.cell_fbase = cell_fbase;
         A.cell_npop = T->d_npop;
         A.st_cell = st_cell;
@@ -1137,6 +1142,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       alloc_store();
     }
     HIPCHK(hipMemsetAsync(cell_nf, 0, n * sizeof(uint32_t), s));
+    HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
   }
   T->stats.passes = passes;
   T->stats.cells_redone = redone;
@@ -1165,9 +1171,13 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->d_face_norm = T->own<double>(3 * FF);
   T->d_face_shift = T->own<double>(3 * FF);
   T->d_face_has_shift = T->own<uint8_t>(FF);
-  uint32_t *link_in = need(TT.get<uint32_t>(n));
-  HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_link_count, dim3(nb), dim3(256), 0, s, cell_nf, link_in, n, T->d_cell_fcnt);
+  uint64_t NL = scan_excl<uint64_t, uint64_t>(T->d_cell_fcnt, T->d_cell_foff, n, s);
+  T->nl = NL;
+  T->d_links = T->own<uint64_t>(NL ? NL : 1);
   if (F) {
+    uint32_t *lfill = need(TT.get<uint32_t>(n));
+    HIPCHK(hipMemsetAsync(lfill, 0, n * sizeof(uint32_t), s));
     FinalArgs FA;
     FA.st_cell = st_cell;
     FA.st_right = st_right;
@@ -1188,21 +1198,13 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     FA.face_shift = T->d_face_shift;
     FA.face_has_shift = T->d_face_has_shift;
     FA.link_in = link_in;
+    FA.link_off = T->d_cell_foff;
+    FA.link_fill = lfill;
+    FA.links = T->d_links;
     hipLaunchKernelGGL(k_place_faces, dim3(nblk(F)), dim3(256), 0, s, FA);
+    hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff, link_in, n,
+                       T->d_links);
     HIPCHK(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_link_count, dim3(nb), dim3(256), 0, s, cell_nf, link_in, n, T->d_cell_fcnt);
-  uint64_t NL = scan_excl<uint64_t, uint64_t>(T->d_cell_fcnt, T->d_cell_foff, n, s);
-  T->nl = NL;
-  T->d_links = T->own<uint64_t>(NL ? NL : 1);
-  if (F) {
-    uint32_t *lfill = need(TT.get<uint32_t>(n));
-    HIPCHK(hipMemsetAsync(lfill, 0, n * sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_fill_links, dim3(nblk(F)), dim3(256), 0, s, T->d_face_left,
-                       T->d_face_right, T->d_face_has_shift, (size_t)F, T->d_cell_foff,
-                       lfill, T->d_links);
-    hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff,
-                       T->d_cell_fcnt, n, T->d_links);
   }
   hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos, n,
                      d_mask, (const uint64_t *)D.d_goff, (const uint32_t *)D.prob,
