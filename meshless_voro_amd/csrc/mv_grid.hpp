@@ -485,6 +485,70 @@ __device__ __forceinline__ void wave_bitonic_lds(double *key, uint64_t *pk, uint
     }
 }
 
+// Lane exchange with lane ^ J: DPP quad permutes for J = 1, 2, a swizzle
+// within 32 lanes for J = 4, 8, 16, a backward permute for J = 32.
+template <int J>
+__device__ __forceinline__ int xchg_i(int v) {
+  if constexpr (J == 1)
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  else if constexpr (J == 2)
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  else if constexpr (J < 32)
+    return __builtin_amdgcn_ds_swizzle(v, (J << 10) | 0x1F);  // and 31, xor J
+  else
+    return __shfl_xor(v, J, 64);
+}
+template <int J>
+__device__ __forceinline__ void xchg_entry(double &k, uint64_t &p, double &ko, uint64_t &po) {
+  const long long kb = __double_as_longlong(k);
+  const int klo = xchg_i<J>((int)(kb & 0xffffffffll)), khi = xchg_i<J>((int)(kb >> 32));
+  const int plo = xchg_i<J>((int)(p & 0xffffffffull)), phi = xchg_i<J>((int)(p >> 32));
+  ko = __longlong_as_double(((long long)khi << 32) | (long long)(unsigned int)klo);
+  po = ((uint64_t)(unsigned int)phi << 32) | (uint64_t)(unsigned int)plo;
+}
+// one compare-exchange stage (k, j < 64) on the H entries a lane holds
+// (entry lane + 64 h of the sequence)
+template <int K, int J, int H>
+__device__ __forceinline__ void bitonic_stage(double (&key)[H], uint64_t (&pk)[H]) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    double ko;
+    uint64_t po;
+    xchg_entry<J>(key[h], pk[h], ko, po);
+    const uint32_t i = lane + 64u * h;
+    // the lower index of a pair keeps the smaller entry in an ascending
+    // block, the larger in a descending one
+    const bool keep_min = ((i & J) == 0) == ((i & K) == 0);
+    if (keep_min != entry_less(key[h], pk[h], ko, po)) {
+      key[h] = ko;
+      pk[h] = po;
+    }
+  }
+}
+template <int K, int J, int H>
+__device__ __forceinline__ void bitonic_merge(double (&key)[H], uint64_t (&pk)[H]) {
+  if constexpr (J == 64) {
+    // pairs (lane, lane + 64) in registers (K = 128: ascending)
+    if (entry_less(key[1], pk[1], key[0], pk[0])) {
+      const double tk = key[0];
+      const uint64_t tp = pk[0];
+      key[0] = key[1];
+      pk[0] = pk[1];
+      key[1] = tk;
+      pk[1] = tp;
+    }
+  } else {
+    bitonic_stage<K, J, H>(key, pk);
+  }
+  if constexpr (J > 1) bitonic_merge<K, J / 2, H>(key, pk);
+}
+template <int K, int N, int H>
+__device__ __forceinline__ void bitonic_sort_reg(double (&key)[H], uint64_t (&pk)[H]) {
+  bitonic_merge<K, K / 2, H>(key, pk);
+  if constexpr (K < N) bitonic_sort_reg<2 * K, N, H>(key, pk);
+}
+
 template <int CAP, int WPB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
@@ -525,15 +589,37 @@ __global__ void __launch_bounds__(256)
     const uint32_t outn = min(cnt, Lw);
     uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
     if (cnt <= 64) {
-      // rank sort: each lane counts the entries smaller than its own
-      for (uint32_t e = lane; e < cnt; e += 64) {
-        const double ke = buf.key[e];
-        const uint64_t pe = buf.pk[e];
-        uint32_t rank = 0;
-        for (uint32_t t = 0; t < cnt; t++)
-          rank += entry_less(buf.key[t], buf.pk[t], ke, pe) ? 1u : 0u;
-        if (rank < outn) out[rank] = pe;
-        if (rank == outn) kdist[w] = entry_dist(spos, box.width, q, pe);
+      // bitonic network across the lanes, one entry per lane
+      double key[1] = {lane < cnt ? buf.key[lane] : INFINITY};
+      uint64_t pk[1] = {lane < cnt ? buf.pk[lane] : ~0ull};
+      bitonic_sort_reg<2, 64, 1>(key, pk);
+      if (lane < outn) out[lane] = pk[0];
+      if (lane == outn && outn < cnt) kdist[w] = entry_dist(spos, box.width, q, pk[0]);
+    } else if (cnt <= 128) {
+      // entries 0..63 and 64..127 sorted by the network one after the
+      // other, then merged: an entry's final rank is its rank in its half
+      // plus the number of smaller entries in the other half (a binary
+      // search over the other half's lanes)
+      double ka[1] = {buf.key[lane]}, kb[1] = {lane + 64 < cnt ? buf.key[lane + 64] : INFINITY};
+      uint64_t pa[1] = {buf.pk[lane]}, pb[1] = {lane + 64 < cnt ? buf.pk[lane + 64] : ~0ull};
+      bitonic_sort_reg<2, 64, 1>(ka, pa);
+      bitonic_sort_reg<2, 64, 1>(kb, pb);
+      uint32_t ra = 0, rb = 0;  // smaller entries of the other half
+#pragma unroll
+      for (uint32_t st = 64; st > 0; st >>= 1) {
+        const uint32_t ia = min(ra + st - 1, 63u), ib = min(rb + st - 1, 63u);
+        const double kbi = __shfl(kb[0], (int)ia, 64), kai = __shfl(ka[0], (int)ib, 64);
+        const uint64_t pbi = (uint64_t)__shfl((unsigned long long)pb[0], (int)ia, 64);
+        const uint64_t pai = (uint64_t)__shfl((unsigned long long)pa[0], (int)ib, 64);
+        if (ra + st - 1 < 64 && entry_less(kbi, pbi, ka[0], pa[0])) ra += st;
+        if (rb + st - 1 < 64 && entry_less(kai, pai, kb[0], pb[0])) rb += st;
+      }
+      const uint32_t posa = lane + ra, posb = lane + rb;
+      if (posa < outn) out[posa] = pa[0];
+      if (posb < outn) out[posb] = pb[0];
+      if (outn < cnt) {
+        if (posa == outn) kdist[w] = entry_dist(spos, box.width, q, pa[0]);
+        if (posb == outn) kdist[w] = entry_dist(spos, box.width, q, pb[0]);
       }
     } else {
       wave_bitonic_lds(buf.key, buf.pk, cnt);
