@@ -203,6 +203,24 @@ __global__ void k_scan_add(T *__restrict__ out, size_t n,
     if (i < n) out[i] += off;
 }
 
+// total of an exclusive scan (last output + last input), written where the
+// host reads it (pinned host memory)
+template <typename TIn, typename TOut>
+__global__ void k_scan_total(const TIn *__restrict__ last_in, const TOut *__restrict__ last_out,
+                             unsigned long long *__restrict__ total) {
+  *total = (unsigned long long)(*last_out + (TOut)*last_in);
+}
+
+// the pass counters (staging, error word, exact tests, overflow) and the
+// capacity flag, written where the host reads them
+__global__ void k_pass_words(const unsigned long long *__restrict__ counters,
+                             const unsigned int *__restrict__ cap_err,
+                             unsigned long long *__restrict__ out) {
+  const int i = threadIdx.x;
+  if (i < 4) out[i] = counters[i];
+  if (i == 4) out[4] = *cap_err;
+}
+
 // ---------------------------------------------------------------------
 // k nearest stream entries (list length L) per work item, one wavefront
 // per item.

@@ -121,10 +121,9 @@ inline unsigned nblk(size_t n, unsigned t = 256) {
   return (unsigned)((n + t - 1) / t);
 }
 
-// exclusive scan; returns the total
+// exclusive scan on the device (tile scans, then the tile offsets)
 template <typename TIn, typename TOut>
-TOut scan_excl(const TIn *in, TOut *out, size_t n, hipStream_t s) {
-  if (n == 0) return 0;
+void scan_excl_dev(const TIn *in, TOut *out, size_t n, hipStream_t s) {
   size_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   Temps T;
   TOut *sums = need(T.get<TOut>(tiles));
@@ -133,17 +132,35 @@ TOut scan_excl(const TIn *in, TOut *out, size_t n, hipStream_t s) {
   HIPCHK(hipGetLastError());
   if (tiles > 1) {
     TOut *offs = need(T.get<TOut>(tiles));
-    scan_excl<TOut, TOut>(sums, offs, tiles, s);
+    scan_excl_dev<TOut, TOut>(sums, offs, tiles, s);
     hipLaunchKernelGGL((k_scan_add<TOut>), dim3((unsigned)tiles),
                        dim3(SCAN_THREADS), 0, s, out, n, offs);
     HIPCHK(hipGetLastError());
   }
-  TIn last_in;
-  TOut last_out;
-  HIPCHK(hipMemcpyAsync(&last_in, in + n - 1, sizeof(TIn), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&last_out, out + n - 1, sizeof(TOut), hipMemcpyDeviceToHost, s));
+}
+
+// a pinned host word per host thread that kernels write totals to (one
+// small kernel and a stream synchronisation per read, no copy)
+unsigned long long *host_word() {
+  thread_local unsigned long long *w = nullptr;
+  if (!w && hipHostMalloc((void **)&w, 64, hipHostMallocDefault) != hipSuccess) {
+    w = nullptr;
+    throw BuildError{MV_E_OOM};
+  }
+  return w;
+}
+
+// exclusive scan; returns the total
+template <typename TIn, typename TOut>
+TOut scan_excl(const TIn *in, TOut *out, size_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  scan_excl_dev<TIn, TOut>(in, out, n, s);
+  unsigned long long *hw = host_word();
+  hipLaunchKernelGGL((k_scan_total<TIn, TOut>), dim3(1), dim3(1), 0, s, in + n - 1,
+                     (const TOut *)(out + n - 1), hw);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
-  return last_out + (TOut)last_in;
+  return (TOut)*(volatile unsigned long long *)hw;
 }
 
 // compaction of a work list by per-item state
@@ -1033,10 +1050,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (nbig) ms_clip += tbig.stop();
       HIPCHK(hipStreamSynchronize(s2));
       unsigned long long cnts[4];
-      unsigned int cap_err_h = 0;
-      HIPCHK(hipMemcpyAsync(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipMemcpyAsync(&cap_err_h, cap_err, sizeof cap_err_h, hipMemcpyDeviceToHost, s));
+      unsigned long long *hw = host_word();
+      hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err, hw);
+      HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(s));
+      for (int i = 0; i < 4; i++) cnts[i] = ((volatile unsigned long long *)hw)[i];
+      const unsigned long long cap_err_h = ((volatile unsigned long long *)hw)[4];
       // a cell that reached kmax entries without its security radius
       if (cap_err_h) throw BuildError{MV_E_CAPACITY};
       if (cnts[1] != ~0ull) {
