@@ -653,10 +653,29 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Global-memory variant for long lists: each wavefront owns `cap` entries of
-// scratch, sorts them with a bitonic network (padded to a power of two with
-// +inf keys) and writes the first L.
-__global__ void __launch_bounds__(256)
+// Global-memory variant for long lists: one block of 1024 threads per item
+// owns `cap` entries of scratch.  Its first wavefront gathers the ball; the
+// block sorts it with a bitonic network padded to a power of two with +inf
+// keys, doing the stages that pair entries less than KB_CHUNK apart in LDS
+// (one chunk at a time) and only the wider ones in global memory, and
+// writes the first L.
+constexpr int KB_THREADS = 1024;
+constexpr uint32_t KB_CHUNK = 2048;  // entries per LDS chunk (32 KiB)
+
+__device__ __forceinline__ void cmpx(double &ka, uint64_t &pa, double &kb, uint64_t &pb,
+                                     bool up) {
+  const bool sw = up ? entry_less(kb, pb, ka, pa) : entry_less(ka, pa, kb, pb);
+  if (sw) {
+    const double tk = ka;
+    const uint64_t tp = pa;
+    ka = kb;
+    pa = pb;
+    kb = tk;
+    pb = tp;
+  }
+}
+
+__global__ void __launch_bounds__(KB_THREADS)
     k_knn_big(const double4 *__restrict__ spos,
               const uint32_t *__restrict__ cell_start_all, Problems PR,
               const uint32_t *__restrict__ work,
@@ -666,67 +685,116 @@ __global__ void __launch_bounds__(256)
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
               uint32_t cap) {
+  __shared__ double lkey[KB_CHUNK];
+  __shared__ uint64_t lpk[KB_CHUNK];
+  __shared__ uint32_t s_cnt;
+  __shared__ double s_R;
+  const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
-  const uint32_t wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
-  double *bkey = gkey + (size_t)wid * cap;
-  uint64_t *bpk = gpk + (size_t)wid * cap;
-  for (uint32_t w = wid; w < nwork; w += nwaves) {
-    // wave-uniform: keep them in scalar registers
-    const uint32_t slot = __builtin_amdgcn_readfirstlane(work[w]);
-    const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(PR, slot));
-    // the problem's parameters in registers (wave-uniform scalar loads)
-    const GridParams g = PR.grids[pr];
+  double *bkey = gkey + (size_t)blockIdx.x * cap;
+  uint64_t *bpk = gpk + (size_t)blockIdx.x * cap;
+  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const uint32_t slot = work[w];
+    const uint32_t pr = problem_of(PR, slot);
     const BoxParams &boxr = PR.boxes[pr];
-    struct {
-      double width[3];
-    } box = {{boxr.width[0], boxr.width[1], boxr.width[2]}};
-    const uint32_t *cell_start = cell_start_all + PR.coff[pr];
+    const double width[3] = {boxr.width[0], boxr.width[1], boxr.width[2]};
     const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
-    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
-                                       : knn_start_radius(cell_start, g, q, Lw);
-    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lw, bkey, bpk, cap);
+    if (tid < 64) {
+      // wave-uniform: keep them in scalar registers
+      const uint32_t pru = __builtin_amdgcn_readfirstlane(pr);
+      const GridParams g = PR.grids[pru];
+      const uint32_t *cell_start = cell_start_all + PR.coff[pru];
+      double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
+                                         : knn_start_radius(cell_start, g, q, Lw);
+      const uint32_t cnt = knn_fill(spos, cell_start, g, width, q, R, Lw, bkey, bpk, cap);
+      if (lane == 0) {
+        s_cnt = cnt;
+        s_R = R;
+      }
+    }
+    __syncthreads();
+    const uint32_t cnt = s_cnt;
+    const double R = s_R;
     if (cnt > cap) {
-      if (lane == 0) kcnt[w] = 0xFFFFFFFFu;
+      if (tid == 0) kcnt[w] = 0xFFFFFFFFu;
+      __syncthreads();
       continue;
     }
-    uint32_t n2 = 64;
+    uint32_t n2 = KB_CHUNK;
     while (n2 < cnt) n2 <<= 1;
-    for (uint32_t e = cnt + lane; e < n2; e += 64) {
+    for (uint32_t e = cnt + tid; e < n2; e += KB_THREADS) {
       bkey[e] = INFINITY;
       bpk[e] = ~0ull;
     }
-    __threadfence();
-    for (uint32_t k = 2; k <= n2; k <<= 1)
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = lane; i < n2; i += 64) {
-          uint32_t l = i ^ j;
-          if (l > i) {
-            double ki = bkey[i], kl = bkey[l];
-            uint64_t pi = bpk[i], pl = bpk[l];
-            bool up = (i & k) == 0;
-            bool sw = up ? entry_less(kl, pl, ki, pi) : entry_less(ki, pi, kl, pl);
-            if (sw) {
-              bkey[i] = kl;
-              bkey[l] = ki;
-              bpk[i] = pl;
-              bpk[l] = pi;
-            }
-          }
-        }
-        __threadfence();
+    __syncthreads();
+    // stages jj < KB_CHUNK of level k on one chunk held in LDS
+    auto chunk_stages = [&](uint32_t c0, uint32_t k, uint32_t j_hi) {
+      for (uint32_t jj = j_hi; jj > 0; jj >>= 1) {
+        const uint32_t i = ((tid & ~(jj - 1)) << 1) | (tid & (jj - 1));  // pair tid
+        double ki = lkey[i], kl = lkey[i | jj];
+        uint64_t pi = lpk[i], pl = lpk[i | jj];
+        cmpx(ki, pi, kl, pl, ((c0 + i) & k) == 0);
+        lkey[i] = ki;
+        lpk[i] = pi;
+        lkey[i | jj] = kl;
+        lpk[i | jj] = pl;
+        __syncthreads();
       }
+    };
+    auto load_chunk = [&](uint32_t c0) {
+      for (uint32_t e = tid; e < KB_CHUNK; e += KB_THREADS) {
+        lkey[e] = bkey[c0 + e];
+        lpk[e] = bpk[c0 + e];
+      }
+      __syncthreads();
+    };
+    auto store_chunk = [&](uint32_t c0) {
+      for (uint32_t e = tid; e < KB_CHUNK; e += KB_THREADS) {
+        bkey[c0 + e] = lkey[e];
+        bpk[c0 + e] = lpk[e];
+      }
+      __syncthreads();
+    };
+    // levels up to the chunk size: each chunk sorted in LDS (alternating
+    // directions, as the network has them)
+    for (uint32_t c0 = 0; c0 < n2; c0 += KB_CHUNK) {
+      load_chunk(c0);
+      for (uint32_t k = 2; k <= KB_CHUNK; k <<= 1) chunk_stages(c0, k, k >> 1);
+      store_chunk(c0);
+    }
+    // wider levels: stages pairing entries a chunk or more apart in global
+    // memory, the rest chunk by chunk in LDS
+    for (uint32_t k = 2 * KB_CHUNK; k <= n2; k <<= 1) {
+      for (uint32_t j = k >> 1; j >= KB_CHUNK; j >>= 1) {
+        for (uint32_t p = tid; p < (n2 >> 1); p += KB_THREADS) {
+          const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+          double ki = bkey[i], kl = bkey[i | j];
+          uint64_t pi = bpk[i], pl = bpk[i | j];
+          cmpx(ki, pi, kl, pl, (i & k) == 0);
+          bkey[i] = ki;
+          bpk[i] = pi;
+          bkey[i | j] = kl;
+          bpk[i | j] = pl;
+        }
+        __syncthreads();
+      }
+      for (uint32_t c0 = 0; c0 < n2; c0 += KB_CHUNK) {
+        load_chunk(c0);
+        chunk_stages(c0, k, KB_CHUNK >> 1);
+        store_chunk(c0);
+      }
+    }
     const uint32_t outn = min(cnt, Lw);
     uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
-    for (uint32_t e = lane; e < outn; e += 64) out[e] = bpk[e];
-    if (lane == 0) {
+    for (uint32_t e = tid; e < outn; e += KB_THREADS) out[e] = bpk[e];
+    if (tid == 0) {
       kcnt[w] = outn;
-      kdist[w] = cnt > outn ? entry_dist(spos, box.width, q, bpk[outn])
-                            : ball_certificate(q, R, box.width);
+      kdist[w] = cnt > outn ? entry_dist(spos, width, q, bpk[outn])
+                            : ball_certificate(q, R, width);
     }
-    __threadfence();
+    __syncthreads();
   }
 }
 
