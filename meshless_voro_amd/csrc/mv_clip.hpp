@@ -188,9 +188,18 @@ __device__ __forceinline__ bool cyc_try_extend(const Lane &S, CellRegs &c,
   return false;
 }
 
+// Maximum over the wavefront (every lane gets it)
+__device__ __forceinline__ double wave_max_all(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 // ConvexCell::clip_by_plane (src/voronoi/convex_cell.rs:358-419).
 // Returns 0, E_* for a reference panic, or 255 for capacity overflow.
-template <int MAXP, int MAXV>
+// COOP: the wavefront builds one cell; its lanes classify the vertices 64 at
+// a time and every lane then runs the sequential steps on the same data.
+template <int MAXP, int MAXV, bool COOP = false>
 __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
                                   const ClipArgs &A, d3 n, d3 p, double d,
                                   double errb, uint64_t ref, d3 ngb) {
@@ -205,7 +214,38 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
   for (int k = 0; k < MW; k++) m[k] = 0;
   double keepmax = -INFINITY;
   bool nan_keep = false;
-  for (int i = 0; i < c.nv; i++) {
+  if constexpr (COOP) {
+    const int lane = (int)__lane_id();
+    for (int i0 = 0; i0 < c.nv; i0 += 64) {
+      const int i = i0 + lane;
+      bool clipped = false, kept = false;
+      double r2 = 0.;
+      if (i < c.nv) {
+        const double4 v = S.vtx[S.at(i)];
+        const double cv = dot3(n, xyz(v)) - d;
+        if (fabs(cv) < errb) {
+          const uint32_t du = S.vdual[S.at(i)];
+          int64_t ia[3], ib[3], ic[3], id[3], iv[3];
+          iloc(*c.B, c.q, ia);
+          iloc(*c.B, right_loc(S, c, A, dual_at(du, 0)), ib);
+          iloc(*c.B, right_loc(S, c, A, dual_at(du, 1)), ic);
+          iloc(*c.B, right_loc(S, c, A, dual_at(du, 2)), id);
+          iloc(*c.B, ngb, iv);
+          clipped = in_sphere_exact(ia, ib, ic, id, iv) < 0;
+          c.nexact++;
+        } else {
+          clipped = cv < 0.;
+        }
+        kept = !clipped;
+        r2 = v.w;
+      }
+      m[i0 >> 6] = __ballot(clipped);
+      nan_keep |= __ballot(kept && isnan(r2)) != 0ull;
+      // kept radii are >= 0; fmax drops NaN as the sequential loop does
+      keepmax = fmax(keepmax, wave_max_all(kept && !isnan(r2) ? r2 : -INFINITY));
+    }
+  }
+  for (int i = 0; !COOP && i < c.nv; i++) {
     double4 v = S.vtx[S.at(i)];
     double cv = dot3(n, xyz(v)) - d;
     bool clipped;
@@ -325,7 +365,7 @@ __device__ __forceinline__ bool should_construct(const Lane &S,
 }
 
 // Runs one cell.  Returns its CellState; on CS_DONE, *nf = faces to emit.
-template <int MAXP, int MAXV>
+template <int MAXP, int MAXV, bool COOP = false>
 __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
                               uint32_t w, uint32_t *nf, uint32_t *npop,
                               uint32_t *err) {
@@ -391,7 +431,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
     // HalfSpace::new (half_space.rs:24-34)
     double errb = 1e-13 * (1. + dot3(abs3(n), abs3(p)));
     double d = dot3(n, p);
-    uint32_t r = clip_by_plane<MAXP, MAXV>(S, c, A, n, p, d, errb, pk, ngb);
+    uint32_t r = clip_by_plane<MAXP, MAXV, COOP>(S, c, A, n, p, d, errb, pk, ngb);
     if (r == 255) return CS_CAPACITY;
     if (r) {
       *err = r;
@@ -600,6 +640,84 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
     }
   }
   if (my_exact) atomicAdd(A.exact_counter, my_exact);
+}
+
+// One cell per wavefront, for the cells over the wavefront kernel's 64
+// vertices or planes (long lists, large cells).  Every lane runs the same
+// sequential steps on the cell's arrays (slot = wavefront index, so the
+// lanes of a wavefront share them and store the same values); the vertex
+// classification of each plane is split over the lanes.  Results, faces
+// and counters are written by lane 0.
+template <int MAXP, int MAXV, bool KEEP>
+__global__ void __launch_bounds__(256) k_clip_coop(ClipArgs A) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  Lane S;
+  S.plA = A.plA;
+  S.plB = A.plB;
+  S.vtx = A.vtx;
+  S.facc = A.facc;
+  S.vdual = A.vdual;
+  S.cyc = A.cyc;
+  S.lane = wid;
+  S.lanes = A.lanes;
+  unsigned long long my_exact = 0;
+  for (uint32_t w = wid; w < A.nwork; w += nwaves) {
+    uint32_t nf = 0;
+    CellRegs c;
+    const uint32_t slot = A.work[w];
+    const uint32_t pr = problem_of(A.PR, slot);
+    c.B = &A.PR.boxes[pr];
+    c.total_stream = A.PR.grids[pr].total_stream;
+    const double4 qq = A.spos[slot];
+    c.q = xyz(qq);
+    c.idx = (uint64_t)__double_as_longlong(qq.w);
+    uint32_t err = 0, npop = 0;
+    const uint8_t st = build_cell<MAXP, MAXV, true>(S, c, A, w, &nf, &npop, &err);
+    my_exact += c.nexact;
+    const bool emit = st == CS_DONE;
+    if (lane == 0) {
+      A.wstate[w] = st;
+      if (st == CS_NEED_MORE) A.need_r[w] = c.sr;
+      if (st == CS_ERROR) atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
+      if (emit) A.cell_npop[c.idx] = npop;
+    }
+    if (KEEP) keep_cell(S, A, emit && lane == 0, c);
+    if (emit && lane == 0) {
+      const unsigned long long fb = nf ? atomicAdd(A.st_counter, (unsigned long long)nf) : 0ull;
+      A.cell_nf[c.idx] = nf;
+      A.cell_fbase[c.idx] = fb;
+      if (fb + nf > A.st_cap) {
+        atomicOr(A.overflow, 1ull);
+      } else {
+        uint64_t o = fb;
+        for (int k = 0; k < c.np; k++) {
+          if (S.cyc[S.at(k)] != 3) continue;
+          const double4 a = S.plA[S.at(k)];
+          const uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(k)].w);
+          const double4 fa = S.facc[S.at(k)];
+          const double fnrm = fa.x > 0. ? 1. / (3. * fa.x) : 0.;
+          A.st_cell[o] = (uint32_t)c.idx;
+          A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+          // a face without periodic shift is also a link of its right cell
+          if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
+          A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
+          A.st_area[o] = fa.x;
+          A.st_cen[3 * o] = fa.y * fnrm;
+          A.st_cen[3 * o + 1] = fa.z * fnrm;
+          A.st_cen[3 * o + 2] = fa.w * fnrm;
+          A.st_norm[3 * o] = a.x;
+          A.st_norm[3 * o + 1] = a.y;
+          A.st_norm[3 * o + 2] = a.z;
+          o++;
+        }
+      }
+    }
+  }
+  // every lane counted every exact test of its vertices
+  for (int o = 32; o > 0; o >>= 1) my_exact += __shfl_xor(my_exact, o, 64);
+  if (lane == 0 && my_exact) atomicAdd(A.exact_counter, my_exact);
 }
 
 }  // namespace mv

@@ -211,7 +211,8 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
 // wavefronts per SIMD, and hands such cells (rare outside lattices) to the
 // EXACT instantiation through A.exact_list.
 template <bool KEEP, bool EXACT>
-__global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_clip_wave(WaveArgs A) {
   __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
   const int lane = (int)__lane_id();
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
         // push the plane (index np); SimpleCycle::grow is the identity
         // pointer already held by lane np
         if (np >= WCAP) {
-          state = CS_CAPACITY;
+          state = EXACT || KEEP ? CS_CAPACITY : CS_EXACT;
           break;
         }
         const int p_idx = np;
@@ -565,6 +566,42 @@ __global__ void __launch_bounds__(256) k_clip_wave(WaveArgs A) {
           break;
         }
         nv += clen;
+        if (EXACT && !KEEP && np >= WCAP) {
+          // the plane list is full: drop the planes no vertex refers to
+          // (they build no face and enter no later vertex), keeping the
+          // order of the others -- faces follow plane order, so the outputs
+          // are unchanged.  The kept-cell store keeps the reference's plane
+          // indices, so KEEP builds hand such cells to the larger kernel;
+          // the kernel without the exact predicate hands them to the one
+          // with it (this code would cost it a wavefront per SIMD).
+          const bool act = lane < nv;
+          const int t0 = vd & 0xFF, t1 = (vd >> 8) & 0xFF, t2 = (vd >> 16) & 0xFF;
+          // the planes the vertices refer to, ORed in an edge-mask word
+          // (zero between boundary steps)
+          if (act) atomicOr(&S.u.e.emat[0], (1ull << t0) | (1ull << t1) | (1ull << t2));
+          lds_fence();
+          const uint64_t live = S.u.e.emat[0];
+          lds_fence();
+          if (lane == 0) S.u.e.emat[0] = 0ull;
+          const int np_new = __popcll(live);
+          if (np_new < np) {
+            auto renum = [&](int x) { return __popcll(live & ((1ull << x) - 1ull)); };
+            if (act) vd = renum(t0) | (renum(t1) << 8) | (renum(t2) << 16);
+            // one double of every plane at a time (few registers live)
+            const bool mv_pl = lane < np && ((live >> lane) & 1ull);
+            const int to = renum(lane);
+#pragma unroll 1
+            for (int c = 0; c < 8; c++) {
+              double *src = c < 4 ? &S.pa[lane].x : &S.pb[lane].x;
+              double *dst = c < 4 ? &S.pa[to].x : &S.pb[to].x;
+              const double v = mv_pl ? src[c & 3] : 0.;
+              lds_fence();
+              if (mv_pl) dst[c & 3] = v;
+              lds_fence();
+            }
+            np = np_new;
+          }
+        }
         // update_safety_radius (convex_cell.rs:443-453) is the maximum of
         // vs over every vertex (kept ones were checked for NaN when
         // created); the stop test above reads the vs directly

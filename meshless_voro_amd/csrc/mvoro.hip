@@ -1025,14 +1025,18 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.overflow = counters + 3;
         if (T->keep) A.cs = T->cs;
         const dim3 grid(lanes / 256);
+        // large cells: one cell per wavefront (a slot per wavefront)
+        const dim3 grid_coop((std::min<uint32_t>(cnt, A.lanes) + 3) / 4);
         if (!big && !T->keep)
           hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, false>), grid, dim3(256), 0, s, A);
         else if (!big)
           hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, true>), grid, dim3(256), 0, s, A);
         else if (!T->keep)
-          hipLaunchKernelGGL((k_clip<MAXP_BIG, MAXV_BIG, false>), grid, dim3(256), 0, s, A);
+          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>), grid_coop, dim3(256), 0,
+                             s, A);
         else
-          hipLaunchKernelGGL((k_clip<MAXP_BIG, MAXV_BIG, true>), grid, dim3(256), 0, s, A);
+          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, true>), grid_coop, dim3(256), 0,
+                             s, A);
         HIPCHK(hipGetLastError());
         ms_clip += tc.stop();
       };
