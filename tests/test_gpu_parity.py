@@ -267,3 +267,5 @@ def test_clusters_in_empty_space_need_long_lists():
     check(g, o)
     assert np.array_equal(g.cell_n_popped, o.cell_n_popped)
     assert o.cell_n_popped.max() > 32768
+    # the rim cells went through the large-cell kernel (over 64 vertices)
+    assert g.stats["big_cells"] > 0
