@@ -211,6 +211,23 @@ __global__ void k_scan_total(const TIn *__restrict__ last_in, const TOut *__rest
   *total = (unsigned long long)(*last_out + (TOut)*last_in);
 }
 
+// largest of n values (one block), written where the host reads it
+__global__ void __launch_bounds__(1024) k_max_u32(const uint32_t *__restrict__ v, uint32_t n,
+                                                  unsigned long long *__restrict__ out) {
+  __shared__ uint32_t part[32];
+  uint32_t m = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = max(m, v[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); w++) r = max(r, part[w]);
+    *out = r;
+  }
+}
+
 // the pass counters (staging, error word, exact tests, overflow) and the
 // capacity flag, written where the host reads them
 __global__ void k_pass_words(const unsigned long long *__restrict__ counters,
@@ -798,36 +815,44 @@ __global__ void __launch_bounds__(KB_THREADS)
   }
 }
 
-// Cells to redo get a longer list: doubled while it is short, quadrupled
-// once it holds count_from entries or more (cells whose security radius
-// spans a large part of the problem reach their length in fewer passes).
-// Their gather starts from the local density (a start at the safety radius
-// of a young cell would gather far more entries than the list holds).  A
-// cell already at kmax sets *cap_err.
+// Cells to redo get a longer list, sized from the cell's safety radius sr
+// when its list ran out and the distance kd of the first entry left out
+// (sr >= kd, else the cell would have stopped): the entries within sr are
+// about L (sr / kd)^3 for a locally uniform density; the length grows by that
+// estimate (+10 %), at least 2x and at most 4x per pass, and the next gather
+// starts at the radius expected to hold it.
 __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
-                            float *__restrict__ Rout, uint32_t kmax, uint32_t count_from,
+                            const double *__restrict__ sr, const double *__restrict__ kd,
+                            float *__restrict__ Rout, uint32_t kmax,
                             unsigned int *__restrict__ cap_err) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork) return;
   const uint32_t Lp = Lio[w];
   if (Lp >= kmax) atomicOr(cap_err, 1u);
-  const uint64_t mult = Lp < count_from ? 2 : 4;
-  Lio[w] = (uint32_t)min(mult * Lp, (uint64_t)kmax);
-  Rout[w] = 0.f;
+  const double k = kd[w], r = sr[w];
+  const double ratio = (k > 0. && r > k && isfinite(r)) ? r / k : 1.;
+  const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
+  const double lo = 2. * (double)Lp, hi = 4. * (double)Lp;
+  const double Ln = fmin(fmax(est, lo), hi);
+  const uint32_t Lnew = (uint32_t)fmin(Ln, (double)kmax);
+  Lio[w] = Lnew;
+  Rout[w] = (k > 0. && isfinite(k)) ? (float)(k * cbrt((double)Lnew / (double)Lp)) : 0.f;
 }
 
 __global__ void k_compact_redo(const uint32_t *__restrict__ in_w,
                                const uint32_t *__restrict__ in_L,
                                const double *__restrict__ in_r,
+                               const double *__restrict__ in_kd,
                                const uint32_t *__restrict__ flag,
                                const uint32_t *__restrict__ pos, uint32_t n,
                                uint32_t *__restrict__ out_w, uint32_t *__restrict__ out_L,
-                               double *__restrict__ out_r) {
+                               double *__restrict__ out_r, double *__restrict__ out_kd) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && flag[i]) {
     out_w[pos[i]] = in_w[i];
     out_L[pos[i]] = in_L[i];
     out_r[pos[i]] = in_r[i];
+    out_kd[pos[i]] = in_kd[i];
   }
 }
 

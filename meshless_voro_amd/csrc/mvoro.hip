@@ -888,16 +888,25 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       passes++;
       double *nr_main = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *nr_big = need(TT.get<double>(std::max<uint32_t>(nbig, 1)));
+      const double *kd_main = nullptr, *kd_big = nullptr;  // this pass's kdist
       // one list on one stream; untimed runs do not wait for their kernels
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
                      uint8_t *state, bool big, hipStream_t s, bool timed) {
         double *need_r = big ? nr_big : nr_main;
         Timer tk(s, timed);
         uint64_t *kofs = need(TT.get<uint64_t>(cnt));
+        // redone lists: the longest one picks the list kernel (the growth
+        // bound is loose); read with the scan's total
+        if (passes > 1)
+          hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s, (const uint32_t *)ll, cnt,
+                             host_word() + 1);
         const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
+        if (passes > 1)
+          lbnd = std::min<uint32_t>(lbnd, (uint32_t)((volatile unsigned long long *)host_word())[1]);
         uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
+        (big ? kd_big : kd_main) = kdist;
         Timer tkk(s, timed);
         launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
                    s);
@@ -1078,7 +1087,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       // longer list (k_redo_size: doubled, or quadrupled once long)
       auto take = [&](const uint32_t *w, const uint32_t *L, const uint8_t *st, uint32_t cnt,
                       uint8_t want, uint32_t *ow, uint32_t *oL, const double *nr,
-                      double *onr) -> uint32_t {
+                      double *onr, const double *kd, double *okd) -> uint32_t {
         if (cnt == 0) return 0;
         Temps T2;
         uint32_t *fl = need(T2.get<uint32_t>(cnt));
@@ -1086,8 +1095,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         hipLaunchKernelGGL(k_flag_state, dim3(nblk(cnt)), dim3(256), 0, s, st, cnt, want, fl);
         uint32_t c = scan_excl<uint32_t, uint32_t>(fl, ps, cnt, s);
         if (c && nr)
-          hipLaunchKernelGGL(k_compact_redo, dim3(nblk(cnt)), dim3(256), 0, s, w, L, nr, fl, ps,
-                             cnt, ow, oL, onr);
+          hipLaunchKernelGGL(k_compact_redo, dim3(nblk(cnt)), dim3(256), 0, s, w, L, nr, kd, fl,
+                             ps, cnt, ow, oL, onr, okd);
         else if (c)
           hipLaunchKernelGGL(k_compact_pairs, dim3(nblk(cnt)), dim3(256), 0, s, w, L, fl, ps,
                              cnt, 1u, kmax, ow, oL);
@@ -1098,25 +1107,27 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       uint32_t *cb2 = lists[ping_b], *cbl2 = lens[ping_b];
       double *rr_m = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *rr_b = need(TT.get<double>(std::max<uint32_t>(nw + nbig, 1)));
-      const uint32_t nw_next = take(cw, cl, wstate, nw, CS_NEED_MORE, cw2, cl2, nr_main, rr_m);
+      double *kk_m = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
+      double *kk_b = need(TT.get<double>(std::max<uint32_t>(nw + nbig, 1)));
+      const uint32_t nw_next =
+          take(cw, cl, wstate, nw, CS_NEED_MORE, cw2, cl2, nr_main, rr_m, kd_main, kk_m);
       const uint32_t nb_next =
-          take(cb, cbl, bstate, nbig, CS_NEED_MORE, cb2, cbl2, nr_big, rr_b);
+          take(cb, cbl, bstate, nbig, CS_NEED_MORE, cb2, cbl2, nr_big, rr_b, kd_big, kk_b);
       const uint32_t ncap = take(cw, cl, wstate, nw, CS_CAPACITY, cb2 + nb_next, cbl2 + nb_next,
-                                 nullptr, nullptr);
+                                 nullptr, nullptr, nullptr, nullptr);
       const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, cw2 + nw_next,
-                                  cl2 + nw_next, nullptr, nullptr);
+                                  cl2 + nw_next, nullptr, nullptr, nullptr, nullptr);
       if (nbcap) throw BuildError{MV_E_CAPACITY};
       float *Rm_next = need(TT.get<float>(std::max<uint32_t>(nw_next, 1)));
       float *Rb_next = need(TT.get<float>(std::max<uint32_t>(nb_next + ncap, 1)));
-      // doubling throughout: quadrupling long lists (COUNT_FROM) made the
-      // clustered many-problem workload slower (its huge lists overshoot)
-      constexpr uint32_t GROW = 4, COUNT_FROM = 0xFFFFFFFFu;
+      // 2x to 4x per pass, from the safety radius at exhaustion
+      constexpr uint32_t GROW = 4;
       if (nw_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
-                           Rm_next, kmax, COUNT_FROM, cap_err);
+                           (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err);
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
-                           Rb_next, kmax, COUNT_FROM, cap_err);
+                           (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err);
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
@@ -1125,9 +1136,9 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       redone += nw_next + nb_next;
       if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
       const uint32_t lb_main_old = lb_main;
-      // bounds on the new lengths (k_redo_size doubles below COUNT_FROM)
+      // bounds on the new lengths (k_redo_size grows them at most GROW x)
       auto grown = [&](uint32_t lb) {
-        return (uint32_t)std::min<uint64_t>((uint64_t)(lb < COUNT_FROM ? 2 : GROW) * lb, kmax);
+        return (uint32_t)std::min<uint64_t>((uint64_t)GROW * lb, kmax);
       };
       lb_main = grown(lb_main);
       lb_big = std::max(grown(lb_big), std::min(lb_main_old, kmax));
