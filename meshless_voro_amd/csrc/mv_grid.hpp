@@ -614,7 +614,11 @@ __global__ void __launch_bounds__(256)
     const uint32_t Lw = Lper ? Lper[w] : L;
     double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
                                        : knn_start_radius(cell_start, g, q, Lw);
-    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lw, buf.key,
+    // a ball with three quarters of the requested entries is kept (its list
+    // ends at the ball, certified by its radius) rather than gathered again:
+    // a cell that needs more is redone with a longer list anyway
+    const uint32_t Lacc = max(1u, (Lw * 3u) / 4u);
+    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lacc, buf.key,
                             buf.pk, CAP);
     __builtin_amdgcn_wave_barrier();
     if (cnt > (uint32_t)CAP) {

@@ -86,43 +86,90 @@ __global__ void __launch_bounds__(256)
     boff[i][0] = boff[i][1] = boff[i][2] = 0.;
   }
   uint32_t n1 = 0, n27 = 0, n125 = 0;
-  for (int da = -2; da <= 2; da++)
-    for (int db = -2; db <= 2; db++)
-      for (int dc = -2; dc <= 2; dc++) {
-        int cc[3] = {C[0] + da, C[1] + db, C[2] + dc};
-        int dd[3] = {da, db, dc};
-        double sh[3] = {0., 0., 0.};
-        bool skip = false;
-        for (int c = 0; c < 3; c++) {
-          if (g.inv_h[c] == 0.) {
-            if (dd[c] != 0) skip = true;
-            continue;
-          }
-          if (cc[c] < 0 || cc[c] >= g.G[c]) {
-            if (!box.periodic || g.G[c] < 5) {
-              skip = true;
-            } else if (cc[c] < 0) {
-              cc[c] += g.G[c];
-              sh[c] = -box.width[c];
-            } else {
-              cc[c] -= g.G[c];
-              sh[c] = box.width[c];
+  // cell index along axis c of offset dd from C, with the periodic wrap of
+  // 5+ cell axes (shift into *sh); false when the cell does not exist
+  auto axis_cell = [&](int c, int dd, int *cc, double *sh) -> bool {
+    *sh = 0.;
+    if (g.inv_h[c] == 0.) {
+      *cc = 0;
+      return dd == 0;
+    }
+    int x = C[c] + dd;
+    if (x < 0 || x >= g.G[c]) {
+      if (!box.periodic || g.G[c] < 5) return false;
+      if (x < 0) {
+        x += g.G[c];
+        *sh = -box.width[c];
+      } else {
+        x -= g.G[c];
+        *sh = box.width[c];
+      }
+    }
+    *cc = x;
+    return true;
+  };
+  for (int da = -2; da <= 2; da++) {
+    int ca;
+    double sa;
+    if (!axis_cell(0, da, &ca, &sa)) continue;
+    for (int db = -2; db <= 2; db++) {
+      int cb;
+      double sb;
+      if (!axis_cell(1, db, &cb, &sb)) continue;
+      const size_t row = ((size_t)ca * g.G[1] + cb) * g.G[2];
+      const bool inner_row = abs(da) <= 1 && abs(db) <= 1;
+      // cells C2-2 .. C2+2 of the row: one range of the sorted seeds when
+      // none wraps (cell_start is a prefix count along the row)
+      const bool flat = g.inv_h[2] != 0. && C[2] - 2 >= 0 && C[2] + 2 < g.G[2];
+      if (flat) {
+        const uint32_t e0 = cell_start[row + C[2] - 2], e1 = cell_start[row + C[2] - 1];
+        const uint32_t e4 = cell_start[row + C[2] + 2], e5 = cell_start[row + C[2] + 3];
+        const uint32_t self = (da == 0 && db == 0) ? 1u : 0u;
+        n125 += e5 - e0 - self;
+        if (!inner_row) continue;
+        n27 += e4 - e1 - self;
+        if (da == 0 && db == 0)
+          n1 += cell_start[row + C[2] + 1] - cell_start[row + C[2]] - 1u;
+        // the three inner cells in order: one range, no shift along c
+        for (uint32_t t = e1; t < e4; t++) {
+          if (t == slot) continue;
+          const double4 p = spos[t];
+          const double o[3] = {p.x + sa - q[0], p.y + sb - q[1], p.z - q[2]};
+          double d2 = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
+          if (!(d2 < best[7])) continue;
+          double v = d2;
+          double vo[3] = {o[0], o[1], o[2]};
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            if (v < best[i]) {
+              double tv = best[i];
+              best[i] = v;
+              v = tv;
+              for (int c = 0; c < 3; c++) {
+                double to = boff[i][c];
+                boff[i][c] = vo[c];
+                vo[c] = to;
+              }
             }
           }
         }
-        if (skip) continue;
-        const size_t cell = ((size_t)cc[0] * g.G[1] + cc[1]) * g.G[2] + cc[2];
-        const uint32_t t0 = cell_start[cell], t1 = cell_start[cell + 1];
-        const bool inner = abs(da) <= 1 && abs(db) <= 1 && abs(dc) <= 1;
-        const uint32_t cntc = t1 - t0 - ((da == 0 && db == 0 && dc == 0) ? 1u : 0u);
+        continue;
+      }
+      for (int dc = -2; dc <= 2; dc++) {
+        int cc2;
+        double sc;
+        if (!axis_cell(2, dc, &cc2, &sc)) continue;
+        const uint32_t t0 = cell_start[row + cc2], t1 = cell_start[row + cc2 + 1];
+        const bool center = da == 0 && db == 0 && dc == 0;
+        const uint32_t cntc = t1 - t0 - (center ? 1u : 0u);
         n125 += cntc;
-        if (!inner) continue;
+        if (!inner_row || abs(dc) > 1) continue;
         n27 += cntc;
-        if (da == 0 && db == 0 && dc == 0) n1 += cntc;
+        if (center) n1 += cntc;
         for (uint32_t t = t0; t < t1; t++) {
           if (t == slot) continue;
           const double4 p = spos[t];
-          const double o[3] = {p.x + sh[0] - q[0], p.y + sh[1] - q[1], p.z + sh[2] - q[2]};
+          const double o[3] = {p.x + sa - q[0], p.y + sb - q[1], p.z + sc - q[2]};
           double d2 = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
           if (!(d2 < best[7])) continue;
           // insertion into the sorted 8 (unrolled, registers only)
@@ -143,6 +190,8 @@ __global__ void __launch_bounds__(256)
           }
         }
       }
+    }
+  }
   double blocks = 1.;
   for (int c = 0; c < nd; c++) blocks *= 3.;
   const double h = (n27 > 0) ? pow(blocks * cellvol / (double)n27, 1.0 / nd)
