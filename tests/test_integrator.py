@@ -180,3 +180,26 @@ def test_learned_mode_keeps_the_same_cells():
     fb = b.compute_face_integrals_array(symmetric=True)
     for p, q in zip(fa, fb):
         assert np.array_equal(p, q)
+
+
+def test_large_cells_keep_the_same_integrals():
+    """Clusters with empty space between them: the rim cells exceed the
+    wavefront kernel's 64 vertices and are built one per wavefront by the
+    large-cell kernel, which also keeps their half-spaces and vertices; the
+    hook path still equals the built-in integrals bit for bit."""
+    rng = np.random.default_rng(78)
+    n = 12000
+    c = rng.random((8, 3))
+    x = np.mod(c[rng.integers(0, 8, n)] + 0.03 * rng.normal(size=(n, 3)), 1.0)
+    x[x >= 1.0] = 0.0
+    v = Voronoi.build(x, ZERO, ONE, 3, True)
+    assert v.stats["big_cells"] > 0
+    it = VoronoiIntegrator.build(x, None, ZERO, ONE, 3, True)
+    c = it.compute_cell_integrals_array(VolumeCentroidIntegral)
+    assert np.array_equal(c[:, 0], v.cell_volume)
+    assert np.array_equal(c[:, 1:], v.cell_centroid)
+    left, right, hs, shift, plane, vals = it.compute_face_integrals_array(
+        AreaCentroidIntegral, symmetric=True)
+    assert np.array_equal(left, v.face_left)
+    assert np.array_equal(right, v.face_right)
+    assert np.array_equal(vals[:, 0], v.face_area)
