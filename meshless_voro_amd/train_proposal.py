@@ -111,8 +111,8 @@ def main():
     t_plain, st_plain = time_mode(xc, True, BuildOptions(mode=MODE_PLAIN))
     print(f"calib plain: {t_plain * 1e3:.1f} ms, redone {st_plain['cells_redone']}", flush=True)
     best = None
-    for rscale in (0.0, 1.05, 1.15):
-        for kscale in (0.9, 1.0, 1.1, 1.25):
+    for rscale in (0.0, 1.05, 1.1, 1.15):
+        for kscale in (0.9, 1.0, 1.1, 1.2, 1.3):
             model.kscale = kscale
             model.rscale = rscale
             t, st = time_mode(xc, True, BuildOptions(mode=MODE_LEARNED, model=model))
