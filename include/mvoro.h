@@ -40,8 +40,12 @@ typedef enum mv_status {
   MV_E_BOUNDARY_WALK = 6,     /* src/voronoi/convex_cell.rs:428 */
   MV_E_DEVICE = 7,            /* HIP runtime error or no GPU */
   MV_E_OOM = 8,               /* device or host allocation failed */
-  MV_E_CAPACITY = 9           /* a cell exceeded the largest per-cell
+  MV_E_CAPACITY = 9,          /* a cell exceeded the largest per-cell
                                  capacity (planes / vertices / neighbours) */
+  MV_E_FACE_WALK = 10,        /* no next vertex around a face,
+                                 src/voronoi/convex_cell.rs:538-541 */
+  MV_E_NOT_3D = 11            /* with_faces outside 3D,
+                                 src/voronoi/convex_cell.rs:462-466 */
 } mv_status;
 
 /* Construction modes. */
@@ -270,8 +274,64 @@ typedef struct mv_cell_geometry {
   int32_t dim;
   uint64_t idx_base;  /* subtracted from the generator index of reference
                          words (non-zero for the problems of a batch) */
+  /* after mv_with_faces (NULL before and after mv_discard_faces): */
+  const uint64_t *fbase;      /* per generator: its first face */
+  const uint32_t *nf;         /* per generator: face count */
+  const int32_t *face_plane;  /* per face: half-space index in the cell */
+  const uint32_t *face_nvert; /* per face: vertex count */
+  const uint64_t *face_voff;  /* per face: offset of its vertices in face_vtx */
+  const uint32_t *face_vtx;   /* vertex indices within the cell, CCW */
 } mv_cell_geometry;
 mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g);
+
+/* The kept cells' ConvexCell fields, copied to host: clipping_planes
+ * (HalfSpace, src/voronoi/half_space.rs:7-19: plane n and p, right_idx
+ * (-1 = None), shift) and vertices (Vertex, src/voronoi/convex_cell.rs:14-26:
+ * loc and dual, the dual's indices into the cell's half-spaces), for every
+ * active cell in generator order: half-spaces plane_offset[i] ..
+ * plane_offset[i+1], vertices vertex_offset[i] .. vertex_offset[i+1]
+ * (n_active + 1 offsets each).  Call with plane_n == NULL to get the
+ * totals; then pass arrays of that size (3 doubles per n, p, shift, loc;
+ * 3 int32 per dual). */
+mv_status mv_kept_cells(mv_tess *t, size_t *n_planes, size_t *n_vertices,
+                        uint64_t *plane_offset, uint64_t *vertex_offset,
+                        double *plane_n, double *plane_p, int64_t *plane_right,
+                        uint8_t *plane_has_shift, double *plane_shift,
+                        double *vertex_loc, int32_t *vertex_dual);
+
+/* VoronoiIntegrator::with_faces (src/voronoi.rs:625-643) and
+ * ConvexCell::with_faces (src/voronoi/convex_cell.rs:461-505): for every
+ * kept cell, the vertices of each half-space holding any, sorted
+ * counter-clockwise as seen from the generator by the reference's walk
+ * (sort_face_vertices, :508-543).  Afterwards mv_tets, mv_cell_integrals,
+ * mv_face_integrals, mv_voronoi_from_integrator and the templates of
+ * mvoro_integrals.hpp decompose each face as the fan (v0, vk, vk+1) of
+ * DecompositionWithFaces (:89-129).  MV_E_NOT_3D outside 3D. */
+mv_status mv_with_faces(mv_tess *t);
+
+/* ConvexCell::discard_faces (src/voronoi/convex_cell.rs:667-673): back to
+ * the decomposition without faces. */
+mv_status mv_discard_faces(mv_tess *t);
+
+/* The faces of mv_with_faces (ConvexCell::face_count / clipping_plane /
+ * face_vertex_count / face_vertices, src/voronoi/convex_cell.rs:675-722):
+ * per active cell i its faces cell_face_offset[i] .. cell_face_offset[i+1]
+ * (n_active + 1 offsets), in half-space order; per face its half-space
+ * index, vertex count and offset into face_vertices, which holds vertex
+ * indices within the cell.  Call with face_plane == NULL to get *n_faces
+ * and *n_entries.  MV_E_ARG when the handle is not in the with-faces
+ * state. */
+mv_status mv_face_vertices(mv_tess *t, size_t *n_faces, size_t *n_entries,
+                           uint64_t *cell_face_offset, int32_t *face_plane,
+                           uint32_t *face_vertex_count, uint64_t *face_vertex_offset,
+                           uint32_t *face_vertices);
+
+/* From<&VoronoiIntegrator<M>> for Voronoi (src/voronoi.rs:493-512,
+ * build_voronoi_cells :811-827): a new handle (free with mv_free) whose
+ * cells and faces are integrated from the kept cells' current
+ * decomposition (fans after mv_with_faces); connectivity, links, normals
+ * and shifts are those of the build. */
+mv_status mv_voronoi_from_integrator(const mv_tess *t, mv_tess **out);
 
 /* ---------------------------------------------------------------------
  * Learned neighbour-count proposal (no counterpart in the reference; it

@@ -22,7 +22,10 @@
 // tets in the reference's order: vertex order, then the six tets
 // (P_t, P_{t-1 mod 6}, vertex) of each vertex, where P_{2k} projects the
 // generator on the vertex's k-th plane and P_{2k+1} on the line of planes
-// k+1 and k (DecompositionWithoutFaces, convex_cell.rs:131-187).
+// k+1 and k (DecompositionWithoutFaces, convex_cell.rs:131-187); after
+// mv_with_faces, face by face the fan (v0, vk, vk+1), k = 1 .. m-2, of the
+// face's counter-clockwise vertices (DecompositionWithFaces,
+// convex_cell.rs:89-129).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,6 +48,15 @@ struct CellGeom {
   int dim;                 // dimensionality 1, 2, 3
   uint64_t idx_base;       // generator index of the problem's first cell
                            // in the reference words (batched builds)
+  // faces (mv_with_faces; null without): per generator its first face and
+  // face count; per face its half-space, vertex count and the offset of
+  // its vertex indices (within the cell, counter-clockwise) in face_vtx
+  const uint64_t *fbase;
+  const uint32_t *nf;
+  const int32_t *face_plane;
+  const uint32_t *face_nvert;
+  const uint64_t *face_voff;
+  const uint32_t *face_vtx;
 };
 
 constexpr uint64_t KEPT_BOX_REF = ~0ull;
@@ -85,9 +97,22 @@ __device__ __forceinline__ bool kept_plane_valid(const CellGeom &G, uint64_t c, 
 // panics there).
 template <class F>
 __device__ bool for_each_tet(const CellGeom &G, uint64_t c, F &f) {
+  const double4 *V = G.vx + G.vbase[c];
+  if (G.face_plane) {
+    // DecompositionWithFaces::next (convex_cell.rs:104-128)
+    const uint64_t fb = G.fbase[c];
+    const uint32_t nfc = G.nf[c];
+    for (uint32_t k = 0; k < nfc; k++) {
+      const uint32_t *fv = G.face_vtx + G.face_voff[fb + k];
+      const uint32_t m = G.face_nvert[fb + k];
+      const int p = G.face_plane[fb + k];
+      const d3 v0 = xyz(V[fv[0]]);
+      for (uint32_t j = 1; j + 1 < m; j++) f(v0, xyz(V[fv[j]]), xyz(V[fv[j + 1]]), p);
+    }
+    return true;
+  }
   const double4 *pa = G.pl_a + G.pbase[c];
   const double4 *pb = G.pl_b + G.pbase[c];
-  const double4 *V = G.vx + G.vbase[c];
   const uint32_t nv = G.nv[c];
   const d3 q = cell_generator(G, c);
   for (uint32_t v = 0; v < nv; v++) {

@@ -61,7 +61,9 @@ class mv_cell_geometry(C.Structure):
     _fields_ = [("pl_a", C.c_void_p), ("pl_b", C.c_void_p), ("vx", C.c_void_p),
                 ("pbase", C.c_void_p), ("vbase", C.c_void_p), ("np", C.c_void_p),
                 ("nv", C.c_void_p), ("cell_loc", C.c_void_p), ("width", C.c_double * 3),
-                ("dim", C.c_int32), ("idx_base", C.c_uint64)]
+                ("dim", C.c_int32), ("idx_base", C.c_uint64), ("fbase", C.c_void_p),
+                ("nf", C.c_void_p), ("face_plane", C.c_void_p), ("face_nvert", C.c_void_p),
+                ("face_voff", C.c_void_p), ("face_vtx", C.c_void_p)]
 
 
 class mv_model(C.Structure):
@@ -111,7 +113,8 @@ EXPORTS = [
     "mv_cell_face_connections", "mv_get_device_view", "mv_get_stats",
     "mv_cell_n_popped", "mv_status_string", "mv_n_active", "mv_cell_integrals",
     "mv_face_integrals", "mv_tets", "mv_get_cell_geometry", "mv_model_features", "mv_model_train",
-    "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact",
+    "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact", "mv_kept_cells",
+    "mv_with_faces", "mv_discard_faces", "mv_face_vertices", "mv_voronoi_from_integrator",
 ]
 
 
@@ -161,6 +164,16 @@ def lib():
     L.mv_face_integrals.argtypes = [vp, u32, u32, C.POINTER(C.c_size_t), vp, vp, vp, vp, vp, vp]
     L.mv_tets.restype = C.c_int
     L.mv_tets.argtypes = [vp, C.POINTER(C.c_size_t), vp, vp, vp]
+    L.mv_kept_cells.restype = C.c_int
+    L.mv_kept_cells.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)] + [vp] * 9
+    L.mv_with_faces.restype = C.c_int
+    L.mv_with_faces.argtypes = [vp]
+    L.mv_discard_faces.restype = C.c_int
+    L.mv_discard_faces.argtypes = [vp]
+    L.mv_face_vertices.restype = C.c_int
+    L.mv_face_vertices.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)] + [vp] * 5
+    L.mv_voronoi_from_integrator.restype = C.c_int
+    L.mv_voronoi_from_integrator.argtypes = [vp, C.POINTER(vp)]
     L.mv_get_cell_geometry.restype = C.c_int
     L.mv_get_cell_geometry.argtypes = [vp, C.POINTER(mv_cell_geometry)]
     L.mv_model_features.restype = C.c_int

@@ -22,6 +22,7 @@
 #include "mv_clip.hpp"
 #include "mv_clipw.hpp"
 #include "mv_common.hpp"
+#include "mv_faces.hpp"
 #include "mv_finalize.hpp"
 #include "mv_grid.hpp"
 #include "mv_learn.hpp"
@@ -252,6 +253,15 @@ struct mv_tess {
   uint64_t idx_base = 0;           // first generator of this problem in the store
   uint8_t *d_mask = nullptr;       // copy of the build's mask (symmetric faces)
   uint64_t *d_active = nullptr;    // active cells in generator order
+  // face lists of the kept cells (mv_with_faces); wf_on selects the fan
+  // decomposition, wf_ready says the lists exist
+  bool wf_on = false, wf_ready = false;
+  uint64_t wf_nfaces = 0, wf_nentries = 0;
+  uint64_t *d_wf_fbase = nullptr, *d_wf_voff = nullptr, *d_wf_cell_foff = nullptr;
+  uint32_t *d_wf_nf = nullptr, *d_wf_nvert = nullptr, *d_wf_vtx = nullptr;
+  int32_t *d_wf_plane = nullptr;
+  // a Voronoi made from an integrator reads arrays of the integrator's build
+  std::shared_ptr<DevStore> parent;
   // host copies
   std::mutex mu;
   bool have_host = false;
@@ -1565,6 +1575,13 @@ CellGeom geom_of(const mv_tess *t) {
   for (int c = 0; c < 3; c++) g.width[c] = t->width[c];
   g.dim = (int)t->dim;
   g.idx_base = t->idx_base;
+  const bool wf = t->wf_on && t->wf_ready;
+  g.fbase = wf ? t->d_wf_fbase : nullptr;
+  g.nf = wf ? t->d_wf_nf : nullptr;
+  g.face_plane = wf ? t->d_wf_plane : nullptr;
+  g.face_nvert = wf ? t->d_wf_nvert : nullptr;
+  g.face_voff = wf ? t->d_wf_voff : nullptr;
+  g.face_vtx = wf ? t->d_wf_vtx : nullptr;
   return g;
 }
 
@@ -1579,10 +1596,15 @@ __global__ void k_compact_index(const uint32_t *__restrict__ flag,
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && flag[i]) out[pos[i]] = i;
 }
+// tets per active cell: six per vertex, or with faces m - 2 per face of m
+// vertices, i.e. 3 nv - 2 nf
 __global__ void k_gather_nv(const uint64_t *__restrict__ cells, uint64_t n,
-                            const uint32_t *__restrict__ nv, uint64_t *__restrict__ out) {
+                            const uint32_t *__restrict__ nv, const uint32_t *__restrict__ nf,
+                            uint64_t *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = 6ull * nv[cells[i]];
+  if (i >= n) return;
+  const uint64_t c = cells[i];
+  out[i] = nf ? 3ull * nv[c] - 2ull * nf[c] : 6ull * nv[c];
 }
 struct TetOut {
   double *v;
@@ -1700,6 +1722,146 @@ void run_face_integrals(mv_tess *t, int symmetric, size_t *count, uint64_t *left
     if (plane) plane[f] = h[f].plane;
     if (values) memcpy(values + (size_t)f * width, &h[f].integral, width * sizeof(double));
   }
+}
+
+// ConvexCell::with_faces for every kept cell (mv_faces.hpp); the lists are
+// built once per handle and kept until mv_free.
+void run_with_faces(mv_tess *t) {
+  if (t->dim != 3) throw BuildError{MV_E_NOT_3D};
+  if (t->wf_ready) {
+    t->wf_on = true;
+    return;
+  }
+  hipStream_t s = 0;
+  Temps TT;
+  ensure_active(t);
+  const uint64_t na = t->n_active;
+  const size_t n = t->n ? t->n : 1;
+  t->d_wf_fbase = t->own<uint64_t>(n);
+  t->d_wf_nf = t->own<uint32_t>(n);
+  t->d_wf_cell_foff = t->own<uint64_t>(na + 1);
+  HIPCHK(hipMemsetAsync(t->d_wf_fbase, 0, n * sizeof(uint64_t), s));
+  HIPCHK(hipMemsetAsync(t->d_wf_nf, 0, n * sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(t->d_wf_cell_foff, 0, (na + 1) * sizeof(uint64_t), s));
+  uint64_t NF = 0, NE = 0;
+  if (na) {
+    CellGeom G = geom_of(t);
+    uint32_t *pcnt = need(TT.get<uint32_t>(t->used_pl ? t->used_pl : 1));
+    uint64_t *nfc = need(TT.get<uint64_t>(na));
+    uint64_t *nec = need(TT.get<uint64_t>(na));
+    uint64_t *eoff = need(TT.get<uint64_t>(na));
+    hipLaunchKernelGGL(k_wf_count, dim3(nblk(na)), dim3(256), 0, s, G,
+                       (const uint64_t *)t->d_active, na, pcnt, nfc, nec);
+    HIPCHK(hipGetLastError());
+    NF = scan_excl<uint64_t, uint64_t>(nfc, t->d_wf_cell_foff, na, s);
+    NE = scan_excl<uint64_t, uint64_t>(nec, eoff, na, s);
+    HIPCHK(hipMemcpyAsync(t->d_wf_cell_foff + na, &NF, sizeof NF, hipMemcpyHostToDevice, s));
+    t->d_wf_plane = t->own<int32_t>(NF ? NF : 1);
+    t->d_wf_nvert = t->own<uint32_t>(NF ? NF : 1);
+    t->d_wf_voff = t->own<uint64_t>(NF ? NF : 1);
+    t->d_wf_vtx = t->own<uint32_t>(NE ? NE : 1);
+    unsigned int *err = need(TT.get<unsigned int>(1));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(k_wf_fill, dim3(nblk(na)), dim3(256), 0, s, G,
+                       (const uint64_t *)t->d_active, na, (const uint64_t *)t->d_wf_cell_foff,
+                       (const uint64_t *)eoff, pcnt, t->d_wf_fbase, t->d_wf_nf, t->d_wf_plane,
+                       t->d_wf_nvert, t->d_wf_voff, t->d_wf_vtx, err);
+    HIPCHK(hipGetLastError());
+    unsigned int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) throw BuildError{MV_E_FACE_WALK};
+  } else {
+    t->d_wf_plane = t->own<int32_t>(1);
+    t->d_wf_nvert = t->own<uint32_t>(1);
+    t->d_wf_voff = t->own<uint64_t>(1);
+    t->d_wf_vtx = t->own<uint32_t>(1);
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  t->wf_nfaces = NF;
+  t->wf_nentries = NE;
+  t->wf_ready = true;
+  t->wf_on = true;
+}
+
+// From<&VoronoiIntegrator<M>> for Voronoi: the build's connectivity with
+// cell and face integrals of the kept cells' current decomposition.
+mv_tess *voronoi_from_kept(mv_tess *t) {
+  hipStream_t s = 0;
+  Temps TT;
+  ensure_active(t);
+  std::unique_ptr<mv_tess> U(new mv_tess);
+  U->parent = t->store;
+  U->n = t->n;
+  U->nf = t->nf;
+  U->nl = t->nl;
+  for (int c = 0; c < 3; c++) {
+    U->anchor[c] = t->anchor[c];
+    U->width[c] = t->width[c];
+  }
+  U->dim = t->dim;
+  U->periodic = t->periodic;
+  U->stats = t->stats;
+  U->d_cell_loc = t->d_cell_loc;
+  U->d_cell_sr = t->d_cell_sr;
+  U->d_cell_foff = t->d_cell_foff;
+  U->d_cell_fcnt = t->d_cell_fcnt;
+  U->d_cell_idx = t->d_cell_idx;
+  U->d_npop = t->d_npop;
+  U->d_face_left = t->d_face_left;
+  U->d_face_right = t->d_face_right;
+  U->d_face_norm = t->d_face_norm;
+  U->d_face_shift = t->d_face_shift;
+  U->d_face_has_shift = t->d_face_has_shift;
+  U->d_links = t->d_links;
+  const size_t n = t->n ? t->n : 1, F = t->nf ? t->nf : 1;
+  U->d_cell_vol = U->own<double>(n);
+  U->d_cell_cen = U->own<double>(3 * n);
+  U->d_face_area = U->own<double>(F);
+  U->d_face_cen = U->own<double>(3 * F);
+  // VoronoiCell::default for the generators without a cell
+  HIPCHK(hipMemsetAsync(U->d_cell_vol, 0, n * sizeof(double), s));
+  HIPCHK(hipMemsetAsync(U->d_cell_cen, 0, 3 * n * sizeof(double), s));
+  const uint64_t na = t->n_active;
+  if (na) {
+    const CellGeom G = geom_of(t);
+    unsigned int *err = need(TT.get<unsigned int>(1));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
+    VolumeCentroidIntegral *ci = need(TT.get<VolumeCentroidIntegral>(na));
+    hipLaunchKernelGGL((cell_integrals<VolumeCentroidIntegral>), dim3(nblk(na)), dim3(256), 0,
+                       s, G, (const uint64_t *)t->d_active, na, (const void *)nullptr, ci, err);
+    hipLaunchKernelGGL(k_scatter_cell_integrals, dim3(nblk(na)), dim3(256), 0, s,
+                       (const uint64_t *)t->d_active, na, (const VolumeCentroidIntegral *)ci,
+                       U->d_cell_vol, U->d_cell_cen);
+    // the faces VoronoiCell::from_convex_cell creates (voronoi_cell.rs:53-73)
+    // are those of compute_face_integrals_sym, in the same order
+    int32_t *face_local = need(TT.get<int32_t>(t->used_pl ? t->used_pl : 1));
+    uint64_t *cnt = need(TT.get<uint64_t>(na));
+    uint64_t *off = need(TT.get<uint64_t>(na));
+    hipLaunchKernelGGL(face_counts, dim3(nblk(na)), dim3(256), 0, s, G,
+                       (const uint64_t *)t->d_active, na, 1, (const uint8_t *)t->d_mask,
+                       face_local, cnt);
+    HIPCHK(hipGetLastError());
+    const uint64_t NF = scan_excl<uint64_t, uint64_t>(cnt, off, na, s);
+    if (NF != t->nf) throw BuildError{MV_E_DEVICE};
+    if (NF) {
+      FaceIntegrator<AreaCentroidIntegral> *fi =
+          need(TT.get<FaceIntegrator<AreaCentroidIntegral>>(NF));
+      hipLaunchKernelGGL((face_integrals<AreaCentroidIntegral>), dim3(nblk(na)), dim3(256), 0,
+                         s, G, (const uint64_t *)t->d_active, na, (const int32_t *)face_local,
+                         (const uint64_t *)off, (const void *)nullptr, fi, err);
+      hipLaunchKernelGGL(k_scatter_face_integrals, dim3(nblk(NF)), dim3(256), 0, s, NF,
+                         (const FaceIntegrator<AreaCentroidIntegral> *)fi, U->d_face_area,
+                         U->d_face_cen);
+    }
+    HIPCHK(hipGetLastError());
+    unsigned int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) throw BuildError{MV_E_PLANE_DET_ZERO};
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return U.release();
 }
 
 }  // namespace
@@ -1914,6 +2076,8 @@ const char *mv_status_string(mv_status s) {
     case MV_E_DEVICE: return "HIP device error or no GPU";
     case MV_E_OOM: return "out of memory";
     case MV_E_CAPACITY: return "per-cell capacity exceeded";
+    case MV_E_FACE_WALK: return "no next vertex connected to the current one around a face";
+    case MV_E_NOT_3D: return "faces can only be extracted in 3D";
   }
   return "unknown status";
 }
@@ -2193,8 +2357,10 @@ mv_status mv_tets(mv_tess *t, size_t *count, double *v012, int32_t *plane,
     }
     uint64_t *cnt = need(TT.get<uint64_t>(na));
     uint64_t *off = need(TT.get<uint64_t>(na));
+    const CellGeom G = geom_of(t);
     hipLaunchKernelGGL(k_gather_nv, dim3(nblk(na)), dim3(256), 0, s,
-                       (const uint64_t *)t->d_active, na, (const uint32_t *)t->cs.nv, cnt);
+                       (const uint64_t *)t->d_active, na, (const uint32_t *)t->cs.nv, G.nf,
+                       cnt);
     const uint64_t K = scan_excl<uint64_t, uint64_t>(cnt, off, na, s);
     if (!v012) {
       *count = K;
@@ -2206,7 +2372,7 @@ mv_status mv_tets(mv_tess *t, size_t *count, double *v012, int32_t *plane,
     int32_t *dp = need(TT.get<int32_t>(K));
     unsigned int *err = need(TT.get<unsigned int>(1));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(unsigned int), s));
-    hipLaunchKernelGGL(k_export_tets, dim3(nblk(na)), dim3(256), 0, s, geom_of(t),
+    hipLaunchKernelGGL(k_export_tets, dim3(nblk(na)), dim3(256), 0, s, G,
                        (const uint64_t *)t->d_active, na, (const uint64_t *)off, dv, dp, err);
     HIPCHK(hipGetLastError());
     unsigned int herr = 0;
@@ -2218,6 +2384,124 @@ mv_status mv_tets(mv_tess *t, size_t *count, double *v012, int32_t *plane,
     cell_offset[na] = K;
     if (herr) throw BuildError{MV_E_PLANE_DET_ZERO};
   });
+}
+
+mv_status mv_kept_cells(mv_tess *t, size_t *n_planes, size_t *n_vertices,
+                        uint64_t *plane_offset, uint64_t *vertex_offset, double *plane_n,
+                        double *plane_p, int64_t *plane_right, uint8_t *plane_has_shift,
+                        double *plane_shift, double *vertex_loc, int32_t *vertex_dual) {
+  if (!t || !t->keep || !n_planes || !n_vertices) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&] {
+    hipStream_t s = 0;
+    Temps TT;
+    ensure_active(t);
+    const uint64_t na = t->n_active;
+    if (!na) {
+      *n_planes = *n_vertices = 0;
+      if (plane_n && plane_offset && vertex_offset) plane_offset[0] = vertex_offset[0] = 0;
+      return;
+    }
+    const CellGeom G = geom_of(t);
+    uint64_t *npc = need(TT.get<uint64_t>(na));
+    uint64_t *nvc = need(TT.get<uint64_t>(na));
+    uint64_t *poff = need(TT.get<uint64_t>(na));
+    uint64_t *voff = need(TT.get<uint64_t>(na));
+    hipLaunchKernelGGL(k_kept_counts, dim3(nblk(na)), dim3(256), 0, s, G,
+                       (const uint64_t *)t->d_active, na, npc, nvc);
+    HIPCHK(hipGetLastError());
+    const uint64_t P = scan_excl<uint64_t, uint64_t>(npc, poff, na, s);
+    const uint64_t V = scan_excl<uint64_t, uint64_t>(nvc, voff, na, s);
+    if (!plane_n) {
+      *n_planes = P;
+      *n_vertices = V;
+      return;
+    }
+    if (*n_planes < P || *n_vertices < V || !plane_p || !plane_right || !plane_has_shift ||
+        !plane_shift || !vertex_loc || !vertex_dual || !plane_offset || !vertex_offset)
+      throw BuildError{MV_E_ARG};
+    *n_planes = P;
+    *n_vertices = V;
+    double *dpn = need(TT.get<double>(3 * P)), *dpp = need(TT.get<double>(3 * P));
+    double *dsh = need(TT.get<double>(3 * P)), *dvl = need(TT.get<double>(3 * V));
+    int64_t *drt = need(TT.get<int64_t>(P));
+    uint8_t *dhs = need(TT.get<uint8_t>(P));
+    int32_t *dvd = need(TT.get<int32_t>(3 * V));
+    hipLaunchKernelGGL(k_kept_export, dim3(nblk(na)), dim3(256), 0, s, G,
+                       (const uint64_t *)t->d_active, na, (const uint64_t *)poff,
+                       (const uint64_t *)voff, dpn, dpp, drt, dhs, dsh, dvl, dvd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(plane_n, dpn, 3 * P * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane_p, dpp, 3 * P * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane_shift, dsh, 3 * P * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane_right, drt, P * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane_has_shift, dhs, P, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vertex_loc, dvl, 3 * V * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vertex_dual, dvd, 3 * V * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(plane_offset, poff, na * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vertex_offset, voff, na * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    plane_offset[na] = P;
+    vertex_offset[na] = V;
+  });
+}
+
+mv_status mv_with_faces(mv_tess *t) {
+  if (!t || !t->keep) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&] { run_with_faces(t); });
+}
+
+mv_status mv_discard_faces(mv_tess *t) {
+  if (!t || !t->keep) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  t->wf_on = false;
+  return MV_OK;
+}
+
+mv_status mv_face_vertices(mv_tess *t, size_t *n_faces, size_t *n_entries,
+                           uint64_t *cell_face_offset, int32_t *face_plane,
+                           uint32_t *face_vertex_count, uint64_t *face_vertex_offset,
+                           uint32_t *face_vertices) {
+  if (!t || !t->keep || !n_faces || !n_entries) return MV_E_ARG;
+  std::lock_guard<std::mutex> g(t->mu);
+  if (!t->wf_on || !t->wf_ready) return MV_E_ARG;
+  const uint64_t NF = t->wf_nfaces, NE = t->wf_nentries, na = t->n_active;
+  if (!face_plane) {
+    *n_faces = NF;
+    *n_entries = NE;
+    return MV_OK;
+  }
+  if (*n_faces < NF || *n_entries < NE || !cell_face_offset || !face_vertex_count ||
+      !face_vertex_offset || !face_vertices)
+    return MV_E_ARG;
+  *n_faces = NF;
+  *n_entries = NE;
+  return guarded([&] {
+    hipStream_t s = 0;
+    HIPCHK(hipMemcpyAsync(cell_face_offset, t->d_wf_cell_foff, (na + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    if (NF) {
+      HIPCHK(hipMemcpyAsync(face_plane, t->d_wf_plane, NF * sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(face_vertex_count, t->d_wf_nvert, NF * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(face_vertex_offset, t->d_wf_voff, NF * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    }
+    if (NE)
+      HIPCHK(hipMemcpyAsync(face_vertices, t->d_wf_vtx, NE * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+mv_status mv_voronoi_from_integrator(const mv_tess *tc, mv_tess **out) {
+  if (!tc || !tc->keep || !out) return MV_E_ARG;
+  *out = nullptr;
+  mv_tess *t = const_cast<mv_tess *>(tc);
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&] { *out = voronoi_from_kept(t); });
 }
 
 mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g) {

@@ -222,11 +222,14 @@ class VoronoiIntegrator:
         self._mask = m
         self._loc = xyz.copy()
         d = int(dimensionality)
+        self._dim = d
         if d < 3:
             self._loc[:, 2] = 0.0
         if d < 2:
             self._loc[:, 1] = 0.0
         self._active = None
+        self._faces_on = False
+        self._cache = {}
         return self
 
     def __del__(self):
@@ -250,7 +253,14 @@ class VoronoiIntegrator:
 
     def tets(self):
         """(v012 [K, 3, 3], plane [K], cell_offset [n_active + 1]) -- the
-        oriented tets of ConvexCell::decompose for every active cell."""
+        oriented tets of ConvexCell::decompose for every active cell (the
+        fans of DecompositionWithFaces after with_faces)."""
+        key = ("tets", self._faces_on)
+        if key not in self._cache:
+            self._cache[key] = self._tets()
+        return self._cache[key]
+
+    def _tets(self):
         L = _native.lib()
         cnt = C.c_size_t(0)
         _check(L.mv_tets(self._h, C.byref(cnt), None, None, None))
@@ -358,3 +368,211 @@ class VoronoiIntegrator:
                     acc.collect(v[k, 0], v[k, 1], v[k, 2], gen)
             out.append(wrap(f, acc.finalize()))
         return out
+
+    # -- faces and cells (VoronoiIntegrator<WithFaces>, ConvexCell) ----------
+    def with_faces(self) -> "VoronoiIntegrator":
+        """VoronoiIntegrator::with_faces (src/voronoi.rs:625-643): sorts the
+        vertices of every face of every kept cell counter-clockwise on the
+        GPU (mv_with_faces).  The reference consumes the integrator and
+        returns its WithFaces form; here the integrator is converted in
+        place and returned.  Integrals and tets from then on use the fan
+        decomposition.  Raises VoronoiError(11) outside 3D."""
+        _check(_native.lib().mv_with_faces(self._h))
+        self._faces_on = True
+        return self
+
+    def discard_faces(self) -> "VoronoiIntegrator":
+        """ConvexCell::discard_faces (src/voronoi/convex_cell.rs:667-673)
+        for every cell: back to the decomposition without faces."""
+        _check(_native.lib().mv_discard_faces(self._h))
+        self._faces_on = False
+        return self
+
+    def has_faces(self) -> bool:
+        return self._faces_on
+
+    def _kept(self):
+        """Half-spaces and vertices of every active cell (mv_kept_cells)."""
+        if "kept" not in self._cache:
+            L = _native.lib()
+            P, V = C.c_size_t(0), C.c_size_t(0)
+            na = self.n_active()
+            _check(L.mv_kept_cells(self._h, C.byref(P), C.byref(V), *([None] * 9)))
+            k = {"poff": np.zeros(na + 1, np.uint64), "voff": np.zeros(na + 1, np.uint64),
+                 "n": np.zeros((P.value, 3)), "p": np.zeros((P.value, 3)),
+                 "right": np.zeros(P.value, np.int64), "has_shift": np.zeros(P.value, np.uint8),
+                 "shift": np.zeros((P.value, 3)), "loc": np.zeros((V.value, 3)),
+                 "dual": np.zeros((V.value, 3), np.int32)}
+            if na:
+                _check(L.mv_kept_cells(
+                    self._h, C.byref(P), C.byref(V), k["poff"].ctypes.data,
+                    k["voff"].ctypes.data, k["n"].ctypes.data, k["p"].ctypes.data,
+                    k["right"].ctypes.data, k["has_shift"].ctypes.data,
+                    k["shift"].ctypes.data, k["loc"].ctypes.data, k["dual"].ctypes.data))
+            self._cache["kept"] = k
+        return self._cache["kept"]
+
+    def face_vertex_arrays(self):
+        """(cell_face_offset [n_active + 1], face_plane, face_vertex_count,
+        face_vertex_offset, face_vertices) of mv_face_vertices: the faces of
+        active cell i are cell_face_offset[i] .. cell_face_offset[i+1];
+        face f lists face_vertex_count[f] vertex indices (within its cell)
+        at face_vertices[face_vertex_offset[f]:]."""
+        if not self._faces_on:
+            raise TypeError("face vertices need with_faces()")
+        if "faces" not in self._cache:
+            L = _native.lib()
+            nf, ne = C.c_size_t(0), C.c_size_t(0)
+            _check(L.mv_face_vertices(self._h, C.byref(nf), C.byref(ne), *([None] * 5)))
+            cfo = np.zeros(self.n_active() + 1, np.uint64)
+            fp = np.zeros(max(nf.value, 1), np.int32)
+            fc = np.zeros(max(nf.value, 1), np.uint32)
+            fo = np.zeros(max(nf.value, 1), np.uint64)
+            fv = np.zeros(max(ne.value, 1), np.uint32)
+            _check(L.mv_face_vertices(self._h, C.byref(nf), C.byref(ne), cfo.ctypes.data,
+                                      fp.ctypes.data, fc.ctypes.data, fo.ctypes.data,
+                                      fv.ctypes.data))
+            self._cache["faces"] = (cfo, fp[:nf.value], fc[:nf.value], fo[:nf.value],
+                                    fv[:ne.value])
+        return self._cache["faces"]
+
+    def get_cell_at(self, index: int) -> Optional["ConvexCell"]:
+        """src/voronoi.rs:652-660: the ConvexCell of generator `index`, or
+        None for a generator without one (partial construction)."""
+        act = self.active_cells()
+        k = int(np.searchsorted(act, index))
+        if k >= len(act) or int(act[k]) != index:
+            return None
+        return ConvexCell(self, k)
+
+    def cells_iter(self):
+        """src/voronoi.rs:662-670: the ConvexCells in generator order."""
+        for k in range(len(self.active_cells())):
+            yield ConvexCell(self, k)
+
+
+class Plane:
+    """geometry.rs:21-27"""
+
+    def __init__(self, n, p):
+        self.n, self.p = n, p
+
+
+class HalfSpace:
+    """half_space.rs:7-19 (public fields: plane, right_idx, shift)"""
+
+    def __init__(self, n, p, right, shift):
+        self.plane = Plane(n, p)
+        self.right_idx = right
+        self.shift = shift
+
+    def normal(self):
+        return self.plane.n
+
+
+class Vertex:
+    """convex_cell.rs:14-26 (public fields: loc, dual)"""
+
+    def __init__(self, loc, dual):
+        self.loc, self.dual = loc, dual
+
+
+class ConvexCell:
+    """ConvexCell (src/voronoi/convex_cell.rs:247-268) of a kept cell: a view
+    of the integrator's k-th active cell, with the face accessors of
+    ConvexCell<WithFaces> (:675-722) when the integrator has faces."""
+
+    def __init__(self, it: VoronoiIntegrator, k: int):
+        self._it, self._k = it, k
+        self.idx = int(it.active_cells()[k])
+        self.loc = it._loc[self.idx].copy()
+        kp = it._kept()
+        p0, p1 = int(kp["poff"][k]), int(kp["poff"][k + 1])
+        v0, v1 = int(kp["voff"][k]), int(kp["voff"][k + 1])
+        self.clipping_planes = [
+            HalfSpace(kp["n"][j].copy(), kp["p"][j].copy(),
+                      int(kp["right"][j]) if kp["right"][j] >= 0 else None,
+                      kp["shift"][j].copy() if kp["has_shift"][j] else None)
+            for j in range(p0, p1)]
+        self.vertices = [Vertex(kp["loc"][j].copy(), [int(d) for d in kp["dual"][j]])
+                         for j in range(v0, v1)]
+        self._faces = None
+        if it.has_faces():
+            cfo, fp, fc, fo, fv = it.face_vertex_arrays()
+            self._faces = [(int(fp[f]), fv[int(fo[f]):int(fo[f]) + int(fc[f])])
+                           for f in range(int(cfo[k]), int(cfo[k + 1]))]
+
+    # ConvexCell<WithFaces> (convex_cell.rs:675-722)
+    def _face(self, face_idx):
+        if self._faces is None:
+            raise TypeError("face accessors need VoronoiIntegrator.with_faces()")
+        return self._faces[face_idx]
+
+    def face_count(self) -> int:
+        if self._faces is None:
+            raise TypeError("face accessors need VoronoiIntegrator.with_faces()")
+        return len(self._faces)
+
+    def clipping_plane(self, face_idx: int) -> Plane:
+        return self.clipping_planes[self._face(face_idx)[0]].plane
+
+    def neighbour(self, face_idx: int) -> Optional[int]:
+        return self.clipping_planes[self._face(face_idx)[0]].right_idx
+
+    def shift(self, face_idx: int):
+        return self.clipping_planes[self._face(face_idx)[0]].shift
+
+    def face_vertex_count(self, face_idx: int) -> int:
+        return len(self._face(face_idx)[1])
+
+    def face_vertices(self, face_idx: int) -> np.ndarray:
+        return self._face(face_idx)[1]
+
+    # integrals of this cell (convex_cell.rs:583-664), folded on the host
+    # over the tets the GPU decomposed
+    def _tets(self):
+        v, p, off = self._it.tets()
+        a, b = int(off[self._k]), int(off[self._k + 1])
+        return v[a:b], p[a:b]
+
+    def compute_cell_integral(self, integral, extra_data=None):
+        v, _ = self._tets()
+        acc = (integral.init(self.idx) if extra_data is None
+               else integral.init_with_data(self.idx, extra_data))
+        for t in range(len(v)):
+            acc.collect(v[t, 0], v[t, 1], v[t, 2], self.loc)
+        return acc.finalize()
+
+    def _face_integrals(self, integral, extra_data, mask):
+        v, p = self._tets()
+        dim = self._it._dim
+        acc = {}
+        for t in range(len(v)):
+            pl = int(p[t])
+            hs = self.clipping_planes[pl]
+            if not Dimensionality(dim).vector_is_valid(hs.normal()):
+                continue
+            if pl not in acc:
+                r = hs.right_idx
+                if (mask is not None and hs.shift is None and r is not None and r < self.idx
+                        and mask[r]):
+                    continue
+                acc[pl] = (integral.init(self.idx, pl) if extra_data is None
+                           else integral.init_with_data(self.idx, pl, extra_data))
+            acc[pl].collect(v[t, 0], v[t, 1], v[t, 2], self.loc)
+        out = []
+        for pl in sorted(acc):
+            hs = self.clipping_planes[pl]
+            out.append(FaceIntegrator(self.idx, hs.right_idx, hs.shift, acc[pl].finalize()))
+        return out
+
+    def compute_face_integrals(self, integral, extra_data=None) -> List[FaceIntegrator]:
+        """convex_cell.rs:598-620"""
+        return self._face_integrals(integral, extra_data, None)
+
+    def compute_face_integrals_sym(self, integral, extra_data=None,
+                                   mask=None) -> List[FaceIntegrator]:
+        """convex_cell.rs:622-664 (mask: cell_is_active of the build)"""
+        if mask is None:
+            mask = np.ones(self._it._n, bool)
+        return self._face_integrals(integral, extra_data, mask)

@@ -181,6 +181,18 @@ class Voronoi:
         return cls._build(generators, mask, anchor, width, dimensionality, periodic, options)
 
     @classmethod
+    def from_integrator(cls, integrator) -> "Voronoi":
+        """From<&VoronoiIntegrator<M>> for Voronoi (src/voronoi.rs:493-512):
+        cells and faces integrated from the integrator's kept cells in their
+        current decomposition (fans after with_faces), with the build's
+        connectivity (mv_voronoi_from_integrator)."""
+        h = C.c_void_p()
+        st = _native.lib().mv_voronoi_from_integrator(integrator._h, C.byref(h))
+        if st != 0:
+            raise VoronoiError(st)
+        return cls._from_handle(h)
+
+    @classmethod
     def build_batch(cls, problems, dimensionality=Dimensionality.ThreeD, masks=None,
                     options: Optional[BuildOptions] = None) -> "list[Voronoi]":
         """Many independent builds in one pass (mv_build_batch): `problems` is

@@ -1498,6 +1498,150 @@ void orc_free_tets(orc_tets *T) {
 }
 
 /* ------------------------------------------------------------------ */
+/* with_faces: face vertex lists and the fan decomposition             */
+/* ------------------------------------------------------------------ */
+
+/* Vertex::plane_idx (convex_cell.rs:50-59): position of half-space p in a
+ * vertex's dual, -1 if absent */
+static int dual_pos(const int32_t *d, int32_t p) {
+  for (int k = 0; k < 3; k++)
+    if (d[k] == p) return k;
+  return -1;
+}
+
+/* sort_face_vertices (convex_cell.rs:508-543): vi holds the m vertices of
+ * half-space `plane` in ascending index; starting from vi[0], repeatedly
+ * move the first later vertex that contains the current edge's other plane
+ * to the next slot (swap), the next edge plane being the one after `plane`
+ * in that vertex's dual. */
+static int sort_face_vertices(const int32_t *dual, uint32_t *vi, uint32_t m,
+                              int32_t plane) {
+  if (m == 0) return ORC_OK;
+  const int32_t *d = dual + 3 * (size_t)vi[0];
+  int pos = dual_pos(d, plane);
+  if (pos < 0) return ORC_E_FACE_WALK;
+  int32_t next_plane = d[(pos + 1) % 3];
+  uint32_t cur = 1;
+  while (cur + 1 < m) {
+    uint32_t t = cur;
+    for (; t < m; t++) {
+      d = dual + 3 * (size_t)vi[t];
+      pos = dual_pos(d, plane);
+      if (pos < 0) return ORC_E_FACE_WALK;
+      if (d[0] == next_plane || d[1] == next_plane || d[2] == next_plane) {
+        next_plane = d[(pos + 1) % 3];
+        const uint32_t sw = vi[cur];
+        vi[cur] = vi[t];
+        vi[t] = sw;
+        cur++;
+        break;
+      }
+    }
+    if (t >= m) return ORC_E_FACE_WALK; /* :538-541 */
+  }
+  return ORC_OK;
+}
+
+/* ConvexCell::with_faces (convex_cell.rs:461-505) per cell, then
+ * DecompositionWithFaces (:89-129): for each face in order, the fan
+ * (v[0], v[k], v[k+1]), k = 1 .. count-2, tagged with the face's
+ * half-space. */
+int orc_with_faces(const orc_tets *T, int dim, orc_faces **out) {
+  orc_faces *F = (orc_faces *)calloc(1, sizeof(orc_faces));
+  *out = F;
+  if (!F) return ORC_E_OOM;
+  if (dim != 3) return F->status = ORC_E_NOT_3D; /* convex_cell.rs:462-466 */
+  const size_t n = T->n_cells;
+  F->n_cells = n;
+  const uint64_t np_all = T->plane_offset[n], nv_all = T->vert_offset[n];
+  F->face_offset = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+  F->conn_offset = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+  F->tet_offset = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+  F->face_plane = (int32_t *)malloc((np_all + 1) * sizeof(int32_t));
+  F->face_count = (uint32_t *)malloc((np_all + 1) * sizeof(uint32_t));
+  F->face_vertex_offset = (uint64_t *)malloc((np_all + 1) * sizeof(uint64_t));
+  F->conn = (uint32_t *)malloc((3 * nv_all + 1) * sizeof(uint32_t));
+  /* a face of m vertices gives m - 2 tets: at most 3 per vertex */
+  F->tet_v = (double *)malloc((9 * 3 * nv_all + 1) * sizeof(double));
+  F->tet_plane = (int32_t *)malloc((3 * nv_all + 1) * sizeof(int32_t));
+  if (!F->face_offset || !F->conn_offset || !F->tet_offset || !F->face_plane ||
+      !F->face_count || !F->face_vertex_offset || !F->conn || !F->tet_v ||
+      !F->tet_plane)
+    return F->status = ORC_E_OOM;
+  uint32_t *cnt = NULL, *cursor = NULL;
+  size_t cap = 0;
+  size_t nf = 0, ne = 0, nt = 0;
+  int st = ORC_OK;
+  for (size_t i = 0; i < n && !st; i++) {
+    const size_t np = T->plane_offset[i + 1] - T->plane_offset[i];
+    const size_t nv = T->vert_offset[i + 1] - T->vert_offset[i];
+    const int32_t *dual = T->vert_dual + 3 * T->vert_offset[i];
+    const double *loc = T->vert_loc + 3 * T->vert_offset[i];
+    if (np > cap) {
+      cap = np;
+      cnt = (uint32_t *)realloc(cnt, cap * sizeof(uint32_t));
+      cursor = (uint32_t *)realloc(cursor, cap * sizeof(uint32_t));
+    }
+    /* face_vertex_connections[dual[k]].push(idx), vertices in order */
+    for (size_t p = 0; p < np; p++) cnt[p] = 0;
+    for (size_t v = 0; v < nv; v++)
+      for (int k = 0; k < 3; k++) cnt[dual[3 * v + k]]++;
+    uint32_t run = 0;
+    for (size_t p = 0; p < np; p++) {
+      cursor[p] = run;
+      run += cnt[p];
+    }
+    uint32_t *conn = F->conn + ne;
+    for (size_t v = 0; v < nv; v++)
+      for (int k = 0; k < 3; k++) conn[cursor[dual[3 * v + k]]++] = (uint32_t)v;
+    /* faces in half-space order; the offset is relative to the cell */
+    run = 0;
+    for (size_t p = 0; p < np && !st; p++) {
+      if (!cnt[p]) continue;
+      st = sort_face_vertices(dual, conn + run, cnt[p], (int32_t)p);
+      if (st) break;
+      F->face_plane[nf] = (int32_t)p;
+      F->face_count[nf] = cnt[p];
+      F->face_vertex_offset[nf] = run;
+      nf++;
+      if (cnt[p] < 3) { /* the fan would index past the face (:113-118) */
+        st = ORC_E_FACE_WALK;
+        break;
+      }
+      const uint32_t *fv = conn + run;
+      for (uint32_t k = 1; k + 1 < cnt[p]; k++, nt++) {
+        memcpy(F->tet_v + 9 * nt, loc + 3 * (size_t)fv[0], 3 * sizeof(double));
+        memcpy(F->tet_v + 9 * nt + 3, loc + 3 * (size_t)fv[k], 3 * sizeof(double));
+        memcpy(F->tet_v + 9 * nt + 6, loc + 3 * (size_t)fv[k + 1], 3 * sizeof(double));
+        F->tet_plane[nt] = (int32_t)p;
+      }
+      run += cnt[p];
+    }
+    ne += 3 * nv;
+    F->face_offset[i + 1] = nf;
+    F->conn_offset[i + 1] = ne;
+    F->tet_offset[i + 1] = nt;
+  }
+  free(cnt);
+  free(cursor);
+  return F->status = st;
+}
+
+void orc_free_faces(orc_faces *F) {
+  if (!F) return;
+  free(F->face_offset);
+  free(F->face_plane);
+  free(F->face_count);
+  free(F->face_vertex_offset);
+  free(F->conn_offset);
+  free(F->conn);
+  free(F->tet_offset);
+  free(F->tet_v);
+  free(F->tet_plane);
+  free(F);
+}
+
+/* ------------------------------------------------------------------ */
 /* exported primitives                                                 */
 /* ------------------------------------------------------------------ */
 void orc_iloc(const double anchor[3], const double width[3], int periodic,

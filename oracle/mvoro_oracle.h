@@ -29,7 +29,9 @@ enum {
   ORC_E_PLANE_DET_ZERO = 4,    /* geometry.rs:70 */
   ORC_E_NAN = 5,               /* convex_cell.rs:448 */
   ORC_E_BOUNDARY_WALK = 6,     /* convex_cell.rs:428 */
-  ORC_E_OOM = 8
+  ORC_E_OOM = 8,
+  ORC_E_FACE_WALK = 10,        /* convex_cell.rs:538-541 */
+  ORC_E_NOT_3D = 11            /* convex_cell.rs:462-466 */
 };
 
 typedef struct orc_voronoi {
@@ -98,6 +100,32 @@ int orc_build_tets(const double *xyz, size_t n, const uint8_t *mask,
                    const double anchor[3], const double width[3], int dim,
                    int periodic, int nthreads, orc_tets **out);
 void orc_free_tets(orc_tets *t);
+
+/* ConvexCell::with_faces (convex_cell.rs:461-505, sort_face_vertices
+ * :508-543) for every cell of a decomposition export, and the fan
+ * decomposition that replaces it (DecompositionWithFaces :89-129).
+ * Faces of cell i: face_offset[i] .. face_offset[i+1], in half-space order
+ * (half-spaces holding at least one vertex); face f has face_plane[f],
+ * face_count[f] vertices at conn[conn_offset[i] + face_vertex_offset[f]]
+ * (vertex indices within the cell, the reference's vertex_offset being
+ * relative to the cell).  Fan tets like orc_tets' tet arrays.  dim must be
+ * 3 (ORC_E_NOT_3D otherwise, where the reference panics). */
+typedef struct orc_faces {
+  int status;
+  size_t n_cells;
+  uint64_t *face_offset;        /* N+1 */
+  int32_t *face_plane;
+  uint32_t *face_count;
+  uint64_t *face_vertex_offset; /* within the cell's connection list */
+  uint64_t *conn_offset;        /* N+1 */
+  uint32_t *conn;
+  uint64_t *tet_offset;         /* N+1 */
+  double *tet_v;                /* 9 per tet */
+  int32_t *tet_plane;
+} orc_faces;
+
+int orc_with_faces(const orc_tets *t, int dim, orc_faces **out);
+void orc_free_faces(orc_faces *f);
 
 /* primitives exported for unit tests */
 int orc_in_sphere_exact(const int64_t a[3], const int64_t b[3],

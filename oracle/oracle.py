@@ -78,6 +78,22 @@ class _OrcTets(C.Structure):
     ]
 
 
+class _OrcFaces(C.Structure):
+    _fields_ = [
+        ("status", C.c_int),
+        ("n_cells", C.c_size_t),
+        ("face_offset", C.POINTER(C.c_uint64)),
+        ("face_plane", C.POINTER(C.c_int32)),
+        ("face_count", C.POINTER(C.c_uint32)),
+        ("face_vertex_offset", C.POINTER(C.c_uint64)),
+        ("conn_offset", C.POINTER(C.c_uint64)),
+        ("conn", C.POINTER(C.c_uint32)),
+        ("tet_offset", C.POINTER(C.c_uint64)),
+        ("tet_v", C.POINTER(C.c_double)),
+        ("tet_plane", C.POINTER(C.c_int32)),
+    ]
+
+
 _libs: dict = {}
 
 _D3 = C.c_double * 3
@@ -101,6 +117,10 @@ def lib(flops: bool = False):
             C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p,
             C.c_int, C.c_int, C.c_int, C.POINTER(C.POINTER(_OrcTets))]
         L.orc_free_tets.argtypes = [C.POINTER(_OrcTets)]
+        L.orc_with_faces.restype = C.c_int
+        L.orc_with_faces.argtypes = [C.POINTER(_OrcTets), C.c_int,
+                                     C.POINTER(C.POINTER(_OrcFaces))]
+        L.orc_free_faces.argtypes = [C.POINTER(_OrcFaces)]
         L.orc_in_sphere_exact.restype = C.c_int
         L.orc_in_sphere_exact.argtypes = [C.c_void_p] * 5
         L.orc_iloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
@@ -198,15 +218,16 @@ def build(xyz, anchor, width, dimensionality=3, periodic=False, mask=None,
 
 class OracleTets:
     """Per-cell decomposition (oriented tets in decomposition order) and
-    clipping planes, as the reference's VoronoiIntegrator sees them."""
+    clipping planes, as the reference's VoronoiIntegrator sees them.  With
+    with_faces=True the tets are those of DecompositionWithFaces
+    (convex_cell.rs:89-129) and the face lists of ConvexCell::with_faces
+    (convex_cell.rs:461-543) are attached: face_offset (per cell),
+    face_plane, face_count, face_vertex_offset (relative to the cell's
+    conn_offset), conn (vertex indices within the cell)."""
 
-    def __init__(self, T: _OrcTets):
+    def __init__(self, T: _OrcTets, F=None):
         n = T.n_cells
         self.n_cells = n
-        self.tet_offset = _arr(T.tet_offset, n + 1, np.uint64)
-        nt = int(self.tet_offset[-1])
-        self.tet_v = _arr(T.tet_v, 9 * nt, np.float64).reshape(nt, 3, 3)
-        self.tet_plane = _arr(T.tet_plane, nt, np.int32)
         self.plane_offset = _arr(T.plane_offset, n + 1, np.uint64)
         npl = int(self.plane_offset[-1])
         self.plane_n = _arr(T.plane_n, 3 * npl, np.float64).reshape(npl, 3)
@@ -218,12 +239,28 @@ class OracleTets:
         nv = int(self.vert_offset[-1])
         self.vert_loc = _arr(T.vert_loc, 3 * nv, np.float64).reshape(nv, 3)
         self.vert_dual = _arr(T.vert_dual, 3 * nv, np.int32).reshape(nv, 3)
+        src = T if F is None else F
+        self.tet_offset = _arr(src.tet_offset, n + 1, np.uint64)
+        nt = int(self.tet_offset[-1])
+        self.tet_v = _arr(src.tet_v, 9 * nt, np.float64).reshape(nt, 3, 3)
+        self.tet_plane = _arr(src.tet_plane, nt, np.int32)
+        self.with_faces = F is not None
+        if F is not None:
+            self.face_offset = _arr(F.face_offset, n + 1, np.uint64)
+            nf = int(self.face_offset[-1])
+            self.face_plane = _arr(F.face_plane, nf, np.int32)
+            self.face_count = _arr(F.face_count, nf, np.uint32)
+            self.face_vertex_offset = _arr(F.face_vertex_offset, nf, np.uint64)
+            self.conn_offset = _arr(F.conn_offset, n + 1, np.uint64)
+            self.conn = _arr(F.conn, int(self.conn_offset[-1]), np.uint32)
 
 
-def build_tets(xyz, anchor, width, dimensionality=3, periodic=False, mask=None):
+def build_tets(xyz, anchor, width, dimensionality=3, periodic=False, mask=None,
+               with_faces=False):
     xyz, m, a, w = _prep(xyz, mask, anchor, width)
     L = lib()
     out = C.POINTER(_OrcTets)()
+    fout = C.POINTER(_OrcFaces)()
     st = L.orc_build_tets(xyz.ctypes.data, xyz.shape[0],
                           m.ctypes.data if m is not None else None,
                           a.ctypes.data, w.ctypes.data, int(dimensionality),
@@ -231,8 +268,15 @@ def build_tets(xyz, anchor, width, dimensionality=3, periodic=False, mask=None):
     try:
         if st != 0:
             raise OracleError(st, -1)
-        return OracleTets(out.contents)
+        if not with_faces:
+            return OracleTets(out.contents)
+        st = L.orc_with_faces(out, int(dimensionality), C.byref(fout))
+        if st != 0:
+            raise OracleError(st, -1)
+        return OracleTets(out.contents, fout.contents)
     finally:
+        if fout:
+            L.orc_free_faces(fout)
         if out:
             L.orc_free_tets(out)
 
