@@ -200,7 +200,6 @@ class VoronoiIntegrator:
               dimensionality=Dimensionality.ThreeD, periodic: bool = False,
               options: Optional[BuildOptions] = None) -> "VoronoiIntegrator":
         """src/voronoi.rs:549-621"""
-        self = object.__new__(cls)
         L = _native.lib()
         xyz = np.ascontiguousarray(np.asarray(generators, dtype=np.float64).reshape(-1, 3))
         n = xyz.shape[0]
@@ -217,11 +216,15 @@ class VoronoiIntegrator:
                           m.ctypes.data if m is not None else None, a.ctypes.data,
                           w.ctypes.data, int(dimensionality), int(bool(periodic)),
                           C.byref(opt), C.byref(h)))
+        return cls._wrap(h, xyz, m, int(dimensionality))
+
+    @classmethod
+    def _wrap(cls, h, xyz, m, d) -> "VoronoiIntegrator":
+        self = object.__new__(cls)
         self._h = h
-        self._n = n
+        self._n = xyz.shape[0]
         self._mask = m
         self._loc = xyz.copy()
-        d = int(dimensionality)
         self._dim = d
         if d < 3:
             self._loc[:, 2] = 0.0
@@ -231,6 +234,27 @@ class VoronoiIntegrator:
         self._faces_on = False
         self._cache = {}
         return self
+
+    @classmethod
+    def build_batch(cls, problems, dimensionality=Dimensionality.ThreeD, masks=None,
+                    options: Optional[BuildOptions] = None) -> "List[VoronoiIntegrator]":
+        """One integrator per problem of a batched build (mv_build_batch with
+        MV_FLAG_KEEP_CELLS): `problems` is a sequence of (generators,
+        anchor, width, periodic), `masks` one mask or None per problem.  Each
+        equals VoronoiIntegrator.build of its problem alone."""
+        from .voronoi import _build_batch_handles
+        if len(problems) == 0:
+            return []
+        hs = _build_batch_handles(problems, dimensionality, masks, options,
+                                  _native.MV_FLAG_KEEP_CELLS)
+        out = []
+        for p, h in enumerate(hs):
+            xyz = np.ascontiguousarray(np.asarray(problems[p][0], dtype=np.float64).reshape(-1, 3))
+            m = None
+            if masks is not None and masks[p] is not None:
+                m = np.ascontiguousarray(np.asarray(masks[p], dtype=np.uint8))
+            out.append(cls._wrap(h, xyz, m, int(dimensionality)))
+        return out
 
     def __del__(self):
         h = getattr(self, "_h", None)

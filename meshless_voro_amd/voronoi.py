@@ -202,36 +202,8 @@ class Voronoi:
         P = len(problems)
         if P == 0:
             return []
-        xs = [np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1, 3))
-              for g, _, _, _ in problems]
-        offsets = np.zeros(P + 1, np.uint64)
-        offsets[1:] = np.cumsum([len(x) for x in xs])
-        xyz = np.ascontiguousarray(np.concatenate(xs, 0)) if offsets[-1] else np.zeros((0, 3))
-        anchors = np.ascontiguousarray(np.stack([_vec3(a) for _, a, _, _ in problems]))
-        widths = np.ascontiguousarray(np.stack([_vec3(w) for _, _, w, _ in problems]))
-        per = np.ascontiguousarray(np.array([int(bool(p)) for _, _, _, p in problems], np.uint32))
-        m = None
-        if masks is not None and any(mk is not None for mk in masks):
-            parts = []
-            for x, mk in zip(xs, masks):
-                if mk is None:
-                    parts.append(np.ones(len(x), np.uint8))
-                else:
-                    mk = np.asarray(mk).astype(np.uint8).reshape(-1)
-                    if mk.shape[0] < len(x):
-                        raise VoronoiError(1)
-                    parts.append(mk[:len(x)])
-            m = np.ascontiguousarray(np.concatenate(parts)) if parts else None
-        L = _native.lib()
-        opt = (options or BuildOptions()).to_c()
-        hs = (C.c_void_p * P)()
-        st = L.mv_build_batch(xyz.ctypes.data if offsets[-1] else None, offsets.ctypes.data, P,
-                              m.ctypes.data if m is not None else None, anchors.ctypes.data,
-                              widths.ctypes.data, per.ctypes.data,
-                              int(Dimensionality(dimensionality)), C.byref(opt), hs)
-        if st != 0:
-            raise VoronoiError(st)
-        return [cls._from_handle(C.c_void_p(hs[p])) for p in range(P)]
+        hs = _build_batch_handles(problems, dimensionality, masks, options, 0)
+        return [cls._from_handle(h) for h in hs]
 
     # --- accessors (src/voronoi.rs:317-359) ---
     def anchor(self) -> np.ndarray:
@@ -254,6 +226,43 @@ class Voronoi:
 
     def periodic(self) -> bool:
         return self._periodic
+
+
+def _build_batch_handles(problems, dimensionality, masks, options, flags):
+    """mv_build_batch over (generators, anchor, width, periodic) problems:
+    one handle per problem (options' flags OR `flags`)."""
+    P = len(problems)
+    xs = [np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1, 3))
+          for g, _, _, _ in problems]
+    offsets = np.zeros(P + 1, np.uint64)
+    offsets[1:] = np.cumsum([len(x) for x in xs])
+    xyz = np.ascontiguousarray(np.concatenate(xs, 0)) if offsets[-1] else np.zeros((0, 3))
+    anchors = np.ascontiguousarray(np.stack([_vec3(a) for _, a, _, _ in problems]))
+    widths = np.ascontiguousarray(np.stack([_vec3(w) for _, _, w, _ in problems]))
+    per = np.ascontiguousarray(np.array([int(bool(p)) for _, _, _, p in problems], np.uint32))
+    m = None
+    if masks is not None and any(mk is not None for mk in masks):
+        parts = []
+        for x, mk in zip(xs, masks):
+            if mk is None:
+                parts.append(np.ones(len(x), np.uint8))
+            else:
+                mk = np.asarray(mk).astype(np.uint8).reshape(-1)
+                if mk.shape[0] < len(x):
+                    raise VoronoiError(1)
+                parts.append(mk[:len(x)])
+        m = np.ascontiguousarray(np.concatenate(parts)) if parts else None
+    L = _native.lib()
+    opt = (options or BuildOptions()).to_c()
+    opt.flags |= flags
+    hs = (C.c_void_p * P)()
+    st = L.mv_build_batch(xyz.ctypes.data if offsets[-1] else None, offsets.ctypes.data, P,
+                          m.ctypes.data if m is not None else None, anchors.ctypes.data,
+                          widths.ctypes.data, per.ctypes.data,
+                          int(Dimensionality(dimensionality)), C.byref(opt), hs)
+    if st != 0:
+        raise VoronoiError(st)
+    return [C.c_void_p(hs[p]) for p in range(P)]
 
 
 class VoronoiCell:

@@ -228,3 +228,28 @@ def test_with_faces_needs_3d():
     with pytest.raises(VoronoiError) as e:
         it.with_faces()
     assert e.value.status == 11
+
+
+def test_batch_problems_with_faces():
+    """Integrators of a batched build (one per problem, sharing the build's
+    device arenas) give the face lists, fan integrals and Voronoi of each
+    problem's own build."""
+    probs = [(W.uniform(700, 930), ZERO, ONE, True),
+             (W.clustered(900, 931), ZERO, ONE, False),
+             (W.uniform(500, 932) * 2.0, ZERO, [2.0] * 3, True)]
+    masks = [None, W.partial_mask(900, 0.4, 933), None]
+    print("case with_faces batch of 3", flush=True)
+    its = VoronoiIntegrator.build_batch(probs, masks=masks)
+    assert len(its) == 3
+    for (x, a, w, per), mk, it in zip(probs, masks, its):
+        one = VoronoiIntegrator.build(x, mk, a, w, 3, per)
+        assert np.array_equal(it.active_cells(), one.active_cells())
+        it.with_faces()
+        one.with_faces()
+        for p, q in zip(it.face_vertex_arrays(), one.face_vertex_arrays()):
+            assert np.array_equal(p, q)
+        assert np.array_equal(it.compute_cell_integrals_array(), one.compute_cell_integrals_array())
+        vb, vo = Voronoi.from_integrator(it), Voronoi.from_integrator(one)
+        for f in ("cell_volume", "cell_centroid", "face_area", "face_centroid", "face_left",
+                  "face_right", "links"):
+            assert np.array_equal(getattr(vb, f), getattr(vo, f)), f
