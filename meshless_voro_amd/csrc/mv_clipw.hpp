@@ -211,7 +211,8 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
 // wavefronts per SIMD, and hands such cells (rare outside lattices) to the
 // EXACT instantiation through A.exact_list.
 template <bool KEEP, bool EXACT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(EXACT || KEEP ? 3 : 4)))
 k_clip_wave(WaveArgs A) {
   __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
@@ -323,28 +324,46 @@ k_clip_wave(WaveArgs A) {
       // entry t's plane is read from LDS one entry ahead
       double4 nd_next = S.u.e.nd[0];
       double2 de_next = S.u.e.de[0];
-      for (int t = 0; t < nb; t++) {
+      int t = 0;
+      while (t < nb) {
+        // scan the entries that leave the cell unchanged (no stop, no
+        // vertex near or inside their plane); only t and the prefetched
+        // plane change from one to the next
+        const uint64_t act_m = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+        const bool act = lane < nv;
+        double4 nd;
+        double errb, cv;
+        bool stop_hit = false;
+        for (;;) {
+          if (t == tbad) break;
+          nd = nd_next;
+          const double dist = de_next.x;
+          errb = de_next.y;
+          nd_next = S.u.e.nd[min(t + 1, 63)];
+          de_next = S.u.e.de[min(t + 1, 63)];
+          // sr < dist: every vertex has vs < dist
+          if (!(__ballot(!(vs < dist)) & act_m)) {
+            stop_hit = true;
+            break;
+          }
+          // classify this lane's vertex (clip_by_plane, convex_cell.rs:368-390)
+          cv = dot3(xyz(nd), mk3(vx, vy, vz)) - nd.w;
+          if (__ballot(act && (fabs(cv) < errb || cv < 0.))) break;
+          if (++t >= nb) break;
+        }
+        if (t >= nb) break;
         if (t == tbad) {
           state = CS_ERROR;
           err = E_DEGENERATE;
           break;
         }
-        const double4 nd = nd_next;
-        const double dist = de_next.x, errb = de_next.y;
-        nd_next = S.u.e.nd[min(t + 1, 63)];
-        de_next = S.u.e.de[min(t + 1, 63)];
-        // sr < dist: every vertex has vs < dist
-        const uint64_t act_m = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
-        if (!(__ballot(!(vs < dist)) & act_m)) {
+        if (stop_hit) {
           stopped = true;
           npop = j0 + (uint32_t)t + 1;
           break;
         }
         const d3 n = xyz(nd);
         const double d = nd.w;
-        // classify this lane's vertex (clip_by_plane, convex_cell.rs:368-390)
-        const bool act = lane < nv;
-        const double cv = dot3(n, mk3(vx, vy, vz)) - d;
         const bool near = act && fabs(cv) < errb;
         bool clipped = act && !near && cv < 0.;
         const uint64_t ref = (uint64_t)__double_as_longlong(S.u.e.pr[t].w);
@@ -368,7 +387,10 @@ k_clip_wave(WaveArgs A) {
           }
         }
         uint64_t M = __ballot(clipped);
-        if (!M) continue;
+        if (!M) {
+          t++;
+          continue;
+        }
         // the swap-with-last scan of clip_by_plane (convex_cell.rs:368-390)
         // in closed form.  With K kept vertices, holes h_1 < h_2 < ... (the
         // clipped slots below K) and b_1 > b_2 > ... (the kept slots from K
@@ -605,6 +627,7 @@ k_clip_wave(WaveArgs A) {
         // update_safety_radius (convex_cell.rs:443-453) is the maximum of
         // vs over every vertex (kept ones were checked for NaN when
         // created); the stop test above reads the vs directly
+        t++;
       }
     }
     const double sr = wave_max_nonneg(lane < nv ? vs : 0.);
