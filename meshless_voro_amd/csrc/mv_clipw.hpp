@@ -321,34 +321,42 @@ k_clip_wave(WaveArgs A) {
       const int tbad = bad_m ? (int)__builtin_ctzll(bad_m) : 64;
       lds_fence();
       const int nb = (int)min(64u, cnt - j0);
-      // entry t's plane is read from LDS one entry ahead
-      double4 nd_next = S.u.e.nd[0];
-      double2 de_next = S.u.e.de[0];
       int t = 0;
       while (t < nb) {
         // scan the entries that leave the cell unchanged (no stop, no
-        // vertex near or inside their plane); only t and the prefetched
-        // plane change from one to the next
+        // vertex near or inside their plane).  Unrolled by two with the
+        // next entry's plane read one ahead into the other buffer, so the
+        // loop carries only t and one buffer.
         const uint64_t act_m = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
         const bool act = lane < nv;
-        double4 nd;
-        double errb, cv;
+        const d3 vl = mk3(vx, vy, vz);
+        // sr < dist: every vertex has vs < dist; an event: a vertex near
+        // or inside the plane (clip_by_plane, convex_cell.rs:368-390)
+        auto stops = [&](double dist) { return !(__ballot(!(vs < dist)) & act_m); };
+        auto touches = [&](double4 e, double errb) {
+          const double c = dot3(xyz(e), vl) - e.w;
+          return __ballot(act && (fabs(c) < errb || c < 0.)) != 0ull;
+        };
         bool stop_hit = false;
+        double4 ndA = S.u.e.nd[t];
+        double2 deA = S.u.e.de[t];
         for (;;) {
           if (t == tbad) break;
-          nd = nd_next;
-          const double dist = de_next.x;
-          errb = de_next.y;
-          nd_next = S.u.e.nd[min(t + 1, 63)];
-          de_next = S.u.e.de[min(t + 1, 63)];
-          // sr < dist: every vertex has vs < dist
-          if (!(__ballot(!(vs < dist)) & act_m)) {
+          const double4 ndB = S.u.e.nd[min(t + 1, 63)];
+          const double2 deB = S.u.e.de[min(t + 1, 63)];
+          if (stops(deA.x)) {
             stop_hit = true;
             break;
           }
-          // classify this lane's vertex (clip_by_plane, convex_cell.rs:368-390)
-          cv = dot3(xyz(nd), mk3(vx, vy, vz)) - nd.w;
-          if (__ballot(act && (fabs(cv) < errb || cv < 0.))) break;
+          if (touches(ndA, deA.y)) break;
+          if (++t >= nb || t == tbad) break;
+          ndA = S.u.e.nd[min(t + 1, 63)];
+          deA = S.u.e.de[min(t + 1, 63)];
+          if (stops(deB.x)) {
+            stop_hit = true;
+            break;
+          }
+          if (touches(ndB, deB.y)) break;
           if (++t >= nb) break;
         }
         if (t >= nb) break;
@@ -362,6 +370,10 @@ k_clip_wave(WaveArgs A) {
           npop = j0 + (uint32_t)t + 1;
           break;
         }
+        // the event's entry, read again
+        const double4 nd = S.u.e.nd[t];
+        const double errb = S.u.e.de[t].y;
+        const double cv = dot3(xyz(nd), vl) - nd.w;
         const d3 n = xyz(nd);
         const double d = nd.w;
         const bool near = act && fabs(cv) < errb;
