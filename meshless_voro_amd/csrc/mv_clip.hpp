@@ -642,26 +642,36 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
   if (my_exact) atomicAdd(A.exact_counter, my_exact);
 }
 
-// One cell per wavefront, for the cells over the wavefront kernel's 64
-// vertices or planes (long lists, large cells).  Every lane runs the same
-// sequential steps on the cell's arrays (slot = wavefront index, so the
-// lanes of a wavefront share them and store the same values); the vertex
-// classification of each plane is split over the lanes.  Results, faces
-// and counters are written by lane 0.
+// The arrays of one cell of k_clip_coop, in LDS (~60 KiB at 255 planes
+// and 1024 vertices).
+template <int MAXP, int MAXV>
+struct CoopLds {
+  double4 plA[MAXP], plB[MAXP], facc[MAXP], vtx[MAXV];
+  uint32_t vdual[MAXV];
+  uint8_t cyc[MAXP];
+};
+
+// One cell per wavefront (a block of one wavefront), for the cells over
+// the wavefront kernel's 64 vertices or planes (long lists, large cells).
+// Every lane runs the same sequential steps on the cell's arrays in LDS
+// (the lanes store the same values); the vertex classification of each
+// plane is split over the lanes.  Results, faces and counters are written
+// by lane 0.
 template <int MAXP, int MAXV, bool KEEP>
-__global__ void __launch_bounds__(256) k_clip_coop(ClipArgs A) {
+__global__ void __launch_bounds__(64) k_clip_coop(ClipArgs A) {
+  __shared__ CoopLds<MAXP, MAXV> L;
   const uint32_t lane = __lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t wid = blockIdx.x;
+  const uint32_t nwaves = gridDim.x;
   Lane S;
-  S.plA = A.plA;
-  S.plB = A.plB;
-  S.vtx = A.vtx;
-  S.facc = A.facc;
-  S.vdual = A.vdual;
-  S.cyc = A.cyc;
-  S.lane = wid;
-  S.lanes = A.lanes;
+  S.plA = L.plA;
+  S.plB = L.plB;
+  S.vtx = L.vtx;
+  S.facc = L.facc;
+  S.vdual = L.vdual;
+  S.cyc = L.cyc;
+  S.lane = 0;
+  S.lanes = 1;
   unsigned long long my_exact = 0;
   for (uint32_t w = wid; w < A.nwork; w += nwaves) {
     uint32_t nf = 0;

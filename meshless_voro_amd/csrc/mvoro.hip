@@ -847,8 +847,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     L.vdual = need(TT.get<uint32_t>((size_t)maxv * lanes));
     return L;
   };
-  LaneBufs LM{}, LB{};
-  bool have_main = false, have_big = false;
+  LaneBufs LM{};
+  bool have_main = false;
   uint64_t redone = 0, bigcells = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
@@ -1014,14 +1014,9 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           A.vdual = LM.vdual; A.cyc = LM.cyc;
           A.lanes = LM.lanes;
         } else {
-          lanes = std::min<uint32_t>(lanes_big, (cnt + 255) / 256 * 256);
-          if (!have_big) {
-            LB = alloc_lanes(lanes_big, MAXP_BIG, MAXV_BIG);
-            have_big = true;
-          }
-          A.plA = LB.plA; A.plB = LB.plB; A.vtx = LB.vtx; A.facc = LB.facc;
-          A.vdual = LB.vdual; A.cyc = LB.cyc;
-          A.lanes = LB.lanes;
+          // k_clip_coop keeps each cell's arrays in LDS
+          lanes = std::min<uint32_t>(lanes_big, cnt);
+          A.lanes = lanes;
         }
         A.wstate = state;
         A.cell_vol = T->d_cell_vol;
@@ -1044,17 +1039,17 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.overflow = counters + 3;
         if (T->keep) A.cs = T->cs;
         const dim3 grid(lanes / 256);
-        // large cells: one cell per wavefront (a slot per wavefront)
-        const dim3 grid_coop((std::min<uint32_t>(cnt, A.lanes) + 3) / 4);
+        // large cells: one cell per wavefront, a block each
+        const dim3 grid_coop(std::max<uint32_t>(lanes, 1));
         if (!big && !T->keep)
           hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, false>), grid, dim3(256), 0, s, A);
         else if (!big)
           hipLaunchKernelGGL((k_clip<MAXP_MAIN, MAXV_MAIN, true>), grid, dim3(256), 0, s, A);
         else if (!T->keep)
-          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>), grid_coop, dim3(256), 0,
+          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>), grid_coop, dim3(64), 0,
                              s, A);
         else
-          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, true>), grid_coop, dim3(256), 0,
+          hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, true>), grid_coop, dim3(64), 0,
                              s, A);
         HIPCHK(hipGetLastError());
         ms_clip += tc.stop();
