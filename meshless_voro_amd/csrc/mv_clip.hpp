@@ -364,6 +364,50 @@ __device__ __forceinline__ bool should_construct(const Lane &S,
   return true;
 }
 
+// COOP: the first entry from j on that the sequential loop of build_cell
+// must run -- one whose distance is not finite and positive, that stops
+// the cell (sr < dist), or whose plane has a vertex near it or inside
+// (the only planes clip_by_plane acts on).  The entries before it leave the
+// cell as it is, so skipping them is exact.  Lane l tests entry j + l
+// against every vertex (the plane as build_cell computes it, the vertex
+// test as clip_by_plane's), 64 entries at a time.  Returns cnt if none.
+__device__ __forceinline__ uint32_t coop_scan(const Lane &S, const CellRegs &c,
+                                              const ClipArgs &A, const uint64_t *list,
+                                              uint32_t j, uint32_t cnt) {
+  const BoxParams &B = *c.B;
+  const uint32_t lane = __lane_id();
+  for (; j < cnt; j += 64) {
+    const uint32_t je = j + lane;
+    bool hit = false;
+    if (je < cnt) {
+      const uint64_t pk = list[je];
+      d3 ngb = xyz(A.spos[pk_slot(pk)]);
+      d3 s;
+      if (image_shift(pk_img(pk), B.width, &s)) ngb = add3(ngb, s);
+      const d3 dx = sub3(c.q, ngb);
+      const double dist = sqrt(dot3(dx, dx));
+      if (!(isfinite(dist) && dist > 0.0) || c.sr < dist) {
+        hit = true;
+      } else {
+        const d3 n = div3(dx, dist);
+        const d3 p = scl3(0.5, add3(c.q, ngb));
+        const double errb = 1e-13 * (1. + dot3(abs3(n), abs3(p)));
+        const double d = dot3(n, p);
+        for (int v = 0; v < c.nv; v++) {
+          const double cv = dot3(n, xyz(S.vtx[S.at(v)])) - d;
+          if (fabs(cv) < errb || cv < 0.) {
+            hit = true;
+            break;
+          }
+        }
+      }
+    }
+    const uint64_t m = __ballot(hit);
+    if (m) return j + (uint32_t)__builtin_ctzll(m);
+  }
+  return cnt;
+}
+
 // Runs one cell.  Returns its CellState; on CS_DONE, *nf = faces to emit.
 template <int MAXP, int MAXV, bool COOP = false>
 __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
@@ -411,6 +455,12 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   bool stopped = false;
   uint32_t j = 1;
   for (; j < cnt; j++) {
+    // past the first entries (which clip often), a large cell looks for
+    // its next event 64 entries at a time
+    if (COOP && j >= 32) {
+      j = coop_scan(S, c, A, list, j, cnt);
+      if (j >= cnt) break;
+    }
     uint64_t pk = list[j];
     double4 g = A.spos[pk_slot(pk)];
     d3 ngb = xyz(g);
