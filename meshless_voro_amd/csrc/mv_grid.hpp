@@ -120,12 +120,21 @@ __global__ void k_grid_scatter(const uint32_t *__restrict__ cellid, size_t n,
 }
 
 // make each grid cell's list ascending in generator index (deterministic
-// slot order)
+// slot order).  A thread sorts a cell of up to GRID_SMALL seeds by
+// insertion; larger cells (dense clusters put tens of thousands of seeds in
+// one cell) are listed for k_grid_sort_big.
+constexpr uint32_t GRID_SMALL = 32;
 __global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
-                                  size_t ncell, uint32_t *__restrict__ order) {
+                                  size_t ncell, uint32_t *__restrict__ order,
+                                  uint32_t *__restrict__ big_list,
+                                  unsigned int *__restrict__ big_cnt) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   uint32_t b = cell_start[c], e = cell_start[c + 1];
+  if (e - b > GRID_SMALL) {
+    big_list[atomicAdd(big_cnt, 1u)] = (uint32_t)c;
+    return;
+  }
   for (uint32_t i = b + 1; i < e; i++) {
     uint32_t v = order[i];
     uint32_t j = i;
@@ -134,6 +143,78 @@ __global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
       j--;
     }
     order[j] = v;
+  }
+}
+
+// One block per large cell (of the list k_grid_sort_cells made): chunks of
+// GS_CHUNK indices sorted by a bitonic network in LDS, then runs of width
+// w merged pairwise between the cell's range of `order` and the same range
+// of `tmp`.  The indices are distinct, so an index's place in the merged
+// run is its rank in its own run plus the number of smaller indices in the
+// partner run (a binary search), with no merge path to share.
+constexpr uint32_t GS_CHUNK = 4096;
+constexpr int GS_THREADS = 1024;
+__global__ void __launch_bounds__(GS_THREADS)
+    k_grid_sort_big(const uint32_t *__restrict__ cell_start,
+                    const uint32_t *__restrict__ big_list,
+                    const unsigned int *__restrict__ big_cnt, uint32_t *__restrict__ order,
+                    uint32_t *__restrict__ tmp) {
+  __shared__ uint32_t s[GS_CHUNK];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nbig = *big_cnt;
+  for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
+    const uint32_t c = big_list[k];
+    const uint32_t b = cell_start[c], m = cell_start[c + 1] - b;
+    for (uint32_t c0 = 0; c0 < m; c0 += GS_CHUNK) {
+      const uint32_t len = min(GS_CHUNK, m - c0);
+      uint32_t n2 = 1;
+      while (n2 < len) n2 <<= 1;
+      for (uint32_t i = tid; i < n2; i += GS_THREADS)
+        s[i] = i < len ? order[b + c0 + i] : 0xFFFFFFFFu;
+      __syncthreads();
+      for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+          for (uint32_t p = tid; p < (n2 >> 1); p += GS_THREADS) {
+            const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+            const uint32_t x = s[i], y = s[i | j];
+            if ((x > y) == ((i & kk) == 0)) {
+              s[i] = y;
+              s[i | j] = x;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t i = tid; i < len; i += GS_THREADS) order[b + c0 + i] = s[i];
+      __syncthreads();
+    }
+    uint32_t *src = order + b, *dst = tmp + b;
+    for (uint32_t w = GS_CHUNK; w < m; w <<= 1) {
+      for (uint32_t i = tid; i < m; i += GS_THREADS) {
+        const uint32_t r0 = i / (2 * w) * (2 * w);
+        const uint32_t a1 = min(r0 + w, m), b1 = min(r0 + 2 * w, m);
+        const bool in_a = i < a1;
+        const uint32_t x = src[i];
+        uint32_t lo = in_a ? a1 : r0, hi = in_a ? b1 : a1;
+        const uint32_t base = lo;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (src[mid] < x)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        dst[r0 + (i - (in_a ? r0 : a1)) + (lo - base)] = x;
+      }
+      __syncthreads();
+      uint32_t *t = src;
+      src = dst;
+      dst = t;
+    }
+    if (src != order + b) {
+      for (uint32_t i = tid; i < m; i += GS_THREADS) order[b + i] = src[i];
+    }
+    __syncthreads();
   }
 }
 

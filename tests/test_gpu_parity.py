@@ -269,3 +269,17 @@ def test_clusters_in_empty_space_need_long_lists():
     assert o.cell_n_popped.max() > 32768
     # the rim cells went through the large-cell kernel (over 64 vertices)
     assert g.stats["big_cells"] > 0
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_dense_cluster_in_one_grid_cell(periodic):
+    """8,000 seeds in a Gaussian cluster of sigma 2e-4 inside one grid cell
+    (the grid is sized for the mean density): the cell's seeds are put in
+    index order by the block sort (LDS chunks of 4,096, then merges), and
+    the tessellation equals the oracle's."""
+    rng = np.random.default_rng(2024)
+    x = np.concatenate([rng.random((12000, 3)),
+                        0.5 + 2e-4 * rng.normal(size=(8000, 3))])
+    g, o = both(x, periodic=periodic)
+    check(g, o)
+    assert np.array_equal(g.cell_n_popped, o.cell_n_popped)
