@@ -756,24 +756,42 @@ __global__ void __launch_bounds__(256)
 }
 
 // Global-memory variant for long lists: one block of 1024 threads per item
-// owns `cap` entries of scratch.  Its first wavefront gathers the ball; the
-// block sorts it with a bitonic network padded to a power of two with +inf
-// keys, doing the stages that pair entries less than KB_CHUNK apart in LDS
-// (one chunk at a time) and only the wider ones in global memory, and
-// writes the first L.
+// owns two buffers of `cap` entries.  Its first wavefront gathers the ball
+// into the first; the block bucket-sorts it: a histogram of the keys over
+// KB_BINS equal key ranges (in LDS), a scatter into bin order in the second
+// buffer, then windows of consecutive bins of at most KB_WIN entries sorted
+// in LDS (a bin larger than that alone, in global memory) back into the
+// first; the first L are written out.  Equal keys share a bin, so sorting
+// each window by (key, index, image) orders the whole ball.
 constexpr int KB_THREADS = 1024;
-constexpr uint32_t KB_CHUNK = 2048;  // entries per LDS chunk (32 KiB)
+constexpr uint32_t KB_BINS = 8192;  // histogram bins (32 KiB of LDS)
+constexpr uint32_t KB_WIN = 2000;   // entries per LDS window (32,000 B)
 
-__device__ __forceinline__ void cmpx(double &ka, uint64_t &pa, double &kb, uint64_t &pb,
-                                     bool up) {
-  const bool sw = up ? entry_less(kb, pb, ka, pa) : entry_less(ka, pa, kb, pb);
-  if (sw) {
-    const double tk = ka;
-    const uint64_t tp = pa;
-    ka = kb;
-    pa = pb;
-    kb = tk;
-    pb = tp;
+// Bitonic sort of m entries, all comparators ascending (the first stage of
+// each merge pairs i with its mirror i ^ (k - 1)), so entries past m act as
+// +inf and are never touched: no padding.  One block of KB_THREADS.
+template <class K, class P>
+__device__ __forceinline__ void flip_bitonic(K *key, P *pk, uint32_t m, uint32_t tid) {
+  uint32_t n2 = 1;
+  while (n2 < m) n2 <<= 1;
+  for (uint32_t k = 2; k <= n2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t p = tid; p < (n2 >> 1); p += KB_THREADS) {
+        const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+        const uint32_t l = j == (k >> 1) ? i ^ (k - 1) : i | j;
+        if (l < m) {
+          const double ka = key[i], kb = key[l];
+          const uint64_t pa = pk[i], pb = pk[l];
+          if (entry_less(kb, pb, ka, pa)) {
+            key[i] = kb;
+            pk[i] = pb;
+            key[l] = ka;
+            pk[l] = pa;
+          }
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -786,15 +804,21 @@ __global__ void __launch_bounds__(KB_THREADS)
               const uint64_t *__restrict__ kofs, uint64_t *__restrict__ kpk,
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
+              double *__restrict__ gkey2, uint64_t *__restrict__ gpk2,
               uint32_t cap) {
-  __shared__ double lkey[KB_CHUNK];
-  __shared__ uint64_t lpk[KB_CHUNK];
+  __shared__ uint32_t hist[KB_BINS];
+  __shared__ double wkey[KB_WIN];
+  __shared__ uint64_t wpk[KB_WIN];
+  __shared__ uint32_t s_wsum[KB_THREADS / 64];
   __shared__ uint32_t s_cnt;
   __shared__ double s_R;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
+  const uint32_t wid = tid >> 6;
   double *bkey = gkey + (size_t)blockIdx.x * cap;
   uint64_t *bpk = gpk + (size_t)blockIdx.x * cap;
+  double *tkey = gkey2 + (size_t)blockIdx.x * cap;
+  uint64_t *tpk = gpk2 + (size_t)blockIdx.x * cap;
   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const uint32_t slot = work[w];
     const uint32_t pr = problem_of(PR, slot);
@@ -824,68 +848,93 @@ __global__ void __launch_bounds__(KB_THREADS)
       __syncthreads();
       continue;
     }
-    uint32_t n2 = KB_CHUNK;
-    while (n2 < cnt) n2 <<= 1;
-    for (uint32_t e = cnt + tid; e < n2; e += KB_THREADS) {
-      bkey[e] = INFINITY;
-      bpk[e] = ~0ull;
+    // histogram of the keys (all kept keys are <= R^2, the gather's bound)
+    const double inv = (double)KB_BINS / (R * R);
+    auto bin_of = [&](double k) { return min(KB_BINS - 1u, (uint32_t)(k * inv)); };
+    for (uint32_t b = tid; b < KB_BINS; b += KB_THREADS) hist[b] = 0u;
+    __syncthreads();
+    for (uint32_t e = tid; e < cnt; e += KB_THREADS) atomicAdd(&hist[bin_of(bkey[e])], 1u);
+    __syncthreads();
+    {
+      // exclusive scan of the bins: eight per thread, then across the block
+      constexpr uint32_t PER = KB_BINS / KB_THREADS;
+      uint32_t v[PER], sum = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) {
+        v[i] = hist[PER * tid + i];
+        sum += v[i];
+      }
+      uint32_t x = sum;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if ((int)lane >= o) x += y;
+      }
+      if (lane == 63) s_wsum[wid] = x;
+      __syncthreads();
+      if (wid == 0) {
+        uint32_t wv = lane < KB_THREADS / 64 ? s_wsum[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < KB_THREADS / 64; o <<= 1) {
+          const uint32_t y = (uint32_t)__shfl_up((int)wv, o, 64);
+          if ((int)lane >= o) wv += y;
+        }
+        if (lane < KB_THREADS / 64) s_wsum[lane] = wv;
+      }
+      __syncthreads();
+      uint32_t base = (wid ? s_wsum[wid - 1] : 0u) + x - sum;
+#pragma unroll
+      for (uint32_t i = 0; i < PER; i++) {
+        hist[PER * tid + i] = base;
+        base += v[i];
+      }
+      __syncthreads();
+    }
+    // scatter into bin order; hist[b] ends as the end of bin b
+    for (uint32_t e = tid; e < cnt; e += KB_THREADS) {
+      const double k = bkey[e];
+      const uint32_t pos = atomicAdd(&hist[bin_of(k)], 1u);
+      tkey[pos] = k;
+      tpk[pos] = bpk[e];
     }
     __syncthreads();
-    // stages jj < KB_CHUNK of level k on one chunk held in LDS
-    auto chunk_stages = [&](uint32_t c0, uint32_t k, uint32_t j_hi) {
-      for (uint32_t jj = j_hi; jj > 0; jj >>= 1) {
-        const uint32_t i = ((tid & ~(jj - 1)) << 1) | (tid & (jj - 1));  // pair tid
-        double ki = lkey[i], kl = lkey[i | jj];
-        uint64_t pi = lpk[i], pl = lpk[i | jj];
-        cmpx(ki, pi, kl, pl, ((c0 + i) & k) == 0);
-        lkey[i] = ki;
-        lpk[i] = pi;
-        lkey[i | jj] = kl;
-        lpk[i | jj] = pl;
-        __syncthreads();
+    // windows of whole bins, sorted back into the first buffer
+    for (uint32_t s0 = 0; s0 < cnt;) {
+      // the bins that end at most KB_WIN past s0 (bin ends do not decrease)
+      uint32_t lo = 0, hi = KB_BINS;
+      const uint32_t lim = s0 + KB_WIN;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hist[mid] <= lim)
+          lo = mid + 1;
+        else
+          hi = mid;
       }
-    };
-    auto load_chunk = [&](uint32_t c0) {
-      for (uint32_t e = tid; e < KB_CHUNK; e += KB_THREADS) {
-        lkey[e] = bkey[c0 + e];
-        lpk[e] = bpk[c0 + e];
-      }
-      __syncthreads();
-    };
-    auto store_chunk = [&](uint32_t c0) {
-      for (uint32_t e = tid; e < KB_CHUNK; e += KB_THREADS) {
-        bkey[c0 + e] = lkey[e];
-        bpk[c0 + e] = lpk[e];
-      }
-      __syncthreads();
-    };
-    // levels up to the chunk size: each chunk sorted in LDS (alternating
-    // directions, as the network has them)
-    for (uint32_t c0 = 0; c0 < n2; c0 += KB_CHUNK) {
-      load_chunk(c0);
-      for (uint32_t k = 2; k <= KB_CHUNK; k <<= 1) chunk_stages(c0, k, k >> 1);
-      store_chunk(c0);
-    }
-    // wider levels: stages pairing entries a chunk or more apart in global
-    // memory, the rest chunk by chunk in LDS
-    for (uint32_t k = 2 * KB_CHUNK; k <= n2; k <<= 1) {
-      for (uint32_t j = k >> 1; j >= KB_CHUNK; j >>= 1) {
-        for (uint32_t p = tid; p < (n2 >> 1); p += KB_THREADS) {
-          const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-          double ki = bkey[i], kl = bkey[i | j];
-          uint64_t pi = bpk[i], pl = bpk[i | j];
-          cmpx(ki, pi, kl, pl, (i & k) == 0);
-          bkey[i] = ki;
-          bpk[i] = pi;
-          bkey[i | j] = kl;
-          bpk[i | j] = pl;
+      const uint32_t ew = lo ? hist[lo - 1] : 0u;
+      if (ew > s0) {
+        const uint32_t m = ew - s0;
+        for (uint32_t i = tid; i < m; i += KB_THREADS) {
+          wkey[i] = tkey[s0 + i];
+          wpk[i] = tpk[s0 + i];
         }
         __syncthreads();
-      }
-      for (uint32_t c0 = 0; c0 < n2; c0 += KB_CHUNK) {
-        load_chunk(c0);
-        chunk_stages(c0, k, KB_CHUNK >> 1);
-        store_chunk(c0);
+        flip_bitonic(wkey, wpk, m, tid);
+        for (uint32_t i = tid; i < m; i += KB_THREADS) {
+          bkey[s0 + i] = wkey[i];
+          bpk[s0 + i] = wpk[i];
+        }
+        __syncthreads();
+        s0 = ew;
+      } else {
+        // bin `lo` starts at s0 and holds more than KB_WIN entries
+        const uint32_t e1 = hist[lo];
+        flip_bitonic(tkey + s0, tpk + s0, e1 - s0, tid);
+        for (uint32_t i = s0 + tid; i < e1; i += KB_THREADS) {
+          bkey[i] = tkey[i];
+          bpk[i] = tpk[i];
+        }
+        __syncthreads();
+        s0 = e1;
       }
     }
     const uint32_t outn = min(cnt, Lw);

@@ -411,8 +411,9 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
 
 // neighbour lists: the LDS gather buffer must hold more than L entries;
 // long lists (and cells that did not fit) use the global-memory variant,
-// one block per item, whose buffer holds at least 2 L entries (a power of
-// two, at least 65,536); at most 8 GiB of such buffers are in flight
+// one block per item, whose two buffers hold at least 2 L entries each (a
+// power of two, at least 65,536); at most 8 GiB of such buffers are in
+// flight
 constexpr uint32_t BIG_KNN_CAP = 65536;
 constexpr uint32_t BIG_KNN_BLOCKS = 1024;
 constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
@@ -435,14 +436,17 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
   } else {
     uint32_t cap = BIG_KNN_CAP;
     while (cap < 2 * (uint64_t)L && cap < (1u << 24)) cap <<= 1;
-    const uint64_t per_block = (uint64_t)cap * (sizeof(double) + sizeof(uint64_t));
+    const uint64_t per_block = 2ull * cap * (sizeof(double) + sizeof(uint64_t));
     const uint32_t max_blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(BIG_KNN_BLOCKS, BIG_KNN_BYTES / per_block));
     const uint32_t blocks = std::min<uint32_t>(cnt, max_blocks);
     double *gkey = need(TT.get<double>((size_t)blocks * cap));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)blocks * cap));
+    double *gkey2 = need(TT.get<double>((size_t)blocks * cap));
+    uint64_t *gpk2 = need(TT.get<uint64_t>((size_t)blocks * cap));
     hipLaunchKernelGGL(k_knn_big, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start, PR,
-                       wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, cap);
+                       wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, gkey2, gpk2,
+                       cap);
   }
   HIPCHK(hipGetLastError());
 }
