@@ -372,10 +372,15 @@ __device__ __forceinline__ double entry_dist(const double4 *__restrict__ spos,
 // the rows' offsets held in the other lanes' registers.  A gather thus waits
 // on one dependent load per 64 rows plus one per 64 candidates, instead of
 // two per row.
+// BLK: the wavefronts of a block share the rows (wavefront wid of nw) and
+// reserve their output slots through the LDS counter *blk_cnt (zeroed by
+// the caller), which ends as the count; the return value is then unused.
+template <bool BLK = false>
 __device__ __forceinline__ uint32_t knn_gather(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
     const GridParams &g, const double *width, const double q[3], double R,
-    double *bkey, uint64_t *bpk, uint32_t cap) {
+    double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt = nullptr, int wid = 0,
+    int nw = 1) {
   const uint32_t lane = __lane_id();
   const double R2 = R * R;
   const double rr = R + g.margin;
@@ -408,7 +413,7 @@ __device__ __forceinline__ uint32_t knn_gather(
         const uint32_t img = (uint32_t)((i + 1) * 9 + (j + 1) * 3 + (k + 1));
         const int nb_ = hi[1] - lo[1] + 1;
         const int nrows = (hi[0] - lo[0] + 1) * nb_;
-        for (int r0 = 0; r0 < nrows; r0 += 64) {
+        for (int r0 = BLK ? 64 * wid : 0; r0 < nrows; r0 += BLK ? 64 * nw : 64) {
           const int r = r0 + (int)lane;
           uint32_t t0 = 0, len = 0;
           if (r < nrows) {
@@ -468,7 +473,13 @@ __device__ __forceinline__ uint32_t knn_gather(
               pk = (gidx << 32) | ((uint64_t)t << 5) | img;
             }
             const uint64_t m = __ballot(keep);
-            const uint32_t pos = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            uint32_t base = cnt;
+            if (BLK && m) {
+              uint32_t b0 = 0;
+              if (lane == 0) b0 = atomicAdd(blk_cnt, (uint32_t)__popcll(m));
+              base = (uint32_t)__shfl((int)b0, 0, 64);
+            }
+            const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             if (keep && pos < cap) {
               bkey[pos] = key;
               bpk[pos] = pk;
@@ -494,6 +505,46 @@ __device__ __forceinline__ uint32_t knn_fill(
   uint32_t cnt = 0;
   for (int it = 0; it < 96; it++) {
     cnt = knn_gather(spos, cell_start, g, width, q, R, bkey, bpk, cap);
+    bool all = (uint64_t)cnt >= g.total_stream;
+    if (cnt <= cap && (cnt >= Lw || all)) return cnt;
+    if (cnt > cap) {
+      r_hi = R;
+      if (r_lo > 0.) {
+        R = 0.5 * (r_lo + r_hi);
+      } else {
+        double f = cbrt(0.6 * (double)cap / (double)cnt);
+        R *= fmax(0.5, fmin(0.9, f));
+      }
+    } else {
+      r_lo = R;
+      if (r_hi > 0.) {
+        R = 0.5 * (r_lo + r_hi);
+      } else {
+        double f = cnt ? cbrt(1.4 * (double)Lw / (double)cnt) : 2.0;
+        R *= fmax(1.2, fmin(4.0, f));
+      }
+    }
+    if (r_hi > 0. && r_lo > 0. && r_hi - r_lo <= 1e-15 * r_hi) break;
+  }
+  return cnt > cap ? cnt : cap + 1;
+}
+
+// knn_fill with every wavefront of a KB_THREADS block gathering (all threads
+// call it; the count comes back through the LDS word *blk_cnt).
+__device__ __forceinline__ uint32_t knn_fill_blk(
+    const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
+    const GridParams &g, const double *width, const double q[3], double &R,
+    uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt, int wid,
+    int nw) {
+  double r_lo = 0., r_hi = 0.;
+  uint32_t cnt = 0;
+  for (int it = 0; it < 96; it++) {
+    if (threadIdx.x == 0) *blk_cnt = 0u;
+    __syncthreads();
+    knn_gather<true>(spos, cell_start, g, width, q, R, bkey, bpk, cap, blk_cnt, wid, nw);
+    __syncthreads();
+    cnt = *blk_cnt;
+    __syncthreads();
     bool all = (uint64_t)cnt >= g.total_stream;
     if (cnt <= cap && (cnt >= Lw || all)) return cnt;
     if (cnt > cap) {
@@ -810,8 +861,7 @@ __global__ void __launch_bounds__(KB_THREADS)
   __shared__ double wkey[KB_WIN];
   __shared__ uint64_t wpk[KB_WIN];
   __shared__ uint32_t s_wsum[KB_THREADS / 64];
-  __shared__ uint32_t s_cnt;
-  __shared__ double s_R;
+  __shared__ uint32_t s_gather;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
   const uint32_t wid = tid >> 6;
@@ -827,22 +877,13 @@ __global__ void __launch_bounds__(KB_THREADS)
     const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
-    if (tid < 64) {
-      // wave-uniform: keep them in scalar registers
-      const uint32_t pru = __builtin_amdgcn_readfirstlane(pr);
-      const GridParams g = PR.grids[pru];
-      const uint32_t *cell_start = cell_start_all + PR.coff[pru];
-      double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
-                                         : knn_start_radius(cell_start, g, q, Lw);
-      const uint32_t cnt = knn_fill(spos, cell_start, g, width, q, R, Lw, bkey, bpk, cap);
-      if (lane == 0) {
-        s_cnt = cnt;
-        s_R = R;
-      }
-    }
-    __syncthreads();
-    const uint32_t cnt = s_cnt;
-    const double R = s_R;
+    // block-uniform: keep them in scalar registers
+    const uint32_t pru = __builtin_amdgcn_readfirstlane(pr);
+    const GridParams g = PR.grids[pru];
+    const uint32_t *cell_start = cell_start_all + PR.coff[pru];
+    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
+    const uint32_t cnt = knn_fill_blk(spos, cell_start, g, width, q, R, Lw, bkey, bpk, cap,
+                                      &s_gather, (int)wid, KB_THREADS / 64);
     if (cnt > cap) {
       if (tid == 0) kcnt[w] = 0xFFFFFFFFu;
       __syncthreads();
