@@ -119,13 +119,29 @@ __global__ void k_grid_scatter(const uint32_t *__restrict__ cellid, size_t n,
   order[pos] = (uint32_t)i;
 }
 
-// make each grid cell's list ascending in generator index (deterministic
-// slot order).  A thread sorts a cell of up to GRID_SMALL seeds by
-// insertion; larger cells (dense clusters put tens of thousands of seeds in
-// one cell) are listed for k_grid_sort_big.
+// The order of a grid cell's seeds: by zkey (the top 32 bits of an
+// order-preserving map of z), then by generator index.  Cells along z are
+// consecutive and seeds fall in cells monotonically in z, so each row of
+// cells along z is then ordered by zkey, and the gather can cut a long row
+// to the ball's chord with two binary searches (the key is truncated, so
+// the cut keeps every seed whose z lies in the chord).  The order is
+// deterministic; nothing else depends on it (lists are sorted by key and
+// generator index after the gather).
+__device__ __forceinline__ uint32_t zkey(double z) {
+  const uint64_t b = (uint64_t)__double_as_longlong(z);
+  return (uint32_t)(((b >> 63) ? ~b : (b | 0x8000000000000000ull)) >> 32);
+}
+__device__ __forceinline__ uint64_t zkey_idx(const double4 *gpos, uint32_t i) {
+  return ((uint64_t)zkey(gpos[i].z) << 32) | i;
+}
+
+// A thread sorts a cell of up to GRID_SMALL seeds by insertion; larger cells
+// (dense clusters put tens of thousands of seeds in one cell) are listed
+// for k_grid_sort_big.
 constexpr uint32_t GRID_SMALL = 32;
-__global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
-                                  size_t ncell, uint32_t *__restrict__ order,
+__global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start, size_t ncell,
+                                  const double4 *__restrict__ gpos,
+                                  uint32_t *__restrict__ order,
                                   uint32_t *__restrict__ big_list,
                                   unsigned int *__restrict__ big_cnt) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,9 +152,10 @@ __global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
     return;
   }
   for (uint32_t i = b + 1; i < e; i++) {
-    uint32_t v = order[i];
+    const uint32_t v = order[i];
+    const uint64_t kv = zkey_idx(gpos, v);
     uint32_t j = i;
-    while (j > b && order[j - 1] > v) {
+    while (j > b && zkey_idx(gpos, order[j - 1]) > kv) {
       order[j] = order[j - 1];
       j--;
     }
@@ -146,20 +163,22 @@ __global__ void k_grid_sort_cells(const uint32_t *__restrict__ cell_start,
   }
 }
 
-// One block per large cell (of the list k_grid_sort_cells made): chunks of
-// GS_CHUNK indices sorted by a bitonic network in LDS, then runs of width
-// w merged pairwise between the cell's range of `order` and the same range
-// of `tmp`.  The indices are distinct, so an index's place in the merged
-// run is its rank in its own run plus the number of smaller indices in the
-// partner run (a binary search), with no merge path to share.
+// One block per large cell (of the list k_grid_sort_cells made): the
+// cell's (zkey, index) words in chunks of GS_CHUNK sorted by a bitonic
+// network in LDS, then runs of width w merged pairwise between two scratch
+// ranges.  The words are distinct, so a word's place in the merged run is
+// its rank in its own run plus the number of smaller words in the partner
+// run (a binary search), with no merge path to share.  The sorted indices
+// go back to `order`.
 constexpr uint32_t GS_CHUNK = 4096;
 constexpr int GS_THREADS = 1024;
 __global__ void __launch_bounds__(GS_THREADS)
     k_grid_sort_big(const uint32_t *__restrict__ cell_start,
                     const uint32_t *__restrict__ big_list,
-                    const unsigned int *__restrict__ big_cnt, uint32_t *__restrict__ order,
-                    uint32_t *__restrict__ tmp) {
-  __shared__ uint32_t s[GS_CHUNK];
+                    const unsigned int *__restrict__ big_cnt,
+                    const double4 *__restrict__ gpos, uint32_t *__restrict__ order,
+                    uint64_t *__restrict__ ta, uint64_t *__restrict__ tb) {
+  __shared__ uint64_t s[GS_CHUNK];
   const uint32_t tid = threadIdx.x;
   const uint32_t nbig = *big_cnt;
   for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
@@ -170,13 +189,13 @@ __global__ void __launch_bounds__(GS_THREADS)
       uint32_t n2 = 1;
       while (n2 < len) n2 <<= 1;
       for (uint32_t i = tid; i < n2; i += GS_THREADS)
-        s[i] = i < len ? order[b + c0 + i] : 0xFFFFFFFFu;
+        s[i] = i < len ? zkey_idx(gpos, order[b + c0 + i]) : ~0ull;
       __syncthreads();
       for (uint32_t kk = 2; kk <= n2; kk <<= 1) {
         for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
           for (uint32_t p = tid; p < (n2 >> 1); p += GS_THREADS) {
             const uint32_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-            const uint32_t x = s[i], y = s[i | j];
+            const uint64_t x = s[i], y = s[i | j];
             if ((x > y) == ((i & kk) == 0)) {
               s[i] = y;
               s[i | j] = x;
@@ -185,16 +204,16 @@ __global__ void __launch_bounds__(GS_THREADS)
           __syncthreads();
         }
       }
-      for (uint32_t i = tid; i < len; i += GS_THREADS) order[b + c0 + i] = s[i];
+      for (uint32_t i = tid; i < len; i += GS_THREADS) ta[b + c0 + i] = s[i];
       __syncthreads();
     }
-    uint32_t *src = order + b, *dst = tmp + b;
+    uint64_t *src = ta + b, *dst = tb + b;
     for (uint32_t w = GS_CHUNK; w < m; w <<= 1) {
       for (uint32_t i = tid; i < m; i += GS_THREADS) {
         const uint32_t r0 = i / (2 * w) * (2 * w);
         const uint32_t a1 = min(r0 + w, m), b1 = min(r0 + 2 * w, m);
         const bool in_a = i < a1;
-        const uint32_t x = src[i];
+        const uint64_t x = src[i];
         uint32_t lo = in_a ? a1 : r0, hi = in_a ? b1 : a1;
         const uint32_t base = lo;
         while (lo < hi) {
@@ -207,13 +226,11 @@ __global__ void __launch_bounds__(GS_THREADS)
         dst[r0 + (i - (in_a ? r0 : a1)) + (lo - base)] = x;
       }
       __syncthreads();
-      uint32_t *t = src;
+      uint64_t *t = src;
       src = dst;
       dst = t;
     }
-    if (src != order + b) {
-      for (uint32_t i = tid; i < m; i += GS_THREADS) order[b + i] = src[i];
-    }
+    for (uint32_t i = tid; i < m; i += GS_THREADS) order[b + i] = (uint32_t)src[i];
     __syncthreads();
   }
 }
@@ -443,6 +460,30 @@ __device__ __forceinline__ uint32_t knn_gather(
                 const size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
                 t0 = cell_start[row + c0];
                 len = cell_start[row + c1 + 1] - t0;
+                if (len > 32u && g.inv_h[2] != 0.) {
+                  // a long row (dense cells): its seeds are in zkey order;
+                  // keep those whose z can lie in the chord
+                  const double hc = sqrt(rr2 - d2);
+                  const uint32_t klo = zkey(qs[2] - hc), khi = zkey(qs[2] + hc);
+                  uint32_t lo = t0, hi = t0 + len;
+                  while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (zkey(spos[mid].z) < klo)
+                      lo = mid + 1;
+                    else
+                      hi = mid;
+                  }
+                  uint32_t lo2 = lo, hi2 = t0 + len;
+                  while (lo2 < hi2) {
+                    const uint32_t mid = (lo2 + hi2) >> 1;
+                    if (zkey(spos[mid].z) <= khi)
+                      lo2 = mid + 1;
+                    else
+                      hi2 = mid;
+                  }
+                  t0 = lo;
+                  len = lo2 - lo;
+                }
               }
             }
           }

@@ -582,18 +582,21 @@ void build_grid(GridData &D, const double *d_xyz, size_t n,
   hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, s, cellid, n, D.cell_start, fill,
                      D.order);
   {
-    // after the scatter, `fill` holds the list of large cells, `counts`
-    // their count and `cellid` the merge buffer
+    // after the scatter, `fill` holds the list of large cells and `counts`
+    // their count; spos, written only by the gather below, serves as the
+    // two sort buffers (8 of its 32 bytes per seed each)
     uint32_t *big_list = fill;
     unsigned int *big_cnt = counts;
-    uint32_t *tmp = cellid;
+    uint64_t *ta = reinterpret_cast<uint64_t *>(D.spos);
+    uint64_t *tb = ta + n;
     HIPCHK(hipMemsetAsync(big_cnt, 0, sizeof(unsigned int), s));
     hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(ncell)), dim3(256), 0, s, D.cell_start,
-                       ncell, D.order, big_list, big_cnt);
+                       ncell, (const double4 *)D.gpos, D.order, big_list, big_cnt);
     // the cells over GRID_SMALL seeds (their count stays on the device)
     hipLaunchKernelGGL(k_grid_sort_big, dim3(1024), dim3(GS_THREADS), 0, s,
                        (const uint32_t *)D.cell_start, (const uint32_t *)big_list,
-                       (const unsigned int *)big_cnt, D.order, tmp);
+                       (const unsigned int *)big_cnt, (const double4 *)D.gpos, D.order, ta,
+                       tb);
   }
   hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos,
                      (const uint32_t *)D.order, n, (const uint32_t *)D.prob, D.spos, D.slot_of,
