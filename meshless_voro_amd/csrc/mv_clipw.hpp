@@ -322,6 +322,25 @@ k_clip_wave(WaveArgs A) {
       lds_fence();
       const int nb = (int)min(64u, cnt - j0);
       int t = 0;
+      if (j0 > 128) {
+        // a long list past its first 128 entries rarely clips: lane l tests
+        // entry l of the batch against every vertex (broadcast from its
+        // lane) and the scan below starts at the first entry that is bad,
+        // stops the cell (sr < dist, sr the largest vertex term) or touches
+        // it; the entries before it change nothing
+        const bool act = lane < nv;
+        const double srm = wave_max_nonneg(act ? vs : 0.);
+        const bool in = lane < nb;
+        const double4 e = S.u.e.nd[in ? lane : 0];
+        const double2 de = S.u.e.de[in ? lane : 0];
+        bool hit = lane == tbad || srm < de.x;
+        for (int v = 0; v < nv; v++) {
+          const double c = dot3(xyz(e), mk3(rl_d(vx, v), rl_d(vy, v), rl_d(vz, v))) - e.w;
+          hit |= (fabs(c) < de.y) | (c < 0.);
+        }
+        const uint64_t hm = __ballot(in && hit);
+        t = hm ? (int)__builtin_ctzll(hm) : nb;
+      }
       while (t < nb) {
         // scan the entries that leave the cell unchanged (no stop, no
         // vertex near or inside their plane).  Unrolled by two with the
