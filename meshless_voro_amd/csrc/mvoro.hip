@@ -164,21 +164,6 @@ TOut scan_excl(const TIn *in, TOut *out, size_t n, hipStream_t s) {
   return (TOut)*(volatile unsigned long long *)hw;
 }
 
-// compaction of a work list by per-item state
-uint32_t compact(const uint32_t *work, const uint8_t *st, uint32_t n,
-                 uint8_t want, uint32_t *out, hipStream_t s) {
-  if (n == 0) return 0;
-  Temps T;
-  uint32_t *flag = need(T.get<uint32_t>(n));
-  uint32_t *pos = need(T.get<uint32_t>(n));
-  hipLaunchKernelGGL(k_flag_state, dim3(nblk(n)), dim3(256), 0, s, st, n, want, flag);
-  uint32_t cnt = scan_excl<uint32_t, uint32_t>(flag, pos, n, s);
-  if (cnt)
-    hipLaunchKernelGGL(k_compact, dim3(nblk(n)), dim3(256), 0, s, work, flag, pos, n, out);
-  HIPCHK(hipGetLastError());
-  return cnt;
-}
-
 struct Timer {
   hipEvent_t a, b;
   hipStream_t s;
