@@ -226,7 +226,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(a multi-GPU build would shard the seeds by slabs with a halo; not built)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

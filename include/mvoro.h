@@ -4,8 +4,8 @@
  *
  * Drop-in boundary for the construction path of the meshless_voronoi crate
  * (v0.7.2).  Every entry point names the reference item it replaces.  The
- * reference has no FFI of its own (it is a Rust crate); the binding stub a
- * Rust caller would add is shown in INTEGRATION.md.
+ * reference has no FFI of its own (it is a Rust crate); a Rust caller binds
+ * these functions with an extern "C" block over the same signatures.
  *
  * Conventions
  *  - positions are f64 AoS [x, y, z] with a 24-byte stride, which is the

@@ -10,8 +10,8 @@ CONNECTIVITY = ("face_left", "face_right", "face_has_shift", "links", "cell_face
 INTEGRALS = ("cell_volume", "cell_centroid", "cell_safety_radius", "face_area",
              "face_centroid", "face_normal", "face_shift", "cell_loc")
 
-# tolerance of the north-star contract for FP64 integrals: relative 1e-12,
-# scaled for near-zero values by the cell size (h^3 volumes, h^2 areas)
+# tolerance for FP64 integrals: relative 1e-12, scaled for near-zero values
+# by the cell size (h^3 volumes, h^2 areas)
 REL_TOL = 1e-12
 
 
