@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--batched", action="store_true",
                     help="also measure many problems in one build (512 problems, ~10.6 M "
                          "seeds; the clustered problems take tens of seconds)")
+    ap.add_argument("--clustered", action="store_true",
+                    help="also time 4,194,304 clustered periodic seeds (80 %% in 64 Gaussian "
+                         "clusters, sigma log-uniform in [1e-3, 1e-1]) in both modes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (elementwise kernel) so that "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE can be scaled")
@@ -165,6 +168,43 @@ def traffic_from_profile(kernel):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
     return float(m.group(1)) if m else None
+
+
+def clustered_4m(L, modes, reps=2):
+    """4,194,304 clustered periodic seeds: device time per full build (HIP
+    events, after one warm-up build) in each mode, and whether both modes
+    give the same connectivity."""
+    import numpy as np
+    import torch
+    from meshless_voro_amd import workloads as W
+    x = W.clustered_4m()
+    n = len(x)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    res, dig = {}, {}
+    for name, mode in modes:
+        print(f"bench: clustered 4M, mode {name}", file=sys.stderr, flush=True)
+        h, _ = build_device(L, xt, n, mode)
+        dig[name] = connectivity_digest(L, h)
+        L.mv_free(h)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            h, st = build_device(L, xt, n, mode)
+            L.mv_free(h)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms_per_build": ms, "cells_per_s": n / (ms / 1e3),
+                     "passes": st["passes"], "big_cells": st["big_cells"],
+                     "k_last": st["k_last"]}
+    if "learned" in dig and "plain" in dig:
+        res["same_topology"] = all(np.array_equal(p, q)
+                                   for p, q in zip(dig["plain"], dig["learned"]))
+    res["workload"] = ("clustered_4m: 4,194,304 periodic seeds, 80 % in 64 wrapped Gaussian "
+                       "clusters (sigma log-uniform in [1e-3, 1e-1]), 20 % uniform")
+    return res
 
 
 def build_device(L, xt, n, mode, periodic=True):
@@ -352,6 +392,8 @@ def main():
         out["batched_c5"] = batched_c5(L, bm, reps=1)
         out["batched_c5"]["mode"] = out["batched_c5_uniform"]["mode"] = (
             "learned" if bm == MODE_LEARNED else "plain")
+    if args.clustered:
+        out["clustered_4m"] = clustered_4m(L, list(modes.items()))
     if args.calibrate_copy:
         calibrate_copy()
     if not args.no_cpu_baseline:

@@ -466,8 +466,12 @@ __global__ void k_proposal(const float *__restrict__ Y,
   L = (L + 7u) & ~7u;  // 8-wide buckets
   L = L < lmin ? lmin : (L > lmax ? lmax : L);
   Lper[i] = L;
-  // 0 makes the gather start from the grid density
-  Rper[i] = expf(fminf(Y[(size_t)i * NO + 1], 8.f)) * hloc[i] * rscale;
+  // the proposed radius, kept within a factor 3 of the radius of a ball of
+  // about 1.3 L + 8 seeds at the block density (inside dense clusters the
+  // proposal alone misses more often, and a gather that misses is redone)
+  const float Rm = expf(fminf(Y[(size_t)i * NO + 1], 8.f)) * hloc[i] * rscale;
+  const float Rd = hloc[i] * cbrtf(0.2387324f * (1.3f * (float)L + 8.f));
+  Rper[i] = fminf(fmaxf(Rm, 0.333f * Rd), 3.f * Rd);
 }
 
 }  // namespace mv
