@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--clustered", action="store_true",
                     help="also time 4,194,304 clustered periodic seeds (80 %% in 64 Gaussian "
                          "clusters, sigma log-uniform in [1e-3, 1e-1]) in both modes")
+    ap.add_argument("--host", action="store_true",
+                    help="also time mv_build with host seeds in and host results out "
+                         "(upload, build, download of every output array)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (elementwise kernel) so that "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE can be scaled")
@@ -204,6 +207,46 @@ def clustered_4m(L, modes, reps=2):
                                    for p, q in zip(dig["plain"], dig["learned"]))
     res["workload"] = ("clustered_4m: 4,194,304 periodic seeds, 80 % in 64 wrapped Gaussian "
                        "clusters (sigma log-uniform in [1e-3, 1e-1]), 20 % uniform")
+    return res
+
+
+def host_to_host(L, x, modes, reps=3):
+    """mv_build from host seeds to host result arrays (the PCIe-inclusive
+    rate): wall-clock median of reps builds after one warm-up build, and the
+    bytes each build moves over the bus."""
+    import numpy as np
+    from meshless_voro_amd import _native
+    from meshless_voro_amd.voronoi import VoronoiError
+    from meshless_voro_amd.autotune import tuned_options
+    n = len(x)
+    xc = np.ascontiguousarray(x, dtype=np.float64)
+    a, w = np.zeros(3), np.ones(3)
+    res = {}
+    for name, mode in modes:
+        print(f"bench: host to host, mode {name}", file=sys.stderr, flush=True)
+        opt = tuned_options(n, True, mode).to_c()
+        ts = []
+        for r in range(reps + 1):
+            h = C.c_void_p()
+            t0 = time.perf_counter()
+            st = L.mv_build(xc.ctypes.data, n, None, a.ctypes.data, w.ctypes.data, 3, 1,
+                            C.byref(opt), C.byref(h))
+            dt = time.perf_counter() - t0
+            if st != 0:
+                raise VoronoiError(st)
+            if r:
+                ts.append(dt)
+            if r == reps:
+                F, NL = L.mv_n_faces(h), L.mv_n_links(h)
+            L.mv_free(h)
+        ms = 1e3 * float(np.median(ts))
+        # cells: loc, centroid (3 f64 each), volume, safety radius (f64), face
+        # offset, count and index (u64), entries used (u32); faces: left (u64),
+        # right (i64), area (f64), centroid, normal, shift (3 f64 each),
+        # has_shift (u8); links (u64)
+        down = n * (24 + 24 + 8 + 8 + 24 + 4) + F * (8 + 8 + 8 + 72 + 1) + NL * 8
+        res[name] = {"ms_per_build": ms, "cells_per_s": n / (ms / 1e3),
+                     "upload_bytes": 24 * n, "download_bytes": down}
     return res
 
 
@@ -394,6 +437,8 @@ def main():
             "learned" if bm == MODE_LEARNED else "plain")
     if args.clustered:
         out["clustered_4m"] = clustered_4m(L, list(modes.items()))
+    if args.host:
+        out["host_to_host"] = host_to_host(L, x, list(modes.items()))
     if args.calibrate_copy:
         calibrate_copy()
     if not args.no_cpu_baseline:

@@ -15,7 +15,7 @@ _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libmvoro.so")
 SOURCES = [os.path.join(_PKG, "csrc", f) for f in (
     "mvoro.hip", "mv_common.hpp", "mv_geom.hpp", "mv_grid.hpp", "mv_clip.hpp",
-    "mv_clipw.hpp", "mv_finalize.hpp", "mv_learn.hpp")] + [
+    "mv_clipw.hpp", "mv_faces.hpp", "mv_finalize.hpp", "mv_learn.hpp")] + [
     os.path.join(_ROOT, "include", f) for f in ("mvoro.h", "mvoro_integrals.hpp")]
 
 HIPCC_FLAGS = [
@@ -127,6 +127,11 @@ def lib():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
             "(hipcc, gfx950); there is no CPU fallback")
+    # torch's HIP runtime first: its libraries ask for "libamdhip64.so" and
+    # would load a second runtime beside one this library brought in (under
+    # its versioned name), and device pointers of one are unknown to the
+    # other.  Loaded after torch, the library binds to torch's runtime.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp, sz, u32 = C.c_void_p, C.c_size_t, C.c_uint32
     L.mv_default_options.argtypes = [C.POINTER(mv_options)]

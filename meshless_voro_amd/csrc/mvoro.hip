@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -215,6 +217,29 @@ struct DevStore {
   }
 };
 
+// std::vector storage whose resize leaves the new elements uninitialised
+// (the host copies are overwritten by the download at once).
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <typename U>
+  void construct(U *p) noexcept {
+    ::new ((void *)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <typename T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
+
 struct mv_tess {
   size_t n = 0, nf = 0, nl = 0;
   double anchor[3], width[3];
@@ -250,14 +275,14 @@ struct mv_tess {
   // host copies
   std::mutex mu;
   bool have_host = false;
-  std::vector<double> h_cell_loc, h_cell_cen, h_cell_vol, h_cell_sr;
-  std::vector<uint64_t> h_cell_foff, h_cell_fcnt, h_cell_idx;
-  std::vector<uint32_t> h_npop;
-  std::vector<uint64_t> h_face_left;
-  std::vector<int64_t> h_face_right;
-  std::vector<double> h_face_area, h_face_cen, h_face_norm, h_face_shift;
-  std::vector<uint8_t> h_face_has_shift;
-  std::vector<uint64_t> h_links;
+  hvec<double> h_cell_loc, h_cell_cen, h_cell_vol, h_cell_sr;
+  hvec<uint64_t> h_cell_foff, h_cell_fcnt, h_cell_idx;
+  hvec<uint32_t> h_npop;
+  hvec<uint64_t> h_face_left;
+  hvec<int64_t> h_face_right;
+  hvec<double> h_face_area, h_face_cen, h_face_norm, h_face_shift;
+  hvec<uint8_t> h_face_has_shift;
+  hvec<uint64_t> h_links;
   mv_stats stats{};
 
   template <typename T>
@@ -278,32 +303,157 @@ struct mv_tess {
 
 namespace {
 
+// Device -> host copy of a handle's arrays.  Into pageable memory the
+// runtime stages every byte through its own pinned buffer on the calling
+// thread; here the bytes are split over up to DL_THREADS threads, each
+// with 2 x DL_HALF of pinned staging and a stream of its own: the DMA into
+// one half overlaps the host copy out of the other, and the destination
+// pages are first touched by many cores at once.
+struct Piece {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+constexpr int DL_THREADS = 8;
+constexpr size_t DL_HALF = size_t(8) << 20;
+constexpr size_t DL_MIN_THREADED = size_t(32) << 20;  // below: one hipMemcpy per array
+
+struct DlStaging {
+  uint8_t *buf[DL_THREADS] = {};
+  hipStream_t st[DL_THREADS] = {};
+  hipEvent_t ev[DL_THREADS][2] = {};
+};
+
+// One download at a time uses a device's staging (held under dl_mu).
+std::mutex dl_mu;
+
+// The staging of device dev, made on its first download and kept until the
+// process ends (16 MiB of pinned host memory per thread); nullptr if it
+// cannot be made.  Called with dl_mu held.
+DlStaging *dl_staging(int dev) {
+  static std::map<int, DlStaging *> made;
+  auto it = made.find(dev);
+  if (it != made.end()) return it->second;
+  DlStaging *S = new DlStaging;
+  bool ok = hipSetDevice(dev) == hipSuccess;
+  for (int w = 0; ok && w < DL_THREADS; w++) {
+    ok = hipHostMalloc((void **)&S->buf[w], 2 * DL_HALF, hipHostMallocDefault) == hipSuccess &&
+         hipStreamCreateWithFlags(&S->st[w], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&S->ev[w][0], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&S->ev[w][1], hipEventDisableTiming) == hipSuccess;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    S = nullptr;  // what was made stays allocated; the plain copies are used
+  }
+  made[dev] = S;
+  return S;
+}
+
+// Worker w: bytes [lo, hi) of the pieces' concatenation, in chunks of at
+// most DL_HALF that do not cross a piece.  Returns with its stream idle.
+void dl_share(DlStaging *S, int w, int dev, const std::vector<Piece> *P, size_t lo,
+              size_t hi, std::atomic<int> *bad) {
+  if (hipSetDevice(dev) != hipSuccess) {
+    bad->store(1);
+    return;
+  }
+  uint8_t *pend_dst = nullptr;
+  size_t pend_n = 0;
+  int h = 0;
+  bool ok = true;
+  size_t base = 0;
+  for (const Piece &p : *P) {
+    const size_t a = std::max(lo, base), b = std::min(hi, base + p.bytes);
+    for (size_t o = a; ok && o < b;) {
+      const size_t c = std::min(DL_HALF, b - o);
+      uint8_t *stg = S->buf[w] + h * DL_HALF;
+      ok = hipMemcpyAsync(stg, (const uint8_t *)p.src + (o - base), c, hipMemcpyDeviceToHost,
+                          S->st[w]) == hipSuccess &&
+           hipEventRecord(S->ev[w][h], S->st[w]) == hipSuccess;
+      if (ok && pend_n) {
+        ok = hipEventSynchronize(S->ev[w][h ^ 1]) == hipSuccess;
+        if (ok) memcpy(pend_dst, S->buf[w] + (h ^ 1) * DL_HALF, pend_n);
+      }
+      pend_dst = (uint8_t *)p.dst + (o - base);
+      pend_n = c;
+      h ^= 1;
+      o += c;
+    }
+    base += p.bytes;
+  }
+  if (ok && pend_n) {
+    ok = hipEventSynchronize(S->ev[w][h ^ 1]) == hipSuccess;
+    if (ok) memcpy(pend_dst, S->buf[w] + (h ^ 1) * DL_HALF, pend_n);
+  }
+  if (!ok) {
+    (void)hipStreamSynchronize(S->st[w]);
+    (void)hipGetLastError();
+    bad->store(1);
+  }
+}
+
+void copy_to_host(const std::vector<Piece> &P) {
+  size_t total = 0;
+  for (const Piece &p : P) total += p.bytes;
+  DlStaging *S = nullptr;
+  int dev = 0;
+  std::unique_lock<std::mutex> lk(dl_mu, std::defer_lock);
+  if (total >= DL_MIN_THREADED) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, P[0].src) == hipSuccess) {
+      dev = at.device;
+      lk.lock();
+      S = dl_staging(dev);
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  if (!S) {
+    for (const Piece &p : P) HIPCHK(hipMemcpy(p.dst, p.src, p.bytes, hipMemcpyDeviceToHost));
+    return;
+  }
+  const int T = (int)std::min<size_t>(DL_THREADS, (total + 4 * DL_HALF - 1) / (4 * DL_HALF));
+  const size_t share = ((total + T - 1) / T + 4095) & ~size_t(4095);
+  std::atomic<int> bad{0};
+  std::vector<std::thread> th;
+  for (int w = 0; w < T; w++) {
+    const size_t lo = std::min(total, w * share), hi = std::min(total, lo + share);
+    th.emplace_back(dl_share, S, w, dev, &P, lo, hi, &bad);
+  }
+  for (auto &x : th) x.join();
+  if (bad.load()) throw BuildError{MV_E_DEVICE};
+}
+
 template <typename T>
-void dl(std::vector<T> &h, const T *d, size_t n) {
+void dl(std::vector<Piece> &P, hvec<T> &h, const T *d, size_t n) {
   h.resize(n);
-  if (n) HIPCHK(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+  if (n) P.push_back({h.data(), d, n * sizeof(T)});
 }
 
 void download(mv_tess *t) {
   std::lock_guard<std::mutex> g(t->mu);
   if (t->have_host) return;
   size_t n = t->n, F = t->nf, L = t->nl;
-  dl(t->h_cell_loc, t->d_cell_loc, 3 * n);
-  dl(t->h_cell_cen, t->d_cell_cen, 3 * n);
-  dl(t->h_cell_vol, t->d_cell_vol, n);
-  dl(t->h_cell_sr, t->d_cell_sr, n);
-  dl(t->h_cell_foff, t->d_cell_foff, n);
-  dl(t->h_cell_fcnt, t->d_cell_fcnt, n);
-  dl(t->h_cell_idx, t->d_cell_idx, n);
-  dl(t->h_npop, t->d_npop, n);
-  dl(t->h_face_left, t->d_face_left, F);
-  dl(t->h_face_right, t->d_face_right, F);
-  dl(t->h_face_area, t->d_face_area, F);
-  dl(t->h_face_cen, t->d_face_cen, 3 * F);
-  dl(t->h_face_norm, t->d_face_norm, 3 * F);
-  dl(t->h_face_shift, t->d_face_shift, 3 * F);
-  dl(t->h_face_has_shift, t->d_face_has_shift, F);
-  dl(t->h_links, t->d_links, L);
+  std::vector<Piece> P;
+  dl(P, t->h_cell_loc, t->d_cell_loc, 3 * n);
+  dl(P, t->h_cell_cen, t->d_cell_cen, 3 * n);
+  dl(P, t->h_cell_vol, t->d_cell_vol, n);
+  dl(P, t->h_cell_sr, t->d_cell_sr, n);
+  dl(P, t->h_cell_foff, t->d_cell_foff, n);
+  dl(P, t->h_cell_fcnt, t->d_cell_fcnt, n);
+  dl(P, t->h_cell_idx, t->d_cell_idx, n);
+  dl(P, t->h_npop, t->d_npop, n);
+  dl(P, t->h_face_left, t->d_face_left, F);
+  dl(P, t->h_face_right, t->d_face_right, F);
+  dl(P, t->h_face_area, t->d_face_area, F);
+  dl(P, t->h_face_cen, t->d_face_cen, 3 * F);
+  dl(P, t->h_face_norm, t->d_face_norm, 3 * F);
+  dl(P, t->h_face_shift, t->d_face_shift, 3 * F);
+  dl(P, t->h_face_has_shift, t->d_face_has_shift, F);
+  dl(P, t->h_links, t->d_links, L);
+  if (!P.empty()) copy_to_host(P);
   t->have_host = true;
 }
 
