@@ -9,7 +9,10 @@
 // test still decides every cell, and a cell whose list was too short is
 // rebuilt with a doubled list, so the output does not depend on the model.
 #pragma once
+#include <type_traits>
+
 #include "mv_common.hpp"
+#include "mv_grid.hpp"  // zkey: the order of a grid row
 
 namespace mv {
 
@@ -20,6 +23,7 @@ constexpr int NF = 16;   // features
 constexpr int NH = 64;   // hidden width
 constexpr int NO = 2;    // outputs
 constexpr int NOP = 16;  // outputs padded to one MFMA tile
+constexpr uint32_t FEAT_WIN = 64;  // own-cell window that bounds the 8th distance
 
 __device__ __forceinline__ short f2bf(float x) {
   uint32_t b = __float_as_uint(x);
@@ -51,6 +55,13 @@ struct FeatParams {
 //  14   1 if periodic
 //  15   spread of the 8 distances (max - min) / mean
 // hloc[w] = local spacing (block volume / block count)^(1/nd).
+//
+// Two instantiations share the work: DENSE = true takes the waves with a
+// seed whose grid cell holds more than FEAT_WIN seeds (dense clusters),
+// DENSE = false every other wave; both evaluate the same wave-uniform test
+// and return at once on the other's waves (one launch each: in one kernel
+// the dense path's registers would cost the common path its occupancy).
+template <bool DENSE>
 __global__ void __launch_bounds__(256)
     k_features(const double4 *__restrict__ spos,
                const uint32_t *__restrict__ cell_start_all, Problems PR,
@@ -86,6 +97,159 @@ __global__ void __launch_bounds__(256)
     boff[i][0] = boff[i][1] = boff[i][2] = 0.;
   }
   uint32_t n1 = 0, n27 = 0, n125 = 0;
+  // Dense cells (clusters put thousands of seeds in one grid cell): a bound
+  // B on the final 8th squared distance cuts the scan of each row to the z
+  // range |dz| < sqrt(B) (rows are in zkey order).  Only seeds whose squared
+  // distance exceeds the final 8th are skipped, in the scan's own order, so
+  // the result is that of the full scan bit for bit.  B is the 8th smallest
+  // squared distance to FEAT_WIN seeds of the own cell next to it in z
+  // order, then tightened to the 8th smallest over the own cell's seeds
+  // within that first range (both are distances the scan itself computes).
+  double zr = INFINITY;
+  uint32_t s0 = 0, s1 = 0;
+  if (g.inv_h[2] != 0.) {
+    const size_t orow = ((size_t)C[0] * g.G[1] + C[1]) * g.G[2] + C[2];
+    s0 = cell_start[orow];
+    s1 = cell_start[orow + 1];
+  }
+  const bool dense_lane = g.inv_h[2] != 0. && s1 - s0 > FEAT_WIN;
+  if (__any(dense_lane) != DENSE) return;
+  // A one-cell wave (its 64 lanes share one grid cell of one problem) scans
+  // the same rows on every lane; the union of their z ranges keeps every
+  // lane on the same entry at each step (one coalesced load per 64 entries,
+  // each broadcast with readlane).  Widening a lane's range only adds
+  // entries it would have tested anyway.
+  bool uni = false;
+  if constexpr (DENSE) {
+    const bool full = __popcll(__ballot(1)) == 64;
+    const uint32_t fp0 = __builtin_amdgcn_readfirstlane(pr);
+    const int f0 = __builtin_amdgcn_readfirstlane(C[0]);
+    const int f1 = __builtin_amdgcn_readfirstlane(C[1]);
+    const int f2 = __builtin_amdgcn_readfirstlane(C[2]);
+    uni = __all(full && pr == fp0 && C[0] == f0 && C[1] == f1 && C[2] == f2);
+  }
+  double zlo = -INFINITY, zhi = INFINITY;
+  auto widen = [&]() {
+    zlo = q[2] - zr;
+    zhi = q[2] + zr;
+    if (uni)
+      for (int o = 32; o > 0; o >>= 1) {
+        zlo = fmin(zlo, __shfl_xor(zlo, o, 64));
+        zhi = fmax(zhi, __shfl_xor(zhi, o, 64));
+      }
+  };
+  // [t0, t1) of a zkey-ordered range cut to z in [zlo, zhi] (unshifted z)
+  auto zcut = [&](uint32_t &t0, uint32_t &t1) {
+    if (!(zr < INFINITY) || t1 - t0 <= 32u) return;
+    const uint32_t klo = zkey(zlo), khi = zkey(zhi);
+    uint32_t lo = t0, hi = t1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (zkey(spos[mid].z) < klo)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    uint32_t lo2 = lo, hi2 = t1;
+    while (lo2 < hi2) {
+      const uint32_t mid = (lo2 + hi2) >> 1;
+      if (zkey(spos[mid].z) <= khi)
+        lo2 = mid + 1;
+      else
+        hi2 = mid;
+    }
+    t0 = lo;
+    t1 = lo2;
+  };
+  // fn(p, t) for the entries t of [t0, t1) in index order: in a one-cell
+  // wave (same range on every lane) 64 entries per coalesced load, each
+  // broadcast with readlane; otherwise one load per lane and entry
+  auto each = [&](uint32_t t0, uint32_t t1, auto &&fn) {
+    if (DENSE && uni) {
+      const uint32_t lane = __lane_id();
+      for (uint32_t t = t0; t < t1; t += 64) {
+        const uint32_t m = min(64u, t1 - t);
+        double4 mine = make_double4(0., 0., 0., 0.);
+        if (lane < m) mine = spos[t + lane];
+        const uint64_t bx = (uint64_t)__double_as_longlong(mine.x);
+        const uint64_t by = (uint64_t)__double_as_longlong(mine.y);
+        const uint64_t bz = (uint64_t)__double_as_longlong(mine.z);
+        for (uint32_t k = 0; k < m; k++) {
+          auto rd = [&](uint64_t b) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
+            return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+          };
+          fn(make_double4(rd(bx), rd(by), rd(bz), 0.), t + k);
+        }
+      }
+    } else {
+      for (uint32_t t = t0; t < t1; t++) fn(spos[t], t);
+    }
+  };
+  if (DENSE && dense_lane) {
+    double b8[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) b8[i] = INFINITY;
+    auto keep8 = [&](const double4 p, uint32_t t) {
+      if (t == slot) return;
+      const double o[3] = {p.x - q[0], p.y - q[1], p.z - q[2]};
+      double v = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
+      if (!(v < b8[7])) return;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        if (v < b8[i]) {
+          const double tv = b8[i];
+          b8[i] = v;
+          v = tv;
+        }
+      }
+    };
+    uint32_t a0 = slot >= s0 + FEAT_WIN / 2 ? slot - FEAT_WIN / 2 : s0;
+    if (a0 + FEAT_WIN > s1) a0 = s1 - FEAT_WIN;
+    for (uint32_t t = a0; t < a0 + FEAT_WIN; t++) keep8(spos[t], t);
+    // |z difference| >= zr gives a squared distance > b8[7] after rounding
+    zr = sqrt(b8[7]) * (1. + 1e-6);
+    widen();
+    // the range holds the window's 8 nearest: start afresh so that none of
+    // them is counted twice
+#pragma unroll
+    for (int i = 0; i < 8; i++) b8[i] = INFINITY;
+    uint32_t u0 = s0, u1 = s1;
+    zcut(u0, u1);
+    each(u0, u1, keep8);
+    zr = sqrt(b8[7]) * (1. + 1e-6);
+    widen();
+  }
+  // entries [t0, t1) with image shift (sx, sy, sz) into the sorted 8
+  // (ZS = false: no shift along z)
+  auto take = [&](const double4 p, double sx, double sy, double sz, auto zs) {
+    const double o[3] = {p.x + sx - q[0], p.y + sy - q[1],
+                         decltype(zs)::value ? p.z + sz - q[2] : p.z - q[2]};
+    const double d2 = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
+    if (!(d2 < best[7])) return;
+    // insertion into the sorted 8 (unrolled, registers only)
+    double v = d2;
+    double vo[3] = {o[0], o[1], o[2]};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      if (v < best[i]) {
+        double tv = best[i];
+        best[i] = v;
+        v = tv;
+        for (int c = 0; c < 3; c++) {
+          double to = boff[i][c];
+          boff[i][c] = vo[c];
+          vo[c] = to;
+        }
+      }
+    }
+  };
+  auto scan = [&](uint32_t t0, uint32_t t1, double sx, double sy, double sz, auto zs) {
+    each(t0, t1, [&](const double4 p, uint32_t t) {
+      if (t != slot) take(p, sx, sy, sz, zs);
+    });
+  };
   // cell index along axis c of offset dd from C, with the periodic wrap of
   // 5+ cell axes (shift into *sh); false when the cell does not exist
   auto axis_cell = [&](int c, int dd, int *cc, double *sh) -> bool {
@@ -131,28 +295,9 @@ __global__ void __launch_bounds__(256)
         if (da == 0 && db == 0)
           n1 += cell_start[row + C[2] + 1] - cell_start[row + C[2]] - 1u;
         // the three inner cells in order: one range, no shift along c
-        for (uint32_t t = e1; t < e4; t++) {
-          if (t == slot) continue;
-          const double4 p = spos[t];
-          const double o[3] = {p.x + sa - q[0], p.y + sb - q[1], p.z - q[2]};
-          double d2 = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
-          if (!(d2 < best[7])) continue;
-          double v = d2;
-          double vo[3] = {o[0], o[1], o[2]};
-#pragma unroll
-          for (int i = 0; i < 8; i++) {
-            if (v < best[i]) {
-              double tv = best[i];
-              best[i] = v;
-              v = tv;
-              for (int c = 0; c < 3; c++) {
-                double to = boff[i][c];
-                boff[i][c] = vo[c];
-                vo[c] = to;
-              }
-            }
-          }
-        }
+        uint32_t t0 = e1, t1 = e4;
+        zcut(t0, t1);
+        scan(t0, t1, sa, sb, 0., std::false_type());
         continue;
       }
       for (int dc = -2; dc <= 2; dc++) {
@@ -166,29 +311,9 @@ __global__ void __launch_bounds__(256)
         if (!inner_row || abs(dc) > 1) continue;
         n27 += cntc;
         if (center) n1 += cntc;
-        for (uint32_t t = t0; t < t1; t++) {
-          if (t == slot) continue;
-          const double4 p = spos[t];
-          const double o[3] = {p.x + sa - q[0], p.y + sb - q[1], p.z + sc - q[2]};
-          double d2 = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
-          if (!(d2 < best[7])) continue;
-          // insertion into the sorted 8 (unrolled, registers only)
-          double v = d2;
-          double vo[3] = {o[0], o[1], o[2]};
-#pragma unroll
-          for (int i = 0; i < 8; i++) {
-            if (v < best[i]) {
-              double tv = best[i];
-              best[i] = v;
-              v = tv;
-              for (int c = 0; c < 3; c++) {
-                double to = boff[i][c];
-                boff[i][c] = vo[c];
-                vo[c] = to;
-              }
-            }
-          }
-        }
+        uint32_t u0 = t0, u1 = t1;
+        if (sc == 0.) zcut(u0, u1);
+        scan(u0, u1, sa, sb, sc, std::true_type());
       }
     }
   }

@@ -895,8 +895,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     const std::vector<FeatParams> fph = feat_params(probs, (int)dim);
     FeatParams *fpd = need(TT.get<FeatParams>(P));
     HIPCHK(hipMemcpyAsync(fpd, fph.data(), P * sizeof(FeatParams), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_features, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start, PR,
-                       (const FeatParams *)fpd, lists[0], nwork, F, hl);
+    // waves with a seed in a dense grid cell, then every other wave
+    hipLaunchKernelGGL(k_features<true>, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start,
+                       PR, (const FeatParams *)fpd, lists[0], nwork, F, hl);
+    hipLaunchKernelGGL(k_features<false>, dim3(nblk(nwork)), dim3(256), 0, s, spos, cell_start,
+                       PR, (const FeatParams *)fpd, lists[0], nwork, F, hl);
     const uint32_t tiles = (nwork + 15) / 16;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
                        0, s, dm, F, nwork, Y);
@@ -2261,8 +2264,10 @@ mv_status mv_model_features(const double *xyz, size_t n, const double anchor[3],
     float *hl = need(TT.get<float>(n));
     float *Fo = need(TT.get<float>(n * NF));
     float *ho = need(TT.get<float>(n));
-    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start, D.PR(),
-                       (const FeatParams *)fpd, ident, (uint32_t)n, F, hl);
+    hipLaunchKernelGGL(k_features<true>, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start,
+                       D.PR(), (const FeatParams *)fpd, ident, (uint32_t)n, F, hl);
+    hipLaunchKernelGGL(k_features<false>, dim3(nblk(n)), dim3(256), 0, s, D.spos, D.cell_start,
+                       D.PR(), (const FeatParams *)fpd, ident, (uint32_t)n, F, hl);
     hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, NF, F, Fo);
     hipLaunchKernelGGL(k_unpermute, dim3(nblk(n)), dim3(256), 0, s, D.order, n, 1, hl, ho);
     HIPCHK(hipGetLastError());

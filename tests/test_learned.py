@@ -150,6 +150,33 @@ def test_features_permutation_invariant():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("periodic", [True, False])
+def test_features_dense_clusters_find_the_nearest(periodic):
+    """Clusters of 10,000 seeds with sigma 2e-3 put thousands of seeds in one
+    grid cell, where the scan of each row is cut to a z range bounded from
+    the own cell (two clusters sit at the periodic z wrap): the 8 distances
+    equal the 8 nearest found by a k-d tree over all seeds, and the features
+    do not depend on the input order."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(17 + periodic)
+    centres = np.array([[0.5, 0.5, 0.5], [0.3, 0.7, 0.004], [0.8, 0.2, 0.997],
+                        [0.1, 0.1, 0.5]])
+    x = np.concatenate([rng.random((20000, 3))] +
+                       [c + 2e-3 * rng.standard_normal((10000, 3)) for c in centres])
+    x = x % 1.0 if periodic else np.clip(x, 0.0, 1.0 - 1e-12)
+    F, h = learned.features(x, ZERO, ONE, 3, periodic)
+    d, _ = cKDTree(x, boxsize=1.0 if periodic else None).query(x, k=9)
+    d = d[:, 1:]
+    dense = d[:, 7] < 2e-3
+    assert dense.sum() > 35000
+    got = F[dense, :8].astype(np.float64) * h[dense, None].astype(np.float64)
+    assert np.allclose(got, d[dense], rtol=2e-6, atol=0)
+    p = rng.permutation(len(x))
+    F2, h2 = learned.features(x[p], ZERO, ONE, 3, periodic)
+    assert np.array_equal(F[p], F2) and np.array_equal(h[p], h2)
+
+
+@pytest.mark.gpu
 def test_features_perfect_grid():
     """On a periodic perfect grid the 6 nearest are at the spacing and the
     next 2 at spacing * sqrt(2)."""
