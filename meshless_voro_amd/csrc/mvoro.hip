@@ -315,8 +315,8 @@ struct Piece {
   size_t bytes;
 };
 
-constexpr int DL_THREADS = 8;
-constexpr size_t DL_HALF = size_t(8) << 20;
+constexpr int DL_THREADS = 16;
+constexpr size_t DL_HALF = size_t(4) << 20;
 constexpr size_t DL_MIN_THREADED = size_t(32) << 20;  // below: one hipMemcpy per array
 
 struct DlStaging {
@@ -329,7 +329,7 @@ struct DlStaging {
 std::mutex dl_mu;
 
 // The staging of device dev, made on its first download and kept until the
-// process ends (16 MiB of pinned host memory per thread); nullptr if it
+// process ends (8 MiB of pinned host memory per thread); nullptr if it
 // cannot be made.  Called with dl_mu held.
 DlStaging *dl_staging(int dev) {
   static std::map<int, DlStaging *> made;
