@@ -398,6 +398,14 @@ mv_status mv_model_loss_grad(const mv_model *model, const float *features,
 mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,
                              int8_t *out_sign);
 
+/* Test hook, no counterpart in the reference: runs the host download's
+ * thread fan-out over `threads` shares, with the start of thread
+ * `fail_after` and later ones made to fail (-1: none), each share writing
+ * out[w] = w + 1.  Shows that shares whose thread could not start run on
+ * the calling thread and that no exception leaves the library.  Needs no
+ * GPU. */
+mv_status mv_test_run_shares(int32_t threads, int32_t fail_after, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,7 @@ EXPORTS = [
     "mv_face_integrals", "mv_tets", "mv_get_cell_geometry", "mv_model_features", "mv_model_train",
     "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact", "mv_kept_cells",
     "mv_with_faces", "mv_discard_faces", "mv_face_vertices", "mv_voronoi_from_integrator",
+    "mv_test_run_shares",
 ]
 
 
@@ -193,5 +194,7 @@ def lib():
                                      C.POINTER(C.c_float), vp]
     L.mv_in_sphere_exact.restype = C.c_int
     L.mv_in_sphere_exact.argtypes = [vp, sz, vp]
+    L.mv_test_run_shares.restype = C.c_int
+    L.mv_test_run_shares.argtypes = [C.c_int32, C.c_int32, vp]
     _lib = L
     return L

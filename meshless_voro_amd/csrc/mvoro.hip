@@ -16,6 +16,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -107,6 +109,20 @@ struct Temps {
 struct BuildError {
   mv_status st;
 };
+
+// The status of the exception being handled (call inside a catch block):
+// nothing thrown by the library may cross the C ABI.
+mv_status status_of_exception() noexcept {
+  try {
+    throw;
+  } catch (const BuildError &e) {
+    return e.st;
+  } catch (const std::bad_alloc &) {
+    return MV_E_OOM;
+  } catch (...) {
+    return MV_E_DEVICE;
+  }
+}
 
 #define HIPCHK(x)                                   \
   do {                                              \
@@ -394,6 +410,30 @@ void dl_share(DlStaging *S, int w, int dev, const std::vector<Piece> *P, size_t 
   }
 }
 
+// Runs fn(0) .. fn(T - 1), one thread each.  A thread that cannot be
+// started (std::system_error under a thread limit, std::bad_alloc) leaves
+// its share, and every later one, to the calling thread; the threads that
+// did start are joined in every case, so no joinable std::thread is ever
+// destroyed.  fail_after >= 0 makes the start of thread fail_after (and of
+// every later one) throw, for the test of this path.
+template <typename F>
+void run_shares(int T, int fail_after, F &&fn) {
+  std::vector<std::thread> th;
+  int started = 0;
+  try {
+    th.reserve((size_t)T);
+    for (; started < T; started++) {
+      if (fail_after >= 0 && started >= fail_after)
+        throw std::system_error(std::make_error_code(std::errc::resource_unavailable_try_again));
+      th.emplace_back([&fn, started] { fn(started); });
+    }
+  } catch (...) {
+    // fewer threads: the rest of the shares run here
+  }
+  for (int w = started; w < T; w++) fn(w);
+  for (auto &x : th) x.join();
+}
+
 void copy_to_host(const std::vector<Piece> &P) {
   size_t total = 0;
   for (const Piece &p : P) total += p.bytes;
@@ -417,12 +457,10 @@ void copy_to_host(const std::vector<Piece> &P) {
   const int T = (int)std::min<size_t>(DL_THREADS, (total + 4 * DL_HALF - 1) / (4 * DL_HALF));
   const size_t share = ((total + T - 1) / T + 4095) & ~size_t(4095);
   std::atomic<int> bad{0};
-  std::vector<std::thread> th;
-  for (int w = 0; w < T; w++) {
+  run_shares(T, -1, [&](int w) {
     const size_t lo = std::min(total, w * share), hi = std::min(total, lo + share);
-    th.emplace_back(dl_share, S, w, dev, &P, lo, hi, &bad);
-  }
-  for (auto &x : th) x.join();
+    dl_share(S, w, dev, &P, lo, hi, &bad);
+  });
   if (bad.load()) throw BuildError{MV_E_DEVICE};
 }
 
@@ -1512,11 +1550,12 @@ mv_status run_build(const double *d_xyz, size_t n, const uint8_t *d_mask,
     probs[0].g0 = 0;
     probs[0].g1 = n;
     build_multi({T}, d_xyz, n, d_mask, probs, dim, o);
-  } catch (const BuildError &e) {
+  } catch (...) {
+    const mv_status st = status_of_exception();
     (void)hipDeviceSynchronize();
     (void)hipGetLastError();
     delete T;
-    return e.st;
+    return st;
   }
   *out = T;
   return MV_OK;
@@ -1655,10 +1694,11 @@ template <typename F>
 mv_status guarded(F &&f) {
   try {
     f();
-  } catch (const BuildError &e) {
+  } catch (...) {
+    const mv_status st = status_of_exception();
     (void)hipDeviceSynchronize();
     (void)hipGetLastError();
-    return e.st;
+    return st;
   }
   return MV_OK;
 }
@@ -1692,11 +1732,12 @@ mv_status run_build_batch(const double *d_xyz, const uint64_t *offsets, uint32_t
       probs[p].g1 = offsets[p + 1];
     }
     build_multi(Ts, d_xyz, offsets[P], d_mask, probs, dim, o);
-  } catch (const BuildError &e) {
+  } catch (...) {
+    const mv_status st = status_of_exception();
     (void)hipDeviceSynchronize();
     (void)hipGetLastError();
     for (mv_tess *t : Ts) delete t;
-    return e.st;
+    return st;
   }
   for (uint32_t p = 0; p < P; p++) out[p] = Ts[p];
   return MV_OK;
@@ -2147,10 +2188,11 @@ mv_status mv_build(const double *xyz, size_t n, const uint8_t *mask,
   if (st == MV_OK) {
     try {
       download(*out);
-    } catch (const BuildError &e) {
+    } catch (...) {
+      const mv_status st2 = status_of_exception();
       delete *out;
       *out = nullptr;
-      return e.st;
+      return st2;
     }
   }
   return st;
@@ -2174,7 +2216,7 @@ size_t mv_n_links(const mv_tess *t) { return t->nl; }
   ret mv_##name(mv_tess *t) {                    \
     try {                                        \
       download(t);                               \
-    } catch (const BuildError &) {               \
+    } catch (...) {                              \
       return nullptr;                            \
     }                                            \
     return t->field.data();                      \
@@ -2664,6 +2706,11 @@ mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g) {
   static_assert(sizeof(CellGeom) == sizeof(mv_cell_geometry), "mv_cell_geometry layout");
   memcpy(g, &G, sizeof G);
   return MV_OK;
+}
+
+mv_status mv_test_run_shares(int32_t threads, int32_t fail_after, int32_t *out) {
+  if (threads < 0 || threads > 4096 || (threads && !out)) return MV_E_ARG;
+  return guarded([&] { run_shares(threads, fail_after, [&](int w) { out[w] = w + 1; }); });
 }
 
 mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,

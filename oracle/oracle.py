@@ -18,10 +18,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def compile_lib(force: bool = False) -> None:
-    """Compile liboracle.so / liboracle_flops.so with the Makefile here."""
+    """Compile liboracle.so / liboracle_flops.so with the Makefile here.
+
+    make rebuilds a library whose sources are newer, so an edited oracle is
+    never tested stale.  Where make is absent (a box that only runs the
+    prebuilt libraries) existing libraries are used as they are."""
     libs = [os.path.join(_HERE, n) for n in ("liboracle.so", "liboracle_flops.so")]
-    if force or not all(os.path.exists(p) for p in libs):
+    try:
         subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    except FileNotFoundError:
+        if not all(os.path.exists(p) for p in libs):
+            raise
 
 
 class _OrcVoronoi(C.Structure):

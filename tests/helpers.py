@@ -84,3 +84,25 @@ def closure_residual(v):
         s = np.sum((v.face_area[fs] * sgn)[:, None] * v.face_normal[fs], 0)
         worst = max(worst, float(np.linalg.norm(s) / np.sum(v.face_area[fs])))
     return worst
+
+
+def degenerate_fixture(r):
+    """The point set of the reference's degenerate_test /
+    degenerate_test_simplified (src/voronoi.rs:1157-1184): perturbed_grid
+    with pert 0 (src/voronoi.rs:840-859), i.e. per component
+    (i + 0.5) * width / count + anchor, with one seed moved.  Returns
+    (positions, anchor, width).  The bits are checked against that formula
+    evaluated in Python floats, in the reference's order of operations."""
+    from meshless_voro_amd import workloads as W
+    count = int(r["grid"])
+    anchor = np.array(r["anchor"], np.float64)
+    width = np.array(r["width"], np.float64)
+    x = W.perturbed_grid(count, 0.0, anchor=anchor, width=width)
+    for n in range(count ** 3):
+        ijk = (n // count ** 2, (n % count ** 2) // count, n % count)
+        for c in range(3):
+            want = (float(ijk[c]) + 0.5) * float(width[c]) / float(count) + float(anchor[c])
+            want = min(max(want, float(anchor[c])), float(anchor[c]) + float(width[c]))
+            assert x[n, c] == want, (n, c, x[n, c], want)
+    x[r["moved_index"]] = r["moved_to"]
+    return x, anchor, width

@@ -60,3 +60,18 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(sn, "size")] == C.sizeof(cls), sn
         for f, _ in cls._fields_:
             assert got[(sn, f)] == getattr(cls, f).offset, (sn, f)
+
+
+def test_download_threads_fall_back_to_the_calling_thread():
+    """The host download's fan-out (copy_to_host): when a thread cannot be
+    started, its share and every later one run on the calling thread, the
+    started threads are joined, and the call returns MV_OK instead of
+    letting an exception cross the C ABI (std::terminate would abort the
+    process).  Exercised through the test hook without a GPU."""
+    import numpy as np
+    L = _native.lib()
+    for T, fail_after in ((16, -1), (16, 0), (16, 5), (16, 15), (3, 1), (0, -1)):
+        out = np.zeros(max(T, 1), np.int32)
+        assert L.mv_test_run_shares(T, fail_after, out.ctypes.data) == 0
+        assert out[:T].tolist() == list(range(1, T + 1))
+    assert L.mv_test_run_shares(-1, -1, None) == 1

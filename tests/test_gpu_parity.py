@@ -131,17 +131,15 @@ def test_reference_fixtures():
     z = r["test_non_perturbed_z"]
     check(*both(np.array(z["points"]), z["anchor"], z["width"]))
     d = r["degenerate_test"]
-    x = W.grid(d["grid"]) * d["width"][0] + np.array(d["anchor"])
-    x[d["moved_index"]] = d["moved_to"]
-    g, o = both(x, d["anchor"], d["width"])
+    x, a, w = degenerate_fixture(d)
+    g, o = both(x, a, w)
     check(g, o)
     consistency_check(g)
     s = r["degenerate_test_simplified"]
-    x = W.grid(s["grid"]) * np.array(s["width"]) + np.array(s["anchor"])
-    x[s["moved_index"]] = s["moved_to"]
+    x, a, w = degenerate_fixture(s)
     m = np.zeros(27, bool)
     m[s["mask_true"]] = True
-    check(*both(x, s["anchor"], s["width"], mask=m))
+    check(*both(x, a, w, mask=m))
 
 
 @pytest.mark.parametrize("dim", [1, 2])

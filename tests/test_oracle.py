@@ -10,7 +10,7 @@ import pytest
 
 from oracle import oracle as O
 from meshless_voro_amd import workloads as W
-from tests.helpers import closure_residual, consistency_check
+from tests.helpers import closure_residual, consistency_check, degenerate_fixture
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 ZERO, ONE = [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]
@@ -145,10 +145,42 @@ def test_neighbour_stream_ties_on_grid():
 
 # -------------------------------------------------------- whole builds
 def test_single_cell():
-    """src/voronoi.rs:902-909"""
+    """src/voronoi.rs:902-909; the six box faces sit at their face centres,
+    with the box planes' inward normals (src/voronoi/boundary.rs:32-39)."""
     v = O.build([[0.5, 0.5, 0.5]], ZERO, ONE)
     assert v.cell_volume[0] == pytest.approx(1.0, abs=1e-15)
     assert len(v.face_area) == 6 and np.allclose(v.face_area, 1.0)
+    assert np.allclose(v.cell_centroid[0], [0.5, 0.5, 0.5], rtol=0, atol=1e-15)
+    centres = {(0.0, 0.5, 0.5), (1.0, 0.5, 0.5), (0.5, 0.0, 0.5), (0.5, 1.0, 0.5),
+               (0.5, 0.5, 0.0), (0.5, 0.5, 1.0)}
+    got = {tuple(float(round(c, 12)) for c in f) for f in v.face_centroid}
+    assert got == centres
+    # the normal points from the face towards the generator
+    for f in range(6):
+        assert np.dot(v.face_normal[f], [0.5, 0.5, 0.5] - v.face_centroid[f]) == pytest.approx(0.5)
+
+
+def test_single_cell_off_centre_box():
+    """One generator off the centre of a non-cubic box: the cell is the box,
+    so volume, centroid and the face centroids are the box's, whatever the
+    generator position (src/voronoi.rs:902-909 generalised)."""
+    a, w = np.array([1.0, -2.0, 0.5]), np.array([2.0, 3.0, 0.25])
+    v = O.build([[1.3, -0.1, 0.6]], a, w)
+    assert v.cell_volume[0] == pytest.approx(1.5, rel=1e-14)
+    mid = a + 0.5 * w
+    assert np.allclose(v.cell_centroid[0], mid, rtol=0, atol=1e-14)
+    want = []
+    for ax in range(3):
+        for end in (a[ax], a[ax] + w[ax]):
+            c = mid.copy()
+            c[ax] = end
+            want.append(c)
+    got = v.face_centroid
+    for c in want:
+        assert np.min(np.max(np.abs(got - c), axis=1)) < 1e-14
+    areas = sorted(float(x) for x in v.face_area)
+    assert areas == pytest.approx(sorted([w[1] * w[2]] * 2 + [w[0] * w[2]] * 2 +
+                                         [w[0] * w[1]] * 2), rel=1e-14)
 
 
 def test_two_cells():
@@ -167,15 +199,39 @@ def test_regular_grid_cells(count):
     assert np.allclose(v.cell_volume, h ** 3, rtol=1e-12)
     assert np.allclose(v.cell_centroid, x, rtol=0, atol=1e-14)
     assert np.allclose(v.face_area, h * h, rtol=1e-12)
+    _grid_face_closed_form(v, x, h)
     assert v.exact_tests > 0
+
+
+def _grid_face_closed_form(v, x, h):
+    """Perfect grid: every face is the h x h square half a spacing from its
+    left generator, so its centroid is loc[left] - (h / 2) * normal (the
+    normal points towards the left generator), it lies on the axis through
+    the generator, and an internal face's centroid is the midpoint of its two
+    generators (src/voronoi/voronoi_face.rs:36-43)."""
+    left = v.face_left.astype(np.int64)
+    want = x[left] - 0.5 * h * v.face_normal
+    assert np.allclose(v.face_centroid, want, rtol=0, atol=1e-14)
+    # the normal is an axis vector
+    assert np.allclose(np.sort(np.abs(v.face_normal), 1), [0.0, 0.0, 1.0], atol=0)
+    internal = (v.face_right >= 0) & (v.face_has_shift == 0)
+    right = v.face_right[internal].astype(np.int64)
+    mid = 0.5 * (x[left[internal]] + x[right])
+    assert np.allclose(v.face_centroid[internal], mid, rtol=0, atol=1e-14)
 
 
 def test_regular_grid_closed_form_periodic():
     x = W.grid(6)
     v = O.build(x, ZERO, ONE, periodic=True)
     assert np.allclose(v.cell_volume, 1.0 / 216, rtol=1e-12)
+    assert np.allclose(v.cell_centroid, x, rtol=0, atol=1e-14)
     # every cell has exactly 6 face links
     assert np.all(v.cell_face_count == 6)
+    _grid_face_closed_form(v, x, 1.0 / 6)
+    # a periodic face is built by both of its cells, each in its own frame
+    per = v.face_has_shift != 0
+    assert per.any()
+    assert np.allclose(v.face_area, 1.0 / 36, rtol=1e-12)
 
 
 def test_125_cells():
@@ -210,20 +266,18 @@ def test_partial():
 def test_degenerate_large_coordinates():
     """degenerate_test (src/voronoi.rs:1157-1165)"""
     r = json.load(open(os.path.join(GOLDEN, "reference_points.json")))["degenerate_test"]
-    x = W.grid(r["grid"]) * r["width"][0] + np.array(r["anchor"])
-    x[r["moved_index"]] = r["moved_to"]
-    v = O.build(x, r["anchor"], r["width"])
+    x, a, w = degenerate_fixture(r)
+    v = O.build(x, a, w)
     consistency_check(v)
 
 
 def test_degenerate_simplified_partial():
     """degenerate_test_simplified (src/voronoi.rs:1167-1184)"""
     r = json.load(open(os.path.join(GOLDEN, "reference_points.json")))["degenerate_test_simplified"]
-    x = W.grid(r["grid"]) * np.array(r["width"]) + np.array(r["anchor"])
-    x[r["moved_index"]] = r["moved_to"]
+    x, a, w = degenerate_fixture(r)
     m = np.zeros(27, bool)
     m[r["mask_true"]] = True
-    v = O.build(x, r["anchor"], r["width"], mask=m)
+    v = O.build(x, a, w, mask=m)
     consistency_check(v)
 
 
@@ -274,8 +328,14 @@ def test_1_d():
 
 @pytest.mark.parametrize("periodic", [False, True])
 def test_scipy_brute_force(periodic):
-    """Volumes, centroids and face areas of every cell against Qhull
-    (scipy HalfspaceIntersection + ConvexHull) on 150 random points."""
+    """Volumes, centroids, face areas and face centroids of every 7th cell
+    against Qhull (scipy HalfspaceIntersection + ConvexHull) on 150 random
+    points.  Each face the oracle reports for a cell (as left, or as the
+    right of an unshifted face) is matched to the hull triangles whose
+    outward normal is the face's (normals of distinct faces of a convex cell
+    differ); the face's area and centroid are the area-weighted sums over
+    those triangles, and the cell centroid is the volume-weighted sum over
+    the tetrahedra from an interior point to every hull triangle."""
     from scipy.spatial import ConvexHull, HalfspaceIntersection
     rng = np.random.default_rng(21)
     x = rng.random((150, 3))
@@ -283,6 +343,7 @@ def test_scipy_brute_force(periodic):
     imgs = [np.array([i, j, k], float) for i in (-1, 0, 1) for j in (-1, 0, 1)
             for k in (-1, 0, 1)] if periodic else [np.zeros(3)]
     pts = np.concatenate([x + s for s in imgs])
+    nface = 0
     for c in range(0, 150, 7):
         q = x[c]
         hs = []
@@ -300,6 +361,34 @@ def test_scipy_brute_force(periodic):
         hsi = HalfspaceIntersection(np.array(hs), q)
         hull = ConvexHull(hsi.intersections)
         assert v.cell_volume[c] == pytest.approx(hull.volume, rel=1e-9)
+        tri = hull.points[hull.simplices]                      # (T, 3, 3)
+        cr = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+        t_area = 0.5 * np.linalg.norm(cr, axis=1)
+        t_cen = tri.mean(axis=1)
+        # cell centroid from tets (interior point m, triangle)
+        m = hull.points[hull.vertices].mean(axis=0)
+        t_vol = np.abs(np.einsum("ij,ij->i", tri[:, 0] - m, cr)) / 6.0
+        cen = np.sum(t_vol[:, None] * (tri.sum(axis=1) + m), 0) / (4.0 * t_vol.sum())
+        assert np.allclose(v.cell_centroid[c], cen, rtol=0, atol=1e-9)
+        # faces of cell c; the face normal points towards the left generator
+        own = np.nonzero(v.face_left == c)[0]
+        other = np.nonzero((v.face_right == c) & (v.face_has_shift == 0))[0]
+        out_n = np.concatenate([-v.face_normal[own], v.face_normal[other]])
+        fids = np.concatenate([own, other])
+        claimed = np.zeros(len(tri), bool)
+        for f, nrm in zip(fids, out_n):
+            sel = np.linalg.norm(hull.equations[:, :3] - nrm, axis=1) < 1e-7
+            claimed |= sel
+            if v.face_area[f] < 1e-6:
+                continue  # Qhull may merge a sliver into its neighbour
+            a = t_area[sel].sum()
+            assert v.face_area[f] == pytest.approx(a, rel=1e-8)
+            fc = np.sum(t_area[sel, None] * t_cen[sel], 0) / a
+            assert np.allclose(v.face_centroid[f], fc, rtol=0, atol=1e-9)
+            nface += 1
+        # every hull triangle of a face of non-negligible size was claimed
+        assert t_area[~claimed].sum() < 1e-6
+    assert nface > 200
 
 
 def test_perturbed_grid_32_invariants():
@@ -325,3 +414,38 @@ def test_flop_count_per_uniform_cell():
     v = O.build(x, ZERO, ONE, periodic=True, flops=True)
     per_cell = v.flops / x.shape[0]
     assert 2.0e4 < per_cell < 4.0e4
+
+
+# the cells at which the jittered 64^3 grid fails (index in the built order),
+# per (periodic, permutation seed); found by full oracle builds
+_JITTER_FAILS = {(False, None): 57798, (False, 1): 14149, (False, 2): 11141,
+                 (True, None): 44063, (True, 1): 14149, (True, 2): 23604}
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("perm_seed", [None, 1, 2])
+def test_grid_jitter_64_boundary_walk_fails(periodic, perm_seed):
+    """The 64^3 grid with 1 % of the seeds moved by 1e-12 makes the
+    reference algorithm's boundary walk fail (compute_boundary's panic,
+    convex_cell.rs:428; status 6) in both boundary modes, and not because of
+    the tie order among equal keys: under two random permutations of the
+    seeds the build fails as well (at another cell).  The identity order in
+    the rectangular box runs the whole build; the other cases build the
+    failing cell alone (the per-cell construction does not depend on the
+    mask)."""
+    x = W.grid_jitter(64)
+    if perm_seed is not None:
+        x = x[np.random.default_rng(perm_seed).permutation(len(x))]
+    cell = _JITTER_FAILS[(periodic, perm_seed)]
+    mask = None
+    if (periodic, perm_seed) != (False, None):
+        mask = np.zeros(len(x), bool)
+        mask[cell] = True
+    with pytest.raises(O.OracleError) as e:
+        O.build(x, ZERO, ONE, periodic=periodic, mask=mask)
+    assert e.value.status == 6
+    assert e.value.cell == cell
+    # the cells before it in a partial build of a few neighbours succeed
+    m = np.zeros(len(x), bool)
+    m[max(0, cell - 3):cell] = True
+    O.build(x, ZERO, ONE, periodic=periodic, mask=m)

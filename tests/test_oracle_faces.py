@@ -116,14 +116,20 @@ def test_fan_integrals_equal_decomposition_without_faces(periodic):
         cid = np.repeat(np.arange(len(x)), np.diff(T.tet_offset.astype(np.int64)))
         key = cid * 1000 + T.tet_plane
         ks, inv = np.unique(key, return_inverse=True)
-        return cell, ks, np.bincount(inv, weights=area)
+        fa = np.bincount(inv, weights=area)
+        # face centroid: area-weighted mean of the triangles' centroids
+        tc = (v0 + v1 + v2) / 3.0
+        fc = np.stack([np.bincount(inv, weights=area * tc[:, i]) for i in range(3)], 1)
+        return cell, ks, fa, fc / np.where(fa > 0, fa, 1.0)[:, None]
 
-    ca, ka, fa = per_cell(A)
-    cb, kb, fb = per_cell(B)
+    ca, ka, fa, fca = per_cell(A)
+    cb, kb, fb, fcb = per_cell(B)
     assert np.allclose(cb, ca, rtol=1e-12, atol=0)
     assert abs(cb.sum() - 27.0) < 1e-11
     assert np.array_equal(ka, kb)
     assert np.allclose(fb, fa, rtol=1e-10, atol=1e-13)
+    big = fa > 1e-8
+    assert np.allclose(fcb[big], fca[big], rtol=0, atol=1e-9)
 
 
 def test_with_faces_needs_3d():
