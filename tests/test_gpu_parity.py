@@ -11,7 +11,7 @@ import pytest
 from meshless_voro_amd import BuildOptions, Voronoi, VoronoiError, workloads as W
 from oracle import oracle as O
 from tests.helpers import (assert_connectivity_equal, assert_integrals_close, closure_residual,
-                           consistency_check)
+                           consistency_check, degenerate_fixture)
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
