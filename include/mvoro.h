@@ -44,8 +44,12 @@ typedef enum mv_status {
                                  capacity (planes / vertices / neighbours) */
   MV_E_FACE_WALK = 10,        /* no next vertex around a face,
                                  src/voronoi/convex_cell.rs:538-541 */
-  MV_E_NOT_3D = 11            /* with_faces outside 3D,
+  MV_E_NOT_3D = 11,           /* with_faces outside 3D,
                                  src/voronoi/convex_cell.rs:462-466 */
+  MV_E_SCRATCH = 12,          /* the allocation-free build's static scratch
+                                 cannot hold this input */
+  MV_E_OUT_SMALL = 13         /* a caller's output array is too short (the
+                                 counts needed are returned) */
 } mv_status;
 
 /* Construction modes. */
@@ -397,6 +401,56 @@ mv_status mv_model_loss_grad(const mv_model *model, const float *features,
  * writes -1, 0 or +1 per tuple. */
 mv_status mv_in_sphere_exact(const int64_t *abcdv, size_t count,
                              int8_t *out_sign);
+
+/* ---- allocation-free build ------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The same construction as
+ * mv_build_device (same kernels, same results bit for bit), but every
+ * scratch buffer -- grid, neighbour lists, face staging, counters, the
+ * learned model's weights -- is carved from one static device array of
+ * MV_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends, and the results go to device arrays the caller passes.
+ * The call allocates nothing (no hipMalloc, hipFree, stream or event
+ * creation, no pinned host memory), runs on the null stream and returns
+ * when the results are written.  Every call on one GPU shares that GPU's
+ * scratch: calls from one process are serialised by the library, but a
+ * call that overlaps another on the same GPU (from another thread's own
+ * copy of the library, say) races.  Limits: n <= MV_NOALLOC_MAX_SEEDS
+ * (larger inputs return MV_E_SCRATCH at once); an input whose lists or
+ * faces outgrow the scratch returns MV_E_SCRATCH too; MV_FLAG_KEEP_CELLS is
+ * refused (MV_E_ARG: kept cells need a handle). */
+#define MV_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define MV_NOALLOC_MAX_SEEDS 32768u
+
+/* Caller-owned device output arrays (layouts as mv_device_view; any
+ * pointer may be NULL to skip that array).  Cell arrays hold n entries
+ * (x3 where noted), face arrays face_cap (x3), links link_cap.  On return
+ * n_faces / n_links hold the counts; when they exceed face_cap / link_cap
+ * nothing is written and the call returns MV_E_OUT_SMALL. */
+typedef struct mv_noalloc_out {
+  double *cell_loc, *cell_centroid, *cell_volume, *cell_safety_radius; /* 3n, 3n, n, n */
+  uint64_t *cell_face_offset, *cell_face_count, *cell_idx;            /* n */
+  uint64_t *face_left;                                                /* face_cap */
+  int64_t *face_right;
+  double *face_area, *face_centroid, *face_normal;                    /* F, 3F, 3F */
+  uint8_t *face_has_shift;
+  double *face_shift;                                                 /* 3F */
+  uint64_t *cell_face_connections;                                    /* link_cap */
+  size_t face_cap, link_cap;
+  size_t n_faces, n_links;                                            /* out */
+  size_t scratch_used;                                                /* out: bytes */
+} mv_noalloc_out;
+
+mv_status mv_build_device_noalloc(const double *d_xyz, size_t n,
+                                  const uint8_t *d_mask, const double anchor[3],
+                                  const double width[3], uint32_t dimensionality,
+                                  uint32_t periodic, const mv_options *opt,
+                                  mv_noalloc_out *out);
+
+/* Test hook, no counterpart in the reference: the number of hipMalloc calls
+ * the library has made in this process. */
+uint64_t mv_test_device_mallocs(void);
 
 /* Test hook, no counterpart in the reference: runs the host download's
  * thread fan-out over `threads` shares, with the start of thread

@@ -101,6 +101,20 @@ class mv_device_view(C.Structure):
         "face_has_shift", "face_shift", "cell_face_connections")]
 
 
+class mv_noalloc_out(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "cell_loc", "cell_centroid", "cell_volume", "cell_safety_radius",
+        "cell_face_offset", "cell_face_count", "cell_idx", "face_left",
+        "face_right", "face_area", "face_centroid", "face_normal",
+        "face_has_shift", "face_shift", "cell_face_connections")] + [
+        (n, C.c_size_t) for n in ("face_cap", "link_cap", "n_faces", "n_links",
+                                  "scratch_used")]
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20
+NOALLOC_MAX_SEEDS = 32768
+
+
 # every symbol declared in include/mvoro.h
 EXPORTS = [
     "mv_default_options", "mv_build", "mv_build_device", "mv_build_batch",
@@ -115,7 +129,7 @@ EXPORTS = [
     "mv_face_integrals", "mv_tets", "mv_get_cell_geometry", "mv_model_features", "mv_model_train",
     "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact", "mv_kept_cells",
     "mv_with_faces", "mv_discard_faces", "mv_face_vertices", "mv_voronoi_from_integrator",
-    "mv_test_run_shares",
+    "mv_test_run_shares", "mv_build_device_noalloc", "mv_test_device_mallocs",
 ]
 
 
@@ -194,6 +208,11 @@ def lib():
                                      C.POINTER(C.c_float), vp]
     L.mv_in_sphere_exact.restype = C.c_int
     L.mv_in_sphere_exact.argtypes = [vp, sz, vp]
+    L.mv_build_device_noalloc.restype = C.c_int
+    L.mv_build_device_noalloc.argtypes = [vp, sz, vp, vp, vp, u32, u32, C.POINTER(mv_options),
+                                          C.POINTER(mv_noalloc_out)]
+    L.mv_test_device_mallocs.restype = C.c_uint64
+    L.mv_test_device_mallocs.argtypes = []
     L.mv_test_run_shares.restype = C.c_int
     L.mv_test_run_shares.argtypes = [C.c_int32, C.c_int32, vp]
     _lib = L

@@ -32,6 +32,10 @@
 #include "mv_learn.hpp"
 #include "mvoro_integrals.hpp"
 
+// Device memory of the allocation-free build (mv_build_device_noalloc):
+// every scratch buffer of such a build is carved from this array.
+static __device__ __align__(256) uint8_t scratch[MV_NOALLOC_SCRATCH_BYTES];
+
 using namespace mv;
 
 namespace {
@@ -48,12 +52,54 @@ Pool &pool() {
   static Pool *p = new Pool();
   return *p;
 }
+// While a thread runs the allocation-free build, its device allocations are
+// carved from one region (the static scratch) by a bump pointer and never
+// released; nothing reaches hipMalloc.
+struct Arena {
+  uint8_t *base = nullptr;
+  size_t size = 0, used = 0;
+  bool exhausted = false;
+  unsigned long long *words = nullptr;  // device words the host reads totals from
+};
+thread_local Arena *t_arena = nullptr;
+bool arena_active() { return t_arena != nullptr; }
+size_t arena_free() {
+  return t_arena && t_arena->used < t_arena->size ? t_arena->size - t_arena->used : 0;
+}
+void *arena_alloc(Arena &A, size_t bytes) {
+  const size_t off = (A.used + 255) & ~size_t(255);
+  const size_t b = bytes ? bytes : 1;
+  if (off > A.size || b > A.size - off) {
+    A.exhausted = true;
+    return nullptr;
+  }
+  A.used = off + b;
+  return A.base + off;
+}
+// hipMalloc calls made by this process (test hook)
+std::atomic<unsigned long long> g_device_mallocs{0};
+
+// The allocation-free build's scratch: one copy per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address; the device address comes from hipGetSymbolAddress (once
+// per device, with that device current) and only that pointer is used.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+struct NoAllocDevice {
+  std::mutex mu;     // serialises this process's calls on the device
+  void *ptr = nullptr;
+};
+NoAllocDevice &noalloc_device(int dev) {
+  static NoAllocDevice d[NOALLOC_MAX_DEVICES];
+  return d[dev];
+}
+
 size_t round_bytes(size_t b) {
   if (b < 512) return 512;
   if (b < (1u << 20)) return (b + 4095) & ~size_t(4095);
   return (b + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
 }
 void *dalloc(size_t bytes) {
+  if (t_arena) return arena_alloc(*t_arena, bytes);
   Pool &P = pool();
   size_t sz = round_bytes(bytes);
   std::lock_guard<std::mutex> g(P.mu);
@@ -66,10 +112,12 @@ void *dalloc(size_t bytes) {
     return p;
   }
   void *p = nullptr;
+  g_device_mallocs++;
   if (hipMalloc(&p, sz) != hipSuccess) {
     (void)hipGetLastError();
     for (auto &kv : P.free_) (void)hipFree(kv.second);
     P.free_.clear();
+    g_device_mallocs++;
     if (hipMalloc(&p, sz) != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
@@ -79,7 +127,8 @@ void *dalloc(size_t bytes) {
   return p;
 }
 void dfree(void *p) {
-  if (!p) return;
+  // arena memory is never returned (nor found among the pool's blocks)
+  if (!p || t_arena) return;
   Pool &P = pool();
   std::lock_guard<std::mutex> g(P.mu);
   auto it = P.live_.find(p);
@@ -158,15 +207,30 @@ void scan_excl_dev(const TIn *in, TOut *out, size_t n, hipStream_t s) {
   }
 }
 
-// a pinned host word per host thread that kernels write totals to (one
-// small kernel and a stream synchronisation per read, no copy)
+// Eight words per host thread that kernels write totals to: pinned host
+// memory (one small kernel and a stream synchronisation per read, no
+// copy), or, in the allocation-free build, device words of its scratch
+// read back with a copy.
 unsigned long long *host_word() {
+  if (t_arena) return t_arena->words;
   thread_local unsigned long long *w = nullptr;
   if (!w && hipHostMalloc((void **)&w, 64, hipHostMallocDefault) != hipSuccess) {
     w = nullptr;
     throw BuildError{MV_E_OOM};
   }
   return w;
+}
+// words [0, k) of host_word(), after the work queued on s has finished
+void read_words(hipStream_t s, unsigned long long *dst, int k) {
+  if (t_arena) {
+    HIPCHK(hipMemcpyAsync(dst, t_arena->words, (size_t)k * sizeof *dst, hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+    return;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  const volatile unsigned long long *w = host_word();
+  for (int i = 0; i < k; i++) dst[i] = w[i];
 }
 
 // exclusive scan; returns the total
@@ -178,15 +242,17 @@ TOut scan_excl(const TIn *in, TOut *out, size_t n, hipStream_t s) {
   hipLaunchKernelGGL((k_scan_total<TIn, TOut>), dim3(1), dim3(1), 0, s, in + n - 1,
                      (const TOut *)(out + n - 1), hw);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s));
-  return (TOut)*(volatile unsigned long long *)hw;
+  unsigned long long tot;
+  read_words(s, &tot, 1);
+  return (TOut)tot;
 }
 
 struct Timer {
   hipEvent_t a, b;
   hipStream_t s;
   bool on;
-  explicit Timer(hipStream_t st, bool enabled = true) : s(st), on(enabled) {
+  // no events in the allocation-free build (it creates nothing)
+  explicit Timer(hipStream_t st, bool enabled = true) : s(st), on(enabled && !t_arena) {
     if (!on) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
@@ -610,8 +676,10 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
     uint32_t cap = BIG_KNN_CAP;
     while (cap < 2 * (uint64_t)L && cap < (1u << 24)) cap <<= 1;
     const uint64_t per_block = 2ull * cap * (sizeof(double) + sizeof(uint64_t));
+    // in the allocation-free build, half of what is left of its scratch
+    const uint64_t budget = arena_active() ? arena_free() / 2 : BIG_KNN_BYTES;
     const uint32_t max_blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(BIG_KNN_BLOCKS, BIG_KNN_BYTES / per_block));
+        1, std::min<uint64_t>(BIG_KNN_BLOCKS, budget / per_block));
     const uint32_t blocks = std::min<uint32_t>(cnt, max_blocks);
     double *gkey = need(TT.get<double>((size_t)blocks * cap));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)blocks * cap));
@@ -884,9 +952,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (h) (void)hipStreamDestroy(h);
     }
   } st2;
-  HIPCHK(hipStreamCreateWithFlags(&st2.h, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&st2.ev, hipEventDisableTiming));
-  hipStream_t s2 = st2.h;
+  // the allocation-free build creates no stream: its big lists run on s
+  if (!arena_active()) {
+    HIPCHK(hipStreamCreateWithFlags(&st2.h, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&st2.ev, hipEventDisableTiming));
+  }
+  hipStream_t s2 = arena_active() ? s : st2.h;
   // ---- grid
   Timer tg(s);
   GridData D;
@@ -1106,8 +1177,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s, (const uint32_t *)ll, cnt,
                              host_word() + 1);
         const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
-        if (passes > 1)
-          lbnd = std::min<uint32_t>(lbnd, (uint32_t)((volatile unsigned long long *)host_word())[1]);
+        if (passes > 1) {
+          unsigned long long w2[2];
+          read_words(s, w2, 2);
+          lbnd = std::min<uint32_t>(lbnd, (uint32_t)w2[1]);
+        }
         uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
@@ -1251,7 +1325,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       };
       // the big list (long lists, large cells: few items, long latency) runs
       // on its own stream beside the main list
-      if (nbig) {
+      if (nbig && s2 != s) {
         // s2 starts after everything queued on s (the lists it reads)
         HIPCHK(hipEventRecord(st2.ev, s));
         HIPCHK(hipStreamWaitEvent(s2, st2.ev, 0));
@@ -1261,13 +1335,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false, s, true);
       if (nbig) ms_clip += tbig.stop();
       HIPCHK(hipStreamSynchronize(s2));
-      unsigned long long cnts[4];
-      unsigned long long *hw = host_word();
-      hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err, hw);
+      unsigned long long cnts[5];
+      hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err, host_word());
       HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(s));
-      for (int i = 0; i < 4; i++) cnts[i] = ((volatile unsigned long long *)hw)[i];
-      const unsigned long long cap_err_h = ((volatile unsigned long long *)hw)[4];
+      read_words(s, cnts, 5);
+      const unsigned long long cap_err_h = cnts[4];
       // a cell that reached kmax entries without its security radius
       if (cap_err_h) throw BuildError{MV_E_CAPACITY};
       if (cnts[1] != ~0ull) {
@@ -2272,6 +2344,9 @@ const char *mv_status_string(mv_status s) {
     case MV_E_CAPACITY: return "per-cell capacity exceeded";
     case MV_E_FACE_WALK: return "no next vertex connected to the current one around a face";
     case MV_E_NOT_3D: return "faces can only be extracted in 3D";
+    case MV_E_SCRATCH:
+      return "input too large for the allocation-free build's static scratch";
+    case MV_E_OUT_SMALL: return "an output array is too short (counts returned)";
   }
   return "unknown status";
 }
@@ -2706,6 +2781,94 @@ mv_status mv_get_cell_geometry(const mv_tess *t, mv_cell_geometry *g) {
   static_assert(sizeof(CellGeom) == sizeof(mv_cell_geometry), "mv_cell_geometry layout");
   memcpy(g, &G, sizeof G);
   return MV_OK;
+}
+
+uint64_t mv_test_device_mallocs(void) { return g_device_mallocs.load(); }
+
+mv_status mv_build_device_noalloc(const double *d_xyz, size_t n, const uint8_t *d_mask,
+                                  const double anchor[3], const double width[3],
+                                  uint32_t dimensionality, uint32_t periodic,
+                                  const mv_options *opt, mv_noalloc_out *out) {
+  if (!out) return MV_E_ARG;
+  out->n_faces = out->n_links = out->scratch_used = 0;
+  if (!valid_args(n, dimensionality, anchor, width) || (n && !d_xyz)) return MV_E_ARG;
+  if (n > MV_NOALLOC_MAX_SEEDS) return MV_E_SCRATCH;
+  mv_options o;
+  mv_default_options(&o);
+  if (opt) o = *opt;
+  if (o.flags & MV_FLAG_KEEP_CELLS) return MV_E_ARG;
+  if (o.mode != MV_MODE_PLAIN && o.mode != MV_MODE_LEARNED) return MV_E_ARG;
+  if (o.mode == MV_MODE_LEARNED &&
+      (!o.model || o.model->magic != MV_MODEL_MAGIC || o.model->version != 1))
+    return MV_E_ARG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    (void)hipGetLastError();
+    return MV_E_DEVICE;
+  }
+  NoAllocDevice &ND = noalloc_device(dev);
+  std::lock_guard<std::mutex> g(ND.mu);
+  if (!ND.ptr) {
+    // first call on this device (current): the device address of its copy
+    // of `scratch`; host code uses only this pointer, never the symbol
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return MV_E_DEVICE;
+    }
+    ND.ptr = p;
+  }
+  Arena A;
+  A.base = (uint8_t *)ND.ptr;
+  A.size = MV_NOALLOC_SCRATCH_BYTES;
+  A.words = (unsigned long long *)arena_alloc(A, 8 * sizeof(unsigned long long));
+  t_arena = &A;
+  struct Leave {
+    ~Leave() { t_arena = nullptr; }
+  } leave;
+  mv_status st = MV_OK;
+  try {
+    mv_tess T;
+    std::vector<ProblemSpec> probs(1);
+    normalise_box(anchor, width, dimensionality, probs[0].anchor, probs[0].width);
+    probs[0].periodic = periodic ? 1 : 0;
+    probs[0].g0 = 0;
+    probs[0].g1 = n;
+    build_multi({&T}, d_xyz, n, d_mask, probs, dimensionality, o);
+    out->n_faces = T.nf;
+    out->n_links = T.nl;
+    if (T.nf > out->face_cap || T.nl > out->link_cap) {
+      st = MV_E_OUT_SMALL;
+    } else {
+      const hipStream_t s = 0;
+      auto put = [&](void *dst, const void *src, size_t bytes) {
+        if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+      };
+      const size_t F = T.nf, NL = T.nl;
+      put(out->cell_loc, T.d_cell_loc, 3 * n * sizeof(double));
+      put(out->cell_centroid, T.d_cell_cen, 3 * n * sizeof(double));
+      put(out->cell_volume, T.d_cell_vol, n * sizeof(double));
+      put(out->cell_safety_radius, T.d_cell_sr, n * sizeof(double));
+      put(out->cell_face_offset, T.d_cell_foff, n * sizeof(uint64_t));
+      put(out->cell_face_count, T.d_cell_fcnt, n * sizeof(uint64_t));
+      put(out->cell_idx, T.d_cell_idx, n * sizeof(uint64_t));
+      put(out->face_left, T.d_face_left, F * sizeof(uint64_t));
+      put(out->face_right, T.d_face_right, F * sizeof(int64_t));
+      put(out->face_area, T.d_face_area, F * sizeof(double));
+      put(out->face_centroid, T.d_face_cen, 3 * F * sizeof(double));
+      put(out->face_normal, T.d_face_norm, 3 * F * sizeof(double));
+      put(out->face_has_shift, T.d_face_has_shift, F);
+      put(out->face_shift, T.d_face_shift, 3 * F * sizeof(double));
+      put(out->cell_face_connections, T.d_links, NL * sizeof(uint64_t));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+  } catch (...) {
+    st = A.exhausted ? MV_E_SCRATCH : status_of_exception();
+    (void)hipDeviceSynchronize();
+    (void)hipGetLastError();
+  }
+  out->scratch_used = A.used;
+  return st;
 }
 
 mv_status mv_test_run_shares(int32_t threads, int32_t fail_after, int32_t *out) {

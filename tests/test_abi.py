@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_without_gpu():
     L = _native.lib()
-    for s in range(12):
+    for s in range(14):
         assert L.mv_status_string(s) != b"unknown status"
 
 
@@ -39,8 +39,11 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"mv_options": _native.mv_options, "mv_stats": _native.mv_stats,
                "mv_model": _native.mv_model, "mv_train_config": _native.mv_train_config,
                "mv_device_view": _native.mv_device_view,
-               "mv_cell_geometry": _native.mv_cell_geometry}
+               "mv_cell_geometry": _native.mv_cell_geometry,
+               "mv_noalloc_out": _native.mv_noalloc_out}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mvoro.h"', "int main(void){"]
+    lines.append('printf("scratch bytes %llu\\n", (unsigned long long)MV_NOALLOC_SCRATCH_BYTES);')
+    lines.append('printf("scratch seeds %u\\n", MV_NOALLOC_MAX_SEEDS);')
     for sn, cls in structs.items():
         lines.append(f'printf("{sn} size %zu\\n", sizeof({sn}));')
         for f, _ in cls._fields_:
@@ -56,6 +59,8 @@ def test_struct_layouts_match_header(tmp_path):
     for line in out.splitlines():
         sn, f, v = line.split()
         got[(sn, f)] = int(v)
+    assert got[("scratch", "bytes")] == _native.NOALLOC_SCRATCH_BYTES
+    assert got[("scratch", "seeds")] == _native.NOALLOC_MAX_SEEDS
     for sn, cls in structs.items():
         assert got[(sn, "size")] == C.sizeof(cls), sn
         for f, _ in cls._fields_:
