@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--n", type=int, default=2097152)
     ap.add_argument("--modes", default="auto")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=524288)
+    ap.add_argument("--cpu-sample", type=int, default=2097152)
     ap.add_argument("--batched", action="store_true",
                     help="also measure many problems in one build (512 problems, ~10.6 M "
                          "seeds; the clustered problems take tens of seconds)")
@@ -280,17 +280,34 @@ def connectivity_digest(L, h):
             arr(L.mv_cell_face_connections(h), NL, C.c_uint64))
 
 
-def cpu_baseline(n_sample):
-    """The oracle (a C restatement of the reference's construction) on a
-    bounded sample of the same distribution, on this host's cores."""
+def cpu_threads():
+    """Threads for the CPU baseline: every core of this process's affinity
+    mask, within the host share the environment grants (OMP_NUM_THREADS, set
+    to the CPU share of one GPU on a shared GPU host)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return (min(aff, int(share)) if share.isdigit() and int(share) > 0 else aff), aff
+
+
+def cpu_baseline(n_sample, reps=3):
+    """The oracle (a C restatement of the reference's construction) on the
+    workload's seeds (the first n_sample of them), on this host's cores: one
+    warm-up build, then the median of `reps` timed builds."""
+    import numpy as np
     from oracle import oracle as O
     from meshless_voro_amd import workloads as W
-    cores = min(16, os.cpu_count() or 1)
-    x = W.uniform(n_sample, 2)
-    O.build(x[:4096], [0, 0, 0], [1, 1, 1], 3, True, nthreads=cores)  # warm-up
-    t0 = time.perf_counter()
-    O.build(x, [0, 0, 0], [1, 1, 1], 3, True, nthreads=cores)
-    dt = time.perf_counter() - t0
+    cores, aff = cpu_threads()
+    x = W.uniform_2m()[:n_sample] if n_sample <= 2097152 else W.uniform(n_sample, 2)
+    O.build(x, [0, 0, 0], [1, 1, 1], 3, True, nthreads=cores)  # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.build(x, [0, 0, 0], [1, 1, 1], 3, True, nthreads=cores)
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
     model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -300,9 +317,13 @@ def cpu_baseline(n_sample):
     except OSError:
         pass
     return {"value": n_sample / dt, "unit": "cells/s", "cores": cores, "kind": "port",
-            "sample": f"{n_sample} i.i.d. uniform seeds (SplitMix64 stream 2), periodic unit "
-                      f"cube, full build incl. grid/finalize, {dt:.2f} s on {cores} threads; "
-                      f"CPU: {model}"}
+            "host_cores": os.cpu_count(), "affinity_cores": aff,
+            "runs_s": [round(t, 3) for t in ts],
+            "sample": f"{n_sample} i.i.d. uniform seeds (SplitMix64 stream 2"
+                      f"{', the whole headline set' if n_sample == 2097152 else ''}), periodic "
+                      f"unit cube, full build incl. grid/finalize; median of {reps} after a "
+                      f"warm-up build: {dt:.2f} s on {cores} threads ({aff} cores in the "
+                      f"affinity mask, {os.cpu_count()} on the host); CPU: {model}"}
 
 
 def main():
@@ -388,10 +409,13 @@ def main():
         avg_ms = cw["ms"] / cw["launches"]
         units = cw["cells"] / cw["launches"]
         achieved = FLOPS_PER_CELL * units / (avg_ms / 1e3) / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 VALU (the FP64 vector and matrix peaks "
-                                          "coincide on MI355X)",
+        roof = {"bound": "valu", "pipe": "fp64 vector ALU (78.6 TFLOP/s counts an FMA as two "
+                                          "flops; the kernel is compiled without contraction, "
+                                          "so its op rate peaks at 39.3 Top/s)",
                 "kernel": "k_clip_wave", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "peak_unfused": FP64_PEAK_TFLOPS / 2,
+                "frac_unfused": achieved / (FP64_PEAK_TFLOPS / 2),
                 "traffic": traffic_from_profile("k_clip_wave"),
                 "per_unit": f"{FLOPS_PER_CELL:.0f} FP64 flops per completed cell",
                 "units_per_launch": units, "avg_launch_ms": avg_ms,
