@@ -477,14 +477,65 @@ k_clip_wave(WaveArgs A) {
         // reference's walk runs step by step.  Either way the new vertices
         // (cur, next, p_idx) follow in cycle order from the start.
         int vc = 0, vn = 0;
-        bool fast;
-        {
+        bool fast = false;
+        const int nrem = nv - num_v;
+        if (nrem <= 2) {
+          // one or two removed vertices (44 % of clips on uniform seeds):
+          // the walk in closed form.  One triangle (a, b, c) is the cycle
+          // a -> b -> c from a; a second one extends it by try_extend's
+          // first rule when exactly one of its planes, x, is off the cycle
+          // and the other two follow each other on it (g -> f): x goes
+          // between them and the start stays a.  Anything else is left to
+          // the step-by-step walk below.
+          const int d0 = rl_i(vd, num_v);
+          const int a = d0 & 0xFF, b = (d0 >> 8) & 0xFF, c = (d0 >> 16) & 0xFF;
+          int p0 = a, p1 = b, p2 = c, p3 = 0;
+          if (nrem == 1) {
+            fast = true;
+            clen = 3;
+          } else {
+            const int d1 = rl_i(vd, num_v + 1);
+            const int t0 = d1 & 0xFF, t1 = (d1 >> 8) & 0xFF, t2 = (d1 >> 16) & 0xFF;
+            const bool o0 = t0 != a && t0 != b && t0 != c;
+            const bool o1 = t1 != a && t1 != b && t1 != c;
+            const bool o2 = t2 != a && t2 != b && t2 != c;
+            if ((int)o0 + (int)o1 + (int)o2 == 1) {
+              // e: the plane off the cycle; f = e + 1, g = e - 1 (mod 3)
+              const int x = o0 ? t0 : o1 ? t1 : t2;
+              const int tf = o0 ? t1 : o1 ? t2 : t0;
+              const int tg = o0 ? t2 : o1 ? t0 : t1;
+              // the successor of g on a -> b -> c -> a
+              const int sg = tg == a ? b : tg == b ? c : a;
+              if (sg == tf) {
+                fast = true;
+                clen = 4;
+                if (tg == a) {
+                  p1 = x; p2 = b; p3 = c;
+                } else if (tg == b) {
+                  p2 = x; p3 = c;
+                } else {
+                  p3 = x;
+                }
+              }
+            }
+          }
+          if (fast) {
+            const int k = lane - num_v;
+            const int q4 = clen == 4 ? p3 : p0;
+            vc = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
+            vn = k == 0 ? p1 : k == 1 ? p2 : k == 2 ? q4 : p0;
+          }
+        }
+        if (!fast) {
           const bool rem = lane >= num_v && lane < nv;
           const int ta = vd & 0xFF, tb = (vd >> 8) & 0xFF, tc = (vd >> 16) & 0xFF;
+          // a directed edge set twice (its bit already there) means the
+          // triangles do not form a disk
+          bool dupe = false;
           if (rem) {
-            atomicOr(&S.u.e.emat[ta], 1ull << tb);
-            atomicOr(&S.u.e.emat[tb], 1ull << tc);
-            atomicOr(&S.u.e.emat[tc], 1ull << ta);
+            dupe = ((atomicOr(&S.u.e.emat[ta], 1ull << tb) >> tb) & 1ull) |
+                   ((atomicOr(&S.u.e.emat[tb], 1ull << tc) >> tc) & 1ull) |
+                   ((atomicOr(&S.u.e.emat[tc], 1ull << ta) >> ta) & 1ull);
             atomicOr(&S.u.e.rmat[tb], 1ull << ta);
             atomicOr(&S.u.e.rmat[tc], 1ull << tb);
             atomicOr(&S.u.e.rmat[ta], 1ull << tc);
@@ -500,15 +551,10 @@ k_clip_wave(WaveArgs A) {
           const int nxt = nbo ? (int)__builtin_ctzll(bo) : 0xFF;
           const int n_edge = __popcll(__ballot(nbo != 0));
           const int n_node = __popcll(__ballot(eo != 0ull));
-          // distinct directed edges (3 per triangle unless one repeats)
-          const int pc = __popcll(eo);
-          int n_dir = 0;
-#pragma unroll
-          for (int k = 0; k < 7; k++) n_dir += __popcll(__ballot((pc >> k) & 1)) << k;
           const int nf_t = nv - num_v;
-          // one boundary edge out of each boundary plane, V - E + F = 1 with
-          // E = (3 F + n_edge) / 2
-          fast = !__ballot(nbo > 1) && n_dir == 3 * nf_t && 2 * n_node == nf_t + n_edge + 2;
+          // one boundary edge out of each boundary plane, no directed edge
+          // twice, V - E + F = 1 with E = (3 F + n_edge) / 2
+          fast = !__ballot(nbo > 1 || dupe) && 2 * n_node == nf_t + n_edge + 2;
           if (fast) {
             const int a0 = rl_i(vd, num_v) & 0xFF;
             int cur = a0;
@@ -540,6 +586,10 @@ k_clip_wave(WaveArgs A) {
           wc_set(cyc, lane, a0, b0);
           wc_set(cyc, lane, b0, c0);
           wc_set(cyc, lane, c0, a0);
+          // SimpleCycle's `ptrs[x] != x` (x on the cycle) as a scalar mask:
+          // no cycle of the walk is shorter than two, so no node on it
+          // points to itself
+          uint64_t incyc = (1ull << a0) | (1ull << b0) | (1ull << c0);
           for (int r = num_v + 1; r < nv && state == CS_DONE; r++) {
             int ix = r;
             for (;;) {
@@ -550,22 +600,35 @@ k_clip_wave(WaveArgs A) {
               }
               const int du = rl_i(vd, ix);
               const int tri[3] = {du & 0xFF, (du >> 8) & 0xFF, (du >> 16) & 0xFF};
-              const int ptr[3] = {rl_i(cyc, tri[0]), rl_i(cyc, tri[1]), rl_i(cyc, tri[2])};
-              const bool con[3] = {ptr[0] != tri[0], ptr[1] != tri[1], ptr[2] != tri[2]};
+              const int cm = (int)((incyc >> tri[0]) & 1ull) |
+                             ((int)((incyc >> tri[1]) & 1ull) << 1) |
+                             ((int)((incyc >> tri[2]) & 1ull) << 2);
               bool ok = false;
-              for (int e = 0; e < 3 && !ok; e++) {
+              if (cm == 7) {
+                // try_extend's second rule: the path g -> f -> e closes
+                // around f, for the first e that has it
+                const int ptr[3] = {rl_i(cyc, tri[0]), rl_i(cyc, tri[1]), rl_i(cyc, tri[2])};
+                for (int e = 0; e < 3 && !ok; e++) {
+                  const int f = e == 2 ? 0 : e + 1, g = e == 0 ? 2 : e - 1;
+                  if (ptr[g] == tri[f] && ptr[f] == tri[e]) {
+                    wc_set(cyc, lane, tri[g], tri[e]);
+                    wc_set(cyc, lane, tri[f], tri[f]);
+                    incyc &= ~(1ull << tri[f]);
+                    if (cstart == tri[f]) cstart = tri[e];
+                    clen -= 1;
+                    ok = true;
+                  }
+                }
+              } else if (cm == 6 || cm == 5 || cm == 3) {
+                // the first rule: tri[e] is the one plane off the cycle and
+                // goes between g and f when the cycle has g -> f
+                const int e = cm == 6 ? 0 : cm == 5 ? 1 : 2;
                 const int f = e == 2 ? 0 : e + 1, g = e == 0 ? 2 : e - 1;
-                if (!con[e] && con[f] && con[g] && ptr[g] == tri[f]) {
+                if (rl_i(cyc, tri[g]) == tri[f]) {
                   wc_set(cyc, lane, tri[g], tri[e]);
                   wc_set(cyc, lane, tri[e], tri[f]);
+                  incyc |= 1ull << tri[e];
                   clen += 1;
-                  ok = true;
-                } else if (con[e] && con[f] && con[g] && ptr[g] == tri[f] &&
-                           ptr[f] == tri[e]) {
-                  wc_set(cyc, lane, tri[g], tri[e]);
-                  wc_set(cyc, lane, tri[f], tri[f]);
-                  if (cstart == tri[f]) cstart = tri[e];
-                  clen -= 1;
                   ok = true;
                 }
               }
