@@ -239,8 +239,8 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
         kept = !clipped;
         r2 = v.w;
       }
-      m[i0 >> 6] = __ballot(clipped);
-      nan_keep |= __ballot(kept && isnan(r2)) != 0ull;
+      m[i0 >> 6] = wave_ballot(clipped);
+      nan_keep |= wave_ballot(kept && isnan(r2)) != 0ull;
       // kept radii are >= 0; fmax drops NaN as the sequential loop does
       keepmax = fmax(keepmax, wave_max_all(kept && !isnan(r2) ? r2 : -INFINITY));
     }
@@ -402,7 +402,7 @@ __device__ __forceinline__ uint32_t coop_scan(const Lane &S, const CellRegs &c,
         }
       }
     }
-    const uint64_t m = __ballot(hit);
+    const uint64_t m = wave_ballot(hit);
     if (m) return j + (uint32_t)__builtin_ctzll(m);
   }
   return cnt;

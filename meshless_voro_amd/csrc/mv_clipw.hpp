@@ -274,12 +274,12 @@ k_clip_wave(WaveArgs A) {
         vs = 2. * sqrt(r2);
         vd = (int)du;
       }
-      if (__ballot(!ok)) {
+      if (wave_ballot(!ok)) {
         state = CS_ERROR;
         err = E_DET_ZERO;
       }
     }
-    if (state == CS_DONE && __ballot(lane < nv && isnan(vs))) {
+    if (state == CS_DONE && wave_ballot(lane < nv && isnan(vs))) {
       state = CS_ERROR;
       err = E_NAN;
     }
@@ -317,7 +317,7 @@ k_clip_wave(WaveArgs A) {
       S.u.e.emat[lane] = 0ull;
       S.u.e.rmat[lane] = 0ull;
       // the first entry whose distance is not finite and positive
-      const uint64_t bad_m = __ballot(je < cnt && !(isfinite(pdist) && pdist > 0.0));
+      const uint64_t bad_m = wave_ballot(je < cnt && !(isfinite(pdist) && pdist > 0.0));
       const int tbad = bad_m ? (int)__builtin_ctzll(bad_m) : 64;
       lds_fence();
       const int nb = (int)min(64u, cnt - j0);
@@ -338,7 +338,7 @@ k_clip_wave(WaveArgs A) {
           const double c = dot3(xyz(e), mk3(rl_d(vx, v), rl_d(vy, v), rl_d(vz, v))) - e.w;
           hit |= (fabs(c) < de.y) | (c < 0.);
         }
-        const uint64_t hm = __ballot(in && hit);
+        const uint64_t hm = wave_ballot(in && hit);
         t = hm ? (int)__builtin_ctzll(hm) : nb;
       }
       while (t < nb) {
@@ -351,10 +351,12 @@ k_clip_wave(WaveArgs A) {
         const d3 vl = mk3(vx, vy, vz);
         // sr < dist: every vertex has vs < dist; an event: a vertex near
         // or inside the plane (clip_by_plane, convex_cell.rs:368-390)
-        auto stops = [&](double dist) { return !(__ballot(!(vs < dist)) & act_m); };
+        auto stops = [&](double dist) { return !(wave_ballot(!(vs < dist)) & act_m); };
         auto touches = [&](double4 e, double errb) {
           const double c = dot3(xyz(e), vl) - e.w;
-          return __ballot(act && (fabs(c) < errb || c < 0.)) != 0ull;
+          // two ballots ORed on the scalar unit: a ballot of the two
+          // compares ORed per lane costs two more VALU instructions
+          return ((wave_ballot(fabs(c) < errb) | wave_ballot(c < 0.)) & act_m) != 0ull;
         };
         bool stop_hit = false;
         double4 ndA = S.u.e.nd[t];
@@ -398,11 +400,11 @@ k_clip_wave(WaveArgs A) {
         const bool near = act && fabs(cv) < errb;
         bool clipped = act && !near && cv < 0.;
         const uint64_t ref = (uint64_t)__double_as_longlong(S.u.e.pr[t].w);
-        if (!EXACT && __ballot(near)) {
+        if (!EXACT && wave_ballot(near)) {
           state = CS_EXACT;
           break;
         }
-        if (EXACT && __ballot(near)) {
+        if (EXACT && wave_ballot(near)) {
           d3 ngb = xyz(A.spos[pk_slot(ref)]);
           d3 sh;
           if (image_shift(pk_img(ref), B.width, &sh)) ngb = add3(ngb, sh);
@@ -417,7 +419,7 @@ k_clip_wave(WaveArgs A) {
             nexact++;
           }
         }
-        uint64_t M = __ballot(clipped);
+        uint64_t M = wave_ballot(clipped);
         if (!M) {
           t++;
           continue;
@@ -549,12 +551,12 @@ k_clip_wave(WaveArgs A) {
           const uint64_t bo = eo & ~ei;
           const int nbo = __popcll(bo);
           const int nxt = nbo ? (int)__builtin_ctzll(bo) : 0xFF;
-          const int n_edge = __popcll(__ballot(nbo != 0));
-          const int n_node = __popcll(__ballot(eo != 0ull));
+          const int n_edge = __popcll(wave_ballot(nbo != 0));
+          const int n_node = __popcll(wave_ballot(eo != 0ull));
           const int nf_t = nv - num_v;
           // one boundary edge out of each boundary plane, no directed edge
           // twice, V - E + F = 1 with E = (3 F + n_edge) / 2
-          fast = !__ballot(nbo > 1 || dupe) && 2 * n_node == nf_t + n_edge + 2;
+          fast = !wave_ballot(nbo > 1 || dupe) && 2 * n_node == nf_t + n_edge + 2;
           if (fast) {
             const int a0 = rl_i(vd, num_v) & 0xFF;
             int cur = a0;
@@ -671,12 +673,12 @@ k_clip_wave(WaveArgs A) {
           vd = (int)pack_dual(vc, vn, p_idx);
           newr = r2;
         }
-        if (__ballot(!ok)) {
+        if (wave_ballot(!ok)) {
           state = CS_ERROR;
           err = E_DET_ZERO;
           break;
         }
-        if (__ballot(mk && isnan(newr))) {
+        if (wave_ballot(mk && isnan(newr))) {
           state = CS_ERROR;
           err = E_NAN;
           break;
@@ -784,7 +786,7 @@ k_clip_wave(WaveArgs A) {
           const double4 ak = S.pa[pk], an = S.pa[pn2];
           lok = project_onto_intersection(xyz(an), an.w, xyz(ak), ak.w, q, &pedge);
         }
-        if (__ballot(!lok)) {
+        if (wave_ballot(!lok)) {
           ok = false;
           break;
         }
@@ -838,7 +840,7 @@ k_clip_wave(WaveArgs A) {
         state = CS_ERROR;
         err = E_DET_ZERO;
       } else {
-        facemask = __ballot(lane < np && cons && has);
+        facemask = wave_ballot(lane < np && cons && has);
         nf = (uint32_t)__popcll(facemask);
         if (lane == 0) {
           // VolumeCentroidIntegral::finalize (integrals.rs:97-105)

@@ -45,6 +45,13 @@ struct GridParams {
 // Ordering the packed value orders by (idx, image) because a generator idx
 // has one slot.  Image code = (i+1)*9 + (j+1)*3 + (k+1) for the shift
 // (i, j, k) * width of the query; 13 = no shift.
+// Lanes of the wavefront whose predicate holds.  The builtin takes the
+// condition itself, so the compare's lane mask is used as it is (HIP's
+// __ballot(int) first turns the condition into an integer and back).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) {
+  return __builtin_amdgcn_ballot_w64(p);
+}
+
 __device__ __forceinline__ uint32_t pk_slot(uint64_t pk) {
   return (uint32_t)(pk >> 5) & 0x7FFFFFFu;
 }

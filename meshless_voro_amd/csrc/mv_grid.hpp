@@ -513,7 +513,7 @@ __device__ __forceinline__ uint32_t knn_gather(
               const uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
               pk = (gidx << 32) | ((uint64_t)t << 5) | img;
             }
-            const uint64_t m = __ballot(keep);
+            const uint64_t m = wave_ballot(keep);
             uint32_t base = cnt;
             if (BLK && m) {
               uint32_t b0 = 0;

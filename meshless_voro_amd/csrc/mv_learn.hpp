@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256)
   // entries it would have tested anyway.
   bool uni = false;
   if constexpr (DENSE) {
-    const bool full = __popcll(__ballot(1)) == 64;
+    const bool full = __popcll(wave_ballot(1)) == 64;
     const uint32_t fp0 = __builtin_amdgcn_readfirstlane(pr);
     const int f0 = __builtin_amdgcn_readfirstlane(C[0]);
     const int f1 = __builtin_amdgcn_readfirstlane(C[1]);
