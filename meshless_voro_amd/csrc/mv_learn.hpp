@@ -61,6 +61,14 @@ struct FeatParams {
 // DENSE = false every other wave; both evaluate the same wave-uniform test
 // and return at once on the other's waves (one launch each: in one kernel
 // the dense path's registers would cost the common path its occupancy).
+//
+// A dense wave whose 64 seeds all lie in one grid cell computes the
+// features of its first seed, the representative, and gives them to all 64
+// (they share the cell, so their counts, spacing and neighbourhood are
+// alike): the lanes scan the representative's candidates 64 at a time,
+// each keeping its own 8 nearest, and the 64 lists are merged by a
+// butterfly.  The features only size the first neighbour lists, so the
+// output of a build does not depend on them.
 template <bool DENSE>
 __global__ void __launch_bounds__(256)
     k_features(const double4 *__restrict__ spos,
@@ -70,14 +78,14 @@ __global__ void __launch_bounds__(256)
                float *__restrict__ hloc) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork) return;
-  const uint32_t slot = work[w];
+  uint32_t slot = work[w];
   const uint32_t pr = problem_of(PR, slot);
   const GridParams &g = PR.grids[pr];
   const BoxParams &box = PR.boxes[pr];
   const FeatParams &fp = fps[pr];
   const uint32_t *cell_start = cell_start_all + PR.coff[pr];
   const double4 qq = spos[slot];
-  const double q[3] = {qq.x, qq.y, qq.z};
+  double q[3] = {qq.x, qq.y, qq.z};
   int C[3], nd = 0;
   double cellvol = 1.;
   for (int c = 0; c < 3; c++) {
@@ -127,7 +135,54 @@ __global__ void __launch_bounds__(256)
     const int f1 = __builtin_amdgcn_readfirstlane(C[1]);
     const int f2 = __builtin_amdgcn_readfirstlane(C[2]);
     uni = __all(full && pr == fp0 && C[0] == f0 && C[1] == f1 && C[2] == f2);
+    if (uni) {
+      // the representative: lane 0's seed (same cell on every lane)
+      slot = (uint32_t)__builtin_amdgcn_readfirstlane((int)slot);
+      for (int c = 0; c < 3; c++) q[c] = __shfl(q[c], 0, 64);
+    }
   }
+  // the 8 smallest of this lane's and every other lane's sorted 8 (with
+  // their offsets when O), by a butterfly, then lane 0's for all lanes so
+  // that equal distances end in one order everywhere
+  auto merge8 = [&](double(&b)[8], double(&bo)[8][3], auto with_off) {
+    constexpr bool O = decltype(with_off)::value;
+    for (int m = 1; m < 64; m <<= 1) {
+      double pb[8], po[8][3];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        pb[i] = __shfl_xor(b[i], m, 64);
+        if constexpr (O)
+          for (int c = 0; c < 3; c++) po[i][c] = __shfl_xor(bo[i][c], m, 64);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        double v = pb[j];
+        double vo[3] = {0., 0., 0.};
+        if constexpr (O)
+          for (int c = 0; c < 3; c++) vo[c] = po[j][c];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          if (v < b[i]) {
+            const double tv = b[i];
+            b[i] = v;
+            v = tv;
+            if constexpr (O)
+              for (int c = 0; c < 3; c++) {
+                const double to = bo[i][c];
+                bo[i][c] = vo[c];
+                vo[c] = to;
+              }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      b[i] = __shfl(b[i], 0, 64);
+      if constexpr (O)
+        for (int c = 0; c < 3; c++) bo[i][c] = __shfl(bo[i][c], 0, 64);
+    }
+  };
   double zlo = -INFINITY, zhi = INFINITY;
   auto widen = [&]() {
     zlo = q[2] - zr;
@@ -161,32 +216,17 @@ __global__ void __launch_bounds__(256)
     t0 = lo;
     t1 = lo2;
   };
-  // fn(p, t) for the entries t of [t0, t1) in index order: in a one-cell
-  // wave (same range on every lane) 64 entries per coalesced load, each
-  // broadcast with readlane; otherwise one load per lane and entry
+  // fn(p, t) for the entries t of [t0, t1): in a one-cell wave each lane
+  // takes every 64th entry (coalesced) for the representative; otherwise
+  // one lane per seed, in index order
   auto each = [&](uint32_t t0, uint32_t t1, auto &&fn) {
     if (DENSE && uni) {
-      const uint32_t lane = __lane_id();
-      for (uint32_t t = t0; t < t1; t += 64) {
-        const uint32_t m = min(64u, t1 - t);
-        double4 mine = make_double4(0., 0., 0., 0.);
-        if (lane < m) mine = spos[t + lane];
-        const uint64_t bx = (uint64_t)__double_as_longlong(mine.x);
-        const uint64_t by = (uint64_t)__double_as_longlong(mine.y);
-        const uint64_t bz = (uint64_t)__double_as_longlong(mine.z);
-        for (uint32_t k = 0; k < m; k++) {
-          auto rd = [&](uint64_t b) {
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
-            return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-          };
-          fn(make_double4(rd(bx), rd(by), rd(bz), 0.), t + k);
-        }
-      }
+      for (uint32_t t = t0 + __lane_id(); t < t1; t += 64) fn(spos[t], t);
     } else {
       for (uint32_t t = t0; t < t1; t++) fn(spos[t], t);
     }
   };
+  double nooff[8][3];
   if (DENSE && dense_lane) {
     double b8[8];
 #pragma unroll
@@ -207,7 +247,8 @@ __global__ void __launch_bounds__(256)
     };
     uint32_t a0 = slot >= s0 + FEAT_WIN / 2 ? slot - FEAT_WIN / 2 : s0;
     if (a0 + FEAT_WIN > s1) a0 = s1 - FEAT_WIN;
-    for (uint32_t t = a0; t < a0 + FEAT_WIN; t++) keep8(spos[t], t);
+    each(a0, a0 + FEAT_WIN, keep8);
+    if (uni) merge8(b8, nooff, std::false_type());
     // |z difference| >= zr gives a squared distance > b8[7] after rounding
     zr = sqrt(b8[7]) * (1. + 1e-6);
     widen();
@@ -218,6 +259,7 @@ __global__ void __launch_bounds__(256)
     uint32_t u0 = s0, u1 = s1;
     zcut(u0, u1);
     each(u0, u1, keep8);
+    if (uni) merge8(b8, nooff, std::false_type());
     zr = sqrt(b8[7]) * (1. + 1e-6);
     widen();
   }
@@ -317,6 +359,8 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
+  if constexpr (DENSE)
+    if (uni) merge8(best, boff, std::true_type());
   double blocks = 1.;
   for (int c = 0; c < nd; c++) blocks *= 3.;
   const double h = (n27 > 0) ? pow(blocks * cellvol / (double)n27, 1.0 / nd)
