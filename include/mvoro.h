@@ -75,7 +75,12 @@ typedef struct mv_options {
   const struct mv_model *model; /* MV_MODE_LEARNED: proposal weights */
   uint32_t clip_blocks_per_cu;  /* one-cell-per-wavefront clip launch:
                                    blocks of 4 wavefronts per CU; 0 = 4 */
-  uint32_t reserved[3];
+  uint32_t coop_max_planes;     /* large-cell kernel: planes and vertices a */
+  uint32_t coop_max_vertices;   /* cell may use there before it moves to the
+                                   wide kernel (device-memory arrays without
+                                   fixed capacity); 0 = the kernel's 255 and
+                                   1024.  Lower values are for testing. */
+  uint32_t reserved;
 } mv_options;
 
 /* Run every cell on the thread-per-cell kernel, which sums each integral in
@@ -218,6 +223,7 @@ typedef struct mv_stats {
   double ms_clip_wave;          /* device time of those clip launches */
   double ms_knn_kernels;        /* device time of the neighbour-list launches */
   double ms_model;              /* features + proposal inference */
+  uint64_t wide_cells;          /* cells moved to the wide kernel */
 } mv_stats;
 void mv_get_stats(const mv_tess *t, mv_stats *s);
 

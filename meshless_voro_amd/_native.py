@@ -48,7 +48,9 @@ class mv_options(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("k0", C.c_uint32), ("k_max", C.c_uint32),
                 ("lanes", C.c_uint32), ("grid_occupancy16", C.c_uint32),
                 ("flags", C.c_uint32), ("model", C.c_void_p),
-                ("clip_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("clip_blocks_per_cu", C.c_uint32),
+                ("coop_max_planes", C.c_uint32), ("coop_max_vertices", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 MV_MODEL_MAGIC = 0x4B4C564D
@@ -90,7 +92,7 @@ class mv_stats(C.Structure):
                 ("clip_wave_cells", C.c_uint64), ("knn_queries", C.c_uint64),
                 ("knn_entries", C.c_uint64),
                 ("ms_clip_wave", C.c_double), ("ms_knn_kernels", C.c_double),
-                ("ms_model", C.c_double)]
+                ("ms_model", C.c_double), ("wide_cells", C.c_uint64)]
 
 
 class mv_device_view(C.Structure):

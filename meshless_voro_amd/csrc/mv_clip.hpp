@@ -57,6 +57,14 @@ struct ClipArgs {
   unsigned long long *exact_counter;
   unsigned long long *overflow;  // bit 0: staging area full, bit 1: cell store full
   CellStore cs;                  // kept cells (cs.pl_a == nullptr: none)
+  // capacities of the large-cell kernel below its arrays' (0: theirs)
+  uint32_t coop_maxp, coop_maxv;
+  // the wide kernel: one slab per block of wmaxp planes and wmaxv vertices
+  double4 *wplA, *wplB, *wfacc, *wvtx;
+  uint64_t *wvdual;
+  uint32_t *wcyc;
+  uint8_t *wflag;
+  uint32_t wmaxp, wmaxv;
 };
 
 constexpr uint64_t BOX_REF = ~0ull;
@@ -69,21 +77,47 @@ enum ErrCode : uint32_t {
   E_WALK = 6
 };
 
-struct Lane {
+// A cell's arrays.  CycT holds SimpleCycle pointers (plane indices) and
+// DualT a vertex's three plane indices: 8 bits each (up to 255 planes) for
+// the kernels with fixed capacity, 21 bits each (up to 2^21 - 1 planes) for
+// the wide kernel, whose capacities are set at launch (maxp, maxv) and
+// whose per-vertex clip decisions go to `flag` instead of a register mask.
+template <typename CycT, typename DualT>
+struct LaneT {
+  using Cyc = CycT;
+  using Dual = DualT;
+  static constexpr bool WIDE = sizeof(DualT) == 8;
   double4 *plA, *plB, *vtx, *facc;
-  uint32_t *vdual;
-  uint8_t *cyc;
+  DualT *vdual;
+  CycT *cyc;
+  uint8_t *flag;
   size_t lane, lanes;
+  int maxp, maxv;
   __device__ __forceinline__ size_t at(int j) const {
     return (size_t)j * lanes + lane;
   }
 };
+using Lane = LaneT<uint8_t, uint32_t>;
+using LaneWide = LaneT<uint32_t, uint64_t>;
 
 __device__ __forceinline__ uint32_t pack_dual(int a, int b, int c) {
   return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16);
 }
 __device__ __forceinline__ int dual_at(uint32_t d, int k) {
   return (int)((d >> (8 * k)) & 0xFFu);
+}
+__device__ __forceinline__ uint64_t pack_dual_wide(int a, int b, int c) {
+  return (uint64_t)a | ((uint64_t)b << 21) | ((uint64_t)c << 42);
+}
+__device__ __forceinline__ int dual_at(uint64_t d, int k) {
+  return (int)((d >> (21 * k)) & 0x1FFFFFull);
+}
+template <class LS>
+__device__ __forceinline__ typename LS::Dual pack_dual_t(int a, int b, int c) {
+  if constexpr (LS::WIDE)
+    return pack_dual_wide(a, b, c);
+  else
+    return pack_dual(a, b, c);
 }
 
 struct CellRegs {
@@ -98,7 +132,8 @@ struct CellRegs {
 };
 
 // Vertex::from_dual (src/voronoi/convex_cell.rs:28-48)
-__device__ __forceinline__ bool make_vertex(const Lane &S, const CellRegs &c,
+template <class LS>
+__device__ __forceinline__ bool make_vertex(const LS &S, const CellRegs &c,
                                             int dim, int i, int j, int k,
                                             double4 *out) {
   double4 a = S.plA[S.at(i)], b = S.plA[S.at(j)], e = S.plA[S.at(k)];
@@ -118,8 +153,8 @@ __device__ __forceinline__ bool make_vertex(const Lane &S, const CellRegs &c,
 }
 
 // update_safety_radius (src/voronoi/convex_cell.rs:443-453)
-__device__ __forceinline__ bool update_safety_radius(const Lane &S,
-                                                     CellRegs &c) {
+template <class LS>
+__device__ __forceinline__ bool update_safety_radius(const LS &S, CellRegs &c) {
   double m = -INFINITY;
   for (int v = 0; v < c.nv; v++) {
     double r = S.vtx[S.at(v)].w;
@@ -131,7 +166,8 @@ __device__ __forceinline__ bool update_safety_radius(const Lane &S,
 }
 
 // HalfSpace::right_loc (src/voronoi/half_space.rs:53-65) of stored plane j
-__device__ __forceinline__ d3 right_loc(const Lane &S, const CellRegs &c,
+template <class LS>
+__device__ __forceinline__ d3 right_loc(const LS &S, const CellRegs &c,
                                         const ClipArgs &A, int j) {
   double4 b = S.plB[S.at(j)];
   uint64_t ref = (uint64_t)__double_as_longlong(b.w);
@@ -148,23 +184,25 @@ __device__ __forceinline__ d3 right_loc(const Lane &S, const CellRegs &c,
 }
 
 // SimpleCycle (src/simple_cycle.rs:1-101) on the lane's pointer array
-__device__ __forceinline__ void cyc_init(const Lane &S, CellRegs &c, int a,
-                                         int b, int e) {
+template <class LS>
+__device__ __forceinline__ void cyc_init(const LS &S, CellRegs &c, int a, int b, int e) {
+  using T = typename LS::Cyc;
   int cur = c.cstart;
   for (int i = 0; i < c.clen; i++) {
     int nx = S.cyc[S.at(cur)];
-    S.cyc[S.at(cur)] = (uint8_t)cur;
+    S.cyc[S.at(cur)] = (T)cur;
     cur = nx;
   }
   c.clen = 3;
   c.cstart = a;
-  S.cyc[S.at(a)] = (uint8_t)b;
-  S.cyc[S.at(b)] = (uint8_t)e;
-  S.cyc[S.at(e)] = (uint8_t)a;
+  S.cyc[S.at(a)] = (T)b;
+  S.cyc[S.at(b)] = (T)e;
+  S.cyc[S.at(e)] = (T)a;
 }
 
-__device__ __forceinline__ bool cyc_try_extend(const Lane &S, CellRegs &c,
-                                               int a, int b, int e) {
+template <class LS>
+__device__ __forceinline__ bool cyc_try_extend(const LS &S, CellRegs &c, int a, int b, int e) {
+  using T = typename LS::Cyc;
   int tri[3] = {a, b, e};
   int pa = S.cyc[S.at(a)], pb = S.cyc[S.at(b)], pe = S.cyc[S.at(e)];
   int ptr[3] = {pa, pb, pe};
@@ -172,14 +210,14 @@ __device__ __forceinline__ bool cyc_try_extend(const Lane &S, CellRegs &c,
   for (int i = 0; i < 3; i++) {
     int j = i == 2 ? 0 : i + 1, k = i == 0 ? 2 : i - 1;
     if (!con[i] && con[j] && con[k] && ptr[k] == tri[j]) {
-      S.cyc[S.at(tri[k])] = (uint8_t)tri[i];
-      S.cyc[S.at(tri[i])] = (uint8_t)tri[j];
+      S.cyc[S.at(tri[k])] = (T)tri[i];
+      S.cyc[S.at(tri[i])] = (T)tri[j];
       c.clen += 1;
       return true;
     } else if (con[i] && con[j] && con[k] && ptr[k] == tri[j] &&
                ptr[j] == tri[i]) {
-      S.cyc[S.at(tri[k])] = (uint8_t)tri[i];
-      S.cyc[S.at(tri[j])] = (uint8_t)tri[j];
+      S.cyc[S.at(tri[k])] = (T)tri[i];
+      S.cyc[S.at(tri[j])] = (T)tri[j];
       if (c.cstart == tri[j]) c.cstart = tri[i];
       c.clen -= 1;
       return true;
@@ -199,19 +237,22 @@ __device__ __forceinline__ double wave_max_all(double v) {
 // Returns 0, E_* for a reference panic, or 255 for capacity overflow.
 // COOP: the wavefront builds one cell; its lanes classify the vertices 64 at
 // a time and every lane then runs the sequential steps on the same data.
-template <int MAXP, int MAXV, bool COOP = false>
-__device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
+template <int MAXP, int MAXV, bool COOP = false, class LS = Lane>
+__device__ uint32_t clip_by_plane(const LS &S, CellRegs &c,
                                   const ClipArgs &A, d3 n, d3 p, double d,
                                   double errb, uint64_t ref, d3 ngb) {
   // Every vertex's decision depends on that vertex alone, so all of them
   // are classified first (independent loads, no stores); the reference's
-  // swap-with-last scan is then replayed on the resulting bit mask, which
-  // moves the same vertices to the same positions.
-  static_assert(MAXV <= 1024, "mask width");
-  constexpr int MW = (MAXV + 63) / 64;
+  // swap-with-last scan is then replayed on the resulting bit mask (the
+  // wide kernel: on per-vertex flags in memory), which moves the same
+  // vertices to the same positions.
+  constexpr bool WIDE = LS::WIDE;
+  static_assert(WIDE || MAXV <= 1024, "mask width");
+  constexpr int MW = WIDE ? 1 : (MAXV + 63) / 64;
   uint64_t m[MW];
 #pragma unroll
   for (int k = 0; k < MW; k++) m[k] = 0;
+  uint64_t anyw = 0;  // WIDE: any vertex clipped
   double keepmax = -INFINITY;
   bool nan_keep = false;
   if constexpr (COOP) {
@@ -224,7 +265,7 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
         const double4 v = S.vtx[S.at(i)];
         const double cv = dot3(n, xyz(v)) - d;
         if (fabs(cv) < errb) {
-          const uint32_t du = S.vdual[S.at(i)];
+          const auto du = S.vdual[S.at(i)];
           int64_t ia[3], ib[3], ic[3], id[3], iv[3];
           iloc(*c.B, c.q, ia);
           iloc(*c.B, right_loc(S, c, A, dual_at(du, 0)), ib);
@@ -238,8 +279,12 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
         }
         kept = !clipped;
         r2 = v.w;
+        if constexpr (WIDE) S.flag[S.at(i)] = clipped ? 1 : 0;
       }
-      m[i0 >> 6] = wave_ballot(clipped);
+      if constexpr (WIDE)
+        anyw |= wave_ballot(clipped);
+      else
+        m[i0 >> 6] = wave_ballot(clipped);
       nan_keep |= wave_ballot(kept && isnan(r2)) != 0ull;
       // kept radii are >= 0; fmax drops NaN as the sequential loop does
       keepmax = fmax(keepmax, wave_max_all(kept && !isnan(r2) ? r2 : -INFINITY));
@@ -251,7 +296,7 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
     bool clipped;
     if (fabs(cv) < errb) {
       // in-sphere test on snapped coordinates (convex_cell.rs:370-382)
-      uint32_t du = S.vdual[S.at(i)];
+      const auto du = S.vdual[S.at(i)];
       int64_t ia[3], ib[3], ic[3], id[3], iv[3];
       iloc(*c.B, c.q, ia);
       iloc(*c.B, right_loc(S, c, A, dual_at(du, 0)), ib);
@@ -264,34 +309,61 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
       clipped = cv < 0.;  // NaN keeps the vertex, as in the reference
     }
     if (clipped) {
-      m[i >> 6] |= 1ull << (i & 63);
+      if constexpr (WIDE) {
+        S.flag[S.at(i)] = 1;
+        anyw = 1;
+      } else {
+        m[i >> 6] |= 1ull << (i & 63);
+      }
     } else {
+      if constexpr (WIDE) S.flag[S.at(i)] = 0;
       nan_keep |= isnan(v.w);
       keepmax = fmax(keepmax, v.w);
     }
   }
-  uint64_t any = 0;
+  if constexpr (WIDE && COOP) {
+    // every lane reads the flags the other lanes wrote (one workgroup of
+    // one wavefront: a workgroup fence makes them visible)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  }
+  uint64_t any = anyw;
 #pragma unroll
   for (int k = 0; k < MW; k++) any |= m[k];
   if (!any) return 0;
   if (nan_keep) return E_NAN;
   // replay of the swap-with-last scan (convex_cell.rs:365-390)
+  auto isclip = [&](int j) -> bool {
+    if constexpr (WIDE)
+      return S.flag[S.at(j)] != 0;
+    else
+      return (m[j >> 6] >> (j & 63)) & 1ull;
+  };
+  auto setclip = [&](int j, bool v) {
+    if constexpr (WIDE) {
+      S.flag[S.at(j)] = v ? 1 : 0;
+    } else {
+      if (v)
+        m[j >> 6] |= 1ull << (j & 63);
+      else
+        m[j >> 6] &= ~(1ull << (j & 63));
+    }
+  };
   int i = 0, num_v = c.nv;
   while (i < num_v) {
-    bool bi = (m[i >> 6] >> (i & 63)) & 1ull;
+    bool bi = isclip(i);
     if (bi) {
       num_v -= 1;
-      bool bn = (m[num_v >> 6] >> (num_v & 63)) & 1ull;
+      bool bn = isclip(num_v);
       if (num_v != i) {
         // position i takes the last element; the clipped vertex's data is
         // dead apart from its dual (used by the boundary walk)
         S.vtx[S.at(i)] = S.vtx[S.at(num_v)];
-        uint32_t di = S.vdual[S.at(i)];
+        const auto di = S.vdual[S.at(i)];
         S.vdual[S.at(i)] = S.vdual[S.at(num_v)];
         S.vdual[S.at(num_v)] = di;
         if (!bn) {
-          m[i >> 6] &= ~(1ull << (i & 63));
-          m[num_v >> 6] |= 1ull << (num_v & 63);
+          setclip(i, false);
+          setclip(num_v, true);
         }
       }
     } else {
@@ -299,21 +371,21 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
     }
   }
   // push the plane and grow the cycle
-  if (c.np >= MAXP) return 255;
+  if (c.np >= S.maxp) return 255;
   const int p_idx = c.np;
   S.plA[S.at(p_idx)] = make_double4(n.x, n.y, n.z, d);
   S.plB[S.at(p_idx)] = make_double4(p.x, p.y, p.z, __longlong_as_double((long long)ref));
-  S.cyc[S.at(p_idx)] = (uint8_t)p_idx;
+  S.cyc[S.at(p_idx)] = (typename LS::Cyc)p_idx;
   c.np += 1;
   // compute_boundary (convex_cell.rs:421-441) over the removed vertices
   {
-    uint32_t d0 = S.vdual[S.at(num_v)];
+    const auto d0 = S.vdual[S.at(num_v)];
     cyc_init(S, c, dual_at(d0, 0), dual_at(d0, 1), dual_at(d0, 2));
     for (int r = num_v + 1; r < c.nv; r++) {
       int idx = r;
       for (;;) {
         if (idx >= c.nv) return E_WALK;
-        uint32_t du = S.vdual[S.at(idx)];
+        const auto du = S.vdual[S.at(idx)];
         if (cyc_try_extend(S, c, dual_at(du, 0), dual_at(du, 1),
                            dual_at(du, 2))) {
           if (idx > r) {
@@ -327,7 +399,7 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
     }
   }
   c.nv = num_v;
-  if (c.nv + c.clen > MAXV) return 255;
+  if (c.nv + c.clen > S.maxv) return 255;
   int cur = c.cstart;
   double newmax = keepmax;
   for (int t = 0; t < c.clen; t++) {
@@ -338,7 +410,7 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
     if (isnan(nvtx.w)) return E_NAN;
     newmax = fmax(newmax, nvtx.w);
     S.vtx[S.at(c.nv)] = nvtx;
-    S.vdual[S.at(c.nv)] = pack_dual(cur, next, p_idx);
+    S.vdual[S.at(c.nv)] = pack_dual_t<LS>(cur, next, p_idx);
     c.nv++;
     cur = next;
   }
@@ -349,8 +421,8 @@ __device__ uint32_t clip_by_plane(const Lane &S, CellRegs &c,
 }
 
 // should a face be built for plane j (voronoi_cell.rs:53-73)
-__device__ __forceinline__ bool should_construct(const Lane &S,
-                                                 const CellRegs &c,
+template <class LS>
+__device__ __forceinline__ bool should_construct(const LS &S, const CellRegs &c,
                                                  const ClipArgs &A, int j) {
   double4 a = S.plA[S.at(j)];
   int dim = c.B->dim;
@@ -371,7 +443,8 @@ __device__ __forceinline__ bool should_construct(const Lane &S,
 // cell as it is, so skipping them is exact.  Lane l tests entry j + l
 // against every vertex (the plane as build_cell computes it, the vertex
 // test as clip_by_plane's), 64 entries at a time.  Returns cnt if none.
-__device__ __forceinline__ uint32_t coop_scan(const Lane &S, const CellRegs &c,
+template <class LS>
+__device__ __forceinline__ uint32_t coop_scan(const LS &S, const CellRegs &c,
                                               const ClipArgs &A, const uint64_t *list,
                                               uint32_t j, uint32_t cnt) {
   const BoxParams &B = *c.B;
@@ -409,10 +482,11 @@ __device__ __forceinline__ uint32_t coop_scan(const Lane &S, const CellRegs &c,
 }
 
 // Runs one cell.  Returns its CellState; on CS_DONE, *nf = faces to emit.
-template <int MAXP, int MAXV, bool COOP = false>
-__device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
+template <int MAXP, int MAXV, bool COOP = false, class LS = Lane>
+__device__ uint8_t build_cell(const LS &S, CellRegs &c, const ClipArgs &A,
                               uint32_t w, uint32_t *nf, uint32_t *npop,
                               uint32_t *err) {
+  using T = typename LS::Cyc;
   const BoxParams &B = *c.B;
   *nf = 0;
   *npop = 0;
@@ -421,7 +495,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
     S.plA[S.at(k)] = make_double4(B.pn[k][0], B.pn[k][1], B.pn[k][2], B.pd[k]);
     S.plB[S.at(k)] = make_double4(B.pp[k][0], B.pp[k][1], B.pp[k][2],
                                   __longlong_as_double((long long)BOX_REF));
-    S.cyc[S.at(k)] = (uint8_t)k;
+    S.cyc[S.at(k)] = (T)k;
   }
   c.np = 6;
   c.cstart = 0;
@@ -437,7 +511,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
       return CS_ERROR;
     }
     S.vtx[S.at(v)] = nv;
-    S.vdual[S.at(v)] = pack_dual(D[v][0], D[v][1], D[v][2]);
+    S.vdual[S.at(v)] = pack_dual_t<LS>(D[v][0], D[v][1], D[v][2]);
   }
   if (!update_safety_radius(S, c)) {
     *err = E_NAN;
@@ -481,7 +555,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
     // HalfSpace::new (half_space.rs:24-34)
     double errb = 1e-13 * (1. + dot3(abs3(n), abs3(p)));
     double d = dot3(n, p);
-    uint32_t r = clip_by_plane<MAXP, MAXV, COOP>(S, c, A, n, p, d, errb, pk, ngb);
+    uint32_t r = clip_by_plane<MAXP, MAXV, COOP, LS>(S, c, A, n, p, d, errb, pk, ngb);
     if (r == 255) return CS_CAPACITY;
     if (r) {
       *err = r;
@@ -506,7 +580,7 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
   d3 cen = mk3(0, 0, 0);
   const d3 q = c.q;
   for (int vi = 0; vi < c.nv; vi++) {
-    const uint32_t du = S.vdual[S.at(vi)];
+    const auto du = S.vdual[S.at(vi)];
     const d3 vl = xyz(S.vtx[S.at(vi)]);
     // projections P_t of DecompositionWithoutFaces::load_vertex, streamed:
     // P_{2k} = q projected on plane dual[k], P_{2k+1} = q projected on the
@@ -573,8 +647,10 @@ __device__ uint8_t build_cell(const Lane &S, CellRegs &c, const ClipArgs &A,
 
 // Append a completed cell's half-spaces and vertices to the cell store
 // (wave-aggregated cursors; every lane of the wavefront calls this).
-__device__ __forceinline__ void keep_cell(const Lane &S, const ClipArgs &A,
+template <class LS>
+__device__ __forceinline__ void keep_cell(const LS &S, const ClipArgs &A,
                                           bool emit, const CellRegs &c) {
+  static_assert(!LS::WIDE, "kept cells store 8-bit plane indices");
   const uint32_t lane = __lane_id();
   const uint32_t np = emit ? (uint32_t)c.np : 0u, nv = emit ? (uint32_t)c.nv : 0u;
   uint32_t tp, tv;
@@ -618,6 +694,9 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
   S.facc = A.facc;
   S.vdual = A.vdual;
   S.cyc = A.cyc;
+  S.flag = nullptr;
+  S.maxp = MAXP;
+  S.maxv = MAXV;
   S.lane = gl;
   S.lanes = A.lanes;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -707,12 +786,75 @@ struct CoopLds {
 // (the lanes store the same values); the vertex classification of each
 // plane is split over the lanes.  Results, faces and counters are written
 // by lane 0.
+// One cell w on the wavefront (every lane runs the same steps; lane 0
+// writes the results).
+template <int MAXP, int MAXV, bool KEEP, class LS>
+__device__ __forceinline__ void coop_cell(const LS &S, const ClipArgs &A, uint32_t w,
+                                          unsigned long long &my_exact) {
+  const uint32_t lane = __lane_id();
+  uint32_t nf = 0;
+  CellRegs c;
+  const uint32_t slot = A.work[w];
+  const uint32_t pr = problem_of(A.PR, slot);
+  c.B = &A.PR.boxes[pr];
+  c.total_stream = A.PR.grids[pr].total_stream;
+  const double4 qq = A.spos[slot];
+  c.q = xyz(qq);
+  c.idx = (uint64_t)__double_as_longlong(qq.w);
+  uint32_t err = 0, npop = 0;
+  const uint8_t st = build_cell<MAXP, MAXV, true, LS>(S, c, A, w, &nf, &npop, &err);
+  my_exact += c.nexact;
+  const bool emit = st == CS_DONE;
+  if (lane == 0) {
+    A.wstate[w] = st;
+    if (st == CS_NEED_MORE) A.need_r[w] = c.sr;
+    if (st == CS_ERROR) atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
+    if (emit) A.cell_npop[c.idx] = npop;
+  }
+  if constexpr (KEEP) keep_cell(S, A, emit && lane == 0, c);
+  if (emit && lane == 0) {
+    const unsigned long long fb = nf ? atomicAdd(A.st_counter, (unsigned long long)nf) : 0ull;
+    A.cell_nf[c.idx] = nf;
+    A.cell_fbase[c.idx] = fb;
+    if (fb + nf > A.st_cap) {
+      atomicOr(A.overflow, 1ull);
+    } else {
+      uint64_t o = fb;
+      for (int k = 0; k < c.np; k++) {
+        if (S.cyc[S.at(k)] != 3) continue;
+        const double4 a = S.plA[S.at(k)];
+        const uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(k)].w);
+        const double4 fa = S.facc[S.at(k)];
+        const double fnrm = fa.x > 0. ? 1. / (3. * fa.x) : 0.;
+        A.st_cell[o] = (uint32_t)c.idx;
+        A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
+        // a face without periodic shift is also a link of its right cell
+        if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
+        A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
+        A.st_area[o] = fa.x;
+        A.st_cen[3 * o] = fa.y * fnrm;
+        A.st_cen[3 * o + 1] = fa.z * fnrm;
+        A.st_cen[3 * o + 2] = fa.w * fnrm;
+        A.st_norm[3 * o] = a.x;
+        A.st_norm[3 * o + 1] = a.y;
+        A.st_norm[3 * o + 2] = a.z;
+        o++;
+      }
+    }
+  }
+}
+
+// One cell per wavefront (a block of one wavefront), for the cells over
+// the wavefront kernel's 64 vertices or planes (long lists, large cells).
+// Every lane runs the same sequential steps on the cell's arrays in LDS
+// (the lanes store the same values); the vertex classification of each
+// plane is split over the lanes.  Results, faces and counters are written
+// by lane 0.  Cells over A.coop_maxp planes or A.coop_maxv vertices (at
+// most the arrays' MAXP, MAXV) report CS_CAPACITY for the wide kernel.
 template <int MAXP, int MAXV, bool KEEP>
 __global__ void __launch_bounds__(64) k_clip_coop(ClipArgs A) {
   __shared__ CoopLds<MAXP, MAXV> L;
   const uint32_t lane = __lane_id();
-  const uint32_t wid = blockIdx.x;
-  const uint32_t nwaves = gridDim.x;
   Lane S;
   S.plA = L.plA;
   S.plB = L.plB;
@@ -720,62 +862,42 @@ __global__ void __launch_bounds__(64) k_clip_coop(ClipArgs A) {
   S.facc = L.facc;
   S.vdual = L.vdual;
   S.cyc = L.cyc;
+  S.flag = nullptr;
+  S.maxp = A.coop_maxp && A.coop_maxp < (uint32_t)MAXP ? (int)A.coop_maxp : MAXP;
+  S.maxv = A.coop_maxv && A.coop_maxv < (uint32_t)MAXV ? (int)A.coop_maxv : MAXV;
   S.lane = 0;
   S.lanes = 1;
   unsigned long long my_exact = 0;
-  for (uint32_t w = wid; w < A.nwork; w += nwaves) {
-    uint32_t nf = 0;
-    CellRegs c;
-    const uint32_t slot = A.work[w];
-    const uint32_t pr = problem_of(A.PR, slot);
-    c.B = &A.PR.boxes[pr];
-    c.total_stream = A.PR.grids[pr].total_stream;
-    const double4 qq = A.spos[slot];
-    c.q = xyz(qq);
-    c.idx = (uint64_t)__double_as_longlong(qq.w);
-    uint32_t err = 0, npop = 0;
-    const uint8_t st = build_cell<MAXP, MAXV, true>(S, c, A, w, &nf, &npop, &err);
-    my_exact += c.nexact;
-    const bool emit = st == CS_DONE;
-    if (lane == 0) {
-      A.wstate[w] = st;
-      if (st == CS_NEED_MORE) A.need_r[w] = c.sr;
-      if (st == CS_ERROR) atomicMin(A.err_word, (unsigned long long)((c.idx << 8) | err));
-      if (emit) A.cell_npop[c.idx] = npop;
-    }
-    if (KEEP) keep_cell(S, A, emit && lane == 0, c);
-    if (emit && lane == 0) {
-      const unsigned long long fb = nf ? atomicAdd(A.st_counter, (unsigned long long)nf) : 0ull;
-      A.cell_nf[c.idx] = nf;
-      A.cell_fbase[c.idx] = fb;
-      if (fb + nf > A.st_cap) {
-        atomicOr(A.overflow, 1ull);
-      } else {
-        uint64_t o = fb;
-        for (int k = 0; k < c.np; k++) {
-          if (S.cyc[S.at(k)] != 3) continue;
-          const double4 a = S.plA[S.at(k)];
-          const uint64_t ref = (uint64_t)__double_as_longlong(S.plB[S.at(k)].w);
-          const double4 fa = S.facc[S.at(k)];
-          const double fnrm = fa.x > 0. ? 1. / (3. * fa.x) : 0.;
-          A.st_cell[o] = (uint32_t)c.idx;
-          A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
-          // a face without periodic shift is also a link of its right cell
-          if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
-          A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
-          A.st_area[o] = fa.x;
-          A.st_cen[3 * o] = fa.y * fnrm;
-          A.st_cen[3 * o + 1] = fa.z * fnrm;
-          A.st_cen[3 * o + 2] = fa.w * fnrm;
-          A.st_norm[3 * o] = a.x;
-          A.st_norm[3 * o + 1] = a.y;
-          A.st_norm[3 * o + 2] = a.z;
-          o++;
-        }
-      }
-    }
-  }
+  for (uint32_t w = blockIdx.x; w < A.nwork; w += gridDim.x)
+    coop_cell<MAXP, MAXV, KEEP>(S, A, w, my_exact);
   // every lane counted every exact test of its vertices
+  for (int o = 32; o > 0; o >>= 1) my_exact += __shfl_xor(my_exact, o, 64);
+  if (lane == 0 && my_exact) atomicAdd(A.exact_counter, my_exact);
+}
+
+// The wide kernel: the large-cell kernel's steps on arrays in device
+// memory, one slab per block (A.wmaxp planes, A.wmaxv vertices, set by the
+// host and doubled when a cell reports CS_CAPACITY), with plane indices of
+// 21 bits.  It takes the cells over the large-cell kernel's capacity, which
+// the reference grows without bound (convex_cell.rs:394-418).
+__global__ void __launch_bounds__(64) k_clip_wide(ClipArgs A) {
+  const uint32_t lane = __lane_id();
+  const size_t b = blockIdx.x;
+  LaneWide S;
+  S.plA = A.wplA + b * A.wmaxp;
+  S.plB = A.wplB + b * A.wmaxp;
+  S.facc = A.wfacc + b * A.wmaxp;
+  S.cyc = A.wcyc + b * A.wmaxp;
+  S.vtx = A.wvtx + b * A.wmaxv;
+  S.vdual = A.wvdual + b * A.wmaxv;
+  S.flag = A.wflag + b * A.wmaxv;
+  S.maxp = (int)A.wmaxp;
+  S.maxv = (int)A.wmaxv;
+  S.lane = 0;
+  S.lanes = 1;
+  unsigned long long my_exact = 0;
+  for (uint32_t w = blockIdx.x; w < A.nwork; w += gridDim.x)
+    coop_cell<0, 0, false>(S, A, w, my_exact);
   for (int o = 32; o > 0; o >>= 1) my_exact += __shfl_xor(my_exact, o, 64);
   if (lane == 0 && my_exact) atomicAdd(A.exact_counter, my_exact);
 }

@@ -654,6 +654,11 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
 // power of two, at least 65,536); at most 8 GiB of such buffers are in
 // flight
 constexpr uint32_t BIG_KNN_CAP = 65536;
+// wide kernel: starting capacities per cell (planes, vertices), their
+// limit (plane indices have 21 bits) and the blocks of one launch
+constexpr uint32_t WIDE_P0 = 1024, WIDE_V0 = 4096;
+constexpr uint32_t WIDE_P_MAX = 1u << 20;
+constexpr uint32_t WIDE_BLOCKS = 256;
 constexpr uint32_t BIG_KNN_BLOCKS = 1024;
 constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
 
@@ -973,9 +978,9 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   uint32_t *ident = need(TT.get<uint32_t>(n));
   uint32_t *pos = need(TT.get<uint32_t>(n));
   // [0]: first-pass list, [1]/[2]: main lists of later passes, [3]/[4]:
-  // lists of the big (global-memory) variants
-  uint32_t *lists[5], *lens[5];
-  for (int k = 0; k < 5; k++) {
+  // lists of the large-cell kernel, [5]/[6]: lists of the wide kernel
+  uint32_t *lists[7], *lens[7];
+  for (int k = 0; k < 7; k++) {
     lists[k] = need(TT.get<uint32_t>(n));
     lens[k] = need(TT.get<uint32_t>(n));
   }
@@ -1090,6 +1095,10 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   }
   uint8_t *wstate = need(TT.get<uint8_t>(n));
   uint8_t *bstate = need(TT.get<uint8_t>(n));
+  uint8_t *xstate = need(TT.get<uint8_t>(n));
+  // wide-kernel capacities (planes, vertices per cell), doubled when a
+  // cell runs out
+  uint32_t wide_p = WIDE_P0, wide_v = WIDE_V0;
   // work-queue heads of the two wave clip launches and the count of cells
   // handed from the first to the second
   unsigned int *queue = need(TT.get<unsigned int>(3));
@@ -1115,7 +1124,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   };
   LaneBufs LM{};
   bool have_main = false;
-  uint64_t redone = 0, bigcells = 0;
+  uint64_t redone = 0, bigcells = 0, widecells = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
   // per-kernel figures (reset with each attempt)
@@ -1143,13 +1152,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
 
     // the first pass reads lists[0], which is never written, so that an
     // attempt can start over; later passes alternate between two buffers
-    uint32_t nw = nwork0, nbig = 0;
+    uint32_t nw = nwork0, nbig = 0, nwide = 0;
     uint32_t *cw = lists[0], *cl = lens[0], *cb = lists[3], *cbl = lens[3];
-    int ping_m = 1, ping_b = 4;
+    uint32_t *cx = lists[5], *cxl = lens[5];
+    int ping_m = 1, ping_b = 4, ping_x = 6;
     // bounds on the list lengths of the main and big lists (all clamped to
     // kmax; a list whose lower bound is kmax cannot grow any more)
-    uint32_t lb_main = std::min(lbound, kmax), lb_big = lb_main;
-    const float *Rbig = nullptr;
+    uint32_t lb_main = std::min(lbound, kmax), lb_big = lb_main, lb_wide = lb_main;
+    const float *Rbig = nullptr, *Rwide = nullptr;
     HIPCHK(hipMemsetAsync(cap_err, 0, sizeof(unsigned int), s));
     const float *Rmain = Rfirst;
     bool overflowed = false;
@@ -1157,18 +1167,24 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     passes = 0;
     redone = 0;
     bigcells = 0;
+    widecells = 0;
     ms_knn_k = ms_cw = 0;
     n_knn = n_cw = 0;
     q_knn = cells_cw = e_knn = 0;
-    while (nw + nbig > 0) {
+    while (nw + nbig + nwide > 0) {
       passes++;
       double *nr_main = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *nr_big = need(TT.get<double>(std::max<uint32_t>(nbig, 1)));
-      const double *kd_main = nullptr, *kd_big = nullptr;  // this pass's kdist
+      double *nr_wide = need(TT.get<double>(std::max<uint32_t>(nwide, 1)));
+      // this pass's kdist per list
+      const double *kd_main = nullptr, *kd_big = nullptr, *kd_wide = nullptr;
       // one list on one stream; untimed runs do not wait for their kernels
+      // tier 0: the main list (wavefront kernel), 1: the large-cell kernel,
+      // 2: the wide kernel; tiers 1 and 2 gather with the long-list kernel
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
-                     uint8_t *state, bool big, hipStream_t s, bool timed) {
-        double *need_r = big ? nr_big : nr_main;
+                     uint8_t *state, int tier, hipStream_t s, bool timed) {
+        const bool big = tier > 0;
+        double *need_r = tier == 2 ? nr_wide : big ? nr_big : nr_main;
         Timer tk(s, timed);
         uint64_t *kofs = need(TT.get<uint64_t>(cnt));
         // redone lists: the longest one picks the list kernel (the growth
@@ -1185,7 +1201,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         uint64_t *kpk = need(TT.get<uint64_t>(total));
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
-        (big ? kd_big : kd_main) = kdist;
+        (tier == 2 ? kd_wide : big ? kd_big : kd_main) = kdist;
         Timer tkk(s, timed);
         launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
                    s);
@@ -1307,6 +1323,25 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.exact_counter = counters + 2;
         A.overflow = counters + 3;
         if (T->keep) A.cs = T->cs;
+        A.coop_maxp = opt.coop_max_planes;
+        A.coop_maxv = opt.coop_max_vertices;
+        if (tier == 2) {
+          // one slab per block, its planes and vertices sized for the cells
+          const uint32_t blocks = std::min<uint32_t>(cnt, WIDE_BLOCKS);
+          A.wmaxp = wide_p;
+          A.wmaxv = wide_v;
+          A.wplA = need(TT.get<double4>((size_t)blocks * wide_p));
+          A.wplB = need(TT.get<double4>((size_t)blocks * wide_p));
+          A.wfacc = need(TT.get<double4>((size_t)blocks * wide_p));
+          A.wcyc = need(TT.get<uint32_t>((size_t)blocks * wide_p));
+          A.wvtx = need(TT.get<double4>((size_t)blocks * wide_v));
+          A.wvdual = need(TT.get<uint64_t>((size_t)blocks * wide_v));
+          A.wflag = need(TT.get<uint8_t>((size_t)blocks * wide_v));
+          hipLaunchKernelGGL(k_clip_wide, dim3(blocks), dim3(64), 0, s, A);
+          HIPCHK(hipGetLastError());
+          ms_clip += tc.stop();
+          return;
+        }
         const dim3 grid(lanes / 256);
         // large cells: one cell per wavefront, a block each
         const dim3 grid_coop(std::max<uint32_t>(lanes, 1));
@@ -1330,10 +1365,15 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         HIPCHK(hipEventRecord(st2.ev, s));
         HIPCHK(hipStreamWaitEvent(s2, st2.ev, 0));
       }
-      Timer tbig(s2, nbig > 0);
-      if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, true, s2, false);
-      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, false, s, true);
-      if (nbig) ms_clip += tbig.stop();
+      if (nwide && !nbig && s2 != s) {
+        HIPCHK(hipEventRecord(st2.ev, s));
+        HIPCHK(hipStreamWaitEvent(s2, st2.ev, 0));
+      }
+      Timer tbig(s2, nbig + nwide > 0);
+      if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, 1, s2, false);
+      if (nwide) run(cx, cxl, nwide, lb_wide, Rwide, xstate, 2, s2, false);
+      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, 0, s, true);
+      if (nbig + nwide) ms_clip += tbig.stop();
       HIPCHK(hipStreamSynchronize(s2));
       unsigned long long cnts[5];
       hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err, host_word());
@@ -1377,21 +1417,37 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       };
       uint32_t *cw2 = lists[ping_m], *cl2 = lens[ping_m];
       uint32_t *cb2 = lists[ping_b], *cbl2 = lens[ping_b];
+      uint32_t *cx2 = lists[ping_x], *cxl2 = lens[ping_x];
       double *rr_m = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *rr_b = need(TT.get<double>(std::max<uint32_t>(nw + nbig, 1)));
+      double *rr_x = need(TT.get<double>(std::max<uint32_t>(nwide, 1)));
       double *kk_m = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *kk_b = need(TT.get<double>(std::max<uint32_t>(nw + nbig, 1)));
+      double *kk_x = need(TT.get<double>(std::max<uint32_t>(nwide, 1)));
       const uint32_t nw_next =
           take(cw, cl, wstate, nw, CS_NEED_MORE, cw2, cl2, nr_main, rr_m, kd_main, kk_m);
       const uint32_t nb_next =
           take(cb, cbl, bstate, nbig, CS_NEED_MORE, cb2, cbl2, nr_big, rr_b, kd_big, kk_b);
+      const uint32_t nx_next =
+          take(cx, cxl, xstate, nwide, CS_NEED_MORE, cx2, cxl2, nr_wide, rr_x, kd_wide, kk_x);
       const uint32_t ncap = take(cw, cl, wstate, nw, CS_CAPACITY, cb2 + nb_next, cbl2 + nb_next,
                                  nullptr, nullptr, nullptr, nullptr);
-      const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, cw2 + nw_next,
-                                  cl2 + nw_next, nullptr, nullptr, nullptr, nullptr);
-      if (nbcap) throw BuildError{MV_E_CAPACITY};
+      // cells over the large-cell kernel's capacity go to the wide kernel
+      // (kept-cell stores hold 8-bit plane indices: not for them)
+      const uint32_t nbcap = take(cb, cbl, bstate, nbig, CS_CAPACITY, cx2 + nx_next,
+                                  cxl2 + nx_next, nullptr, nullptr, nullptr, nullptr);
+      if (nbcap && T->keep) throw BuildError{MV_E_CAPACITY};
+      // cells over the wide kernel's capacity run again with twice as much
+      const uint32_t nxcap = take(cx, cxl, xstate, nwide, CS_CAPACITY, cx2 + nx_next + nbcap,
+                                  cxl2 + nx_next + nbcap, nullptr, nullptr, nullptr, nullptr);
+      if (nxcap) {
+        if (wide_p >= WIDE_P_MAX) throw BuildError{MV_E_CAPACITY};
+        wide_p *= 2;
+        wide_v *= 2;
+      }
       float *Rm_next = need(TT.get<float>(std::max<uint32_t>(nw_next, 1)));
       float *Rb_next = need(TT.get<float>(std::max<uint32_t>(nb_next + ncap, 1)));
+      float *Rx_next = need(TT.get<float>(std::max<uint32_t>(nx_next + nbcap + nxcap, 1)));
       // 2x to 4x per pass, from the safety radius at exhaustion
       constexpr uint32_t GROW = 4;
       if (nw_next)
@@ -1400,30 +1456,44 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
                            (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err);
+      if (nx_next)
+        hipLaunchKernelGGL(k_redo_size, dim3(nblk(nx_next)), dim3(256), 0, s, nx_next, cxl2,
+                           (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err);
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
+      if (nbcap + nxcap)
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nbcap + nxcap)), dim3(256), 0, s,
+                           (uint32_t *)(Rx_next + nx_next), nbcap + nxcap, 0u);
       HIPCHK(hipGetLastError());
       bigcells += ncap;
-      redone += nw_next + nb_next;
+      widecells += nbcap;
+      redone += nw_next + nb_next + nx_next;
       if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
       const uint32_t lb_main_old = lb_main;
       // bounds on the new lengths (k_redo_size grows them at most GROW x)
       auto grown = [&](uint32_t lb) {
         return (uint32_t)std::min<uint64_t>((uint64_t)GROW * lb, kmax);
       };
+      const uint32_t lb_big_old = lb_big;
       lb_main = grown(lb_main);
       lb_big = std::max(grown(lb_big), std::min(lb_main_old, kmax));
+      lb_wide = std::max(grown(lb_wide), std::min(lb_big_old, kmax));
       Rmain = Rm_next;
       Rbig = Rb_next;
+      Rwide = Rx_next;
       nw = nw_next;
       nbig = nb_next + ncap;
+      nwide = nx_next + nbcap + nxcap;
       cw = cw2;
       cl = cl2;
       cb = cb2;
       cbl = cbl2;
+      cx = cx2;
+      cxl = cxl2;
       ping_m = 3 - ping_m;
       ping_b = 7 - ping_b;
+      ping_x = 11 - ping_x;
     }
     if (!overflowed) {
       unsigned long long cnts[4];
@@ -1452,6 +1522,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->stats.passes = passes;
   T->stats.cells_redone = redone;
   T->stats.big_cells = bigcells;
+  T->stats.wide_cells = widecells;
   T->stats.ms_knn = ms_knn + ms_model;
   T->stats.ms_clip = ms_clip;
   T->stats.clip_wave_launches = n_cw;

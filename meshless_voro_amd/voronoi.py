@@ -57,12 +57,16 @@ class BuildOptions:
 
     def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 4194304,
                  lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
-                 model=None, clip_blocks_per_cu: int = 0, length_order: bool = True):
+                 model=None, clip_blocks_per_cu: int = 0, length_order: bool = True,
+                 coop_max_planes: int = 0, coop_max_vertices: int = 0):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
         self.exact_order = exact_order
         self.clip_blocks_per_cu = clip_blocks_per_cu
         self.length_order = length_order
+        # capacities of the large-cell kernel before a cell moves to the wide
+        # kernel (0: the kernel's own; lower values force the wide path)
+        self.coop_max_planes, self.coop_max_vertices = coop_max_planes, coop_max_vertices
         # learned mode: a meshless_voro_amd.learned.Model (None: the
         # committed checkpoint)
         self.model = model
@@ -75,6 +79,8 @@ class BuildOptions:
         o.flags = (_native.MV_FLAG_EXACT_ORDER if self.exact_order else 0) | (
             0 if self.length_order else _native.MV_FLAG_NO_LENGTH_ORDER)
         o.clip_blocks_per_cu = self.clip_blocks_per_cu
+        o.coop_max_planes = self.coop_max_planes
+        o.coop_max_vertices = self.coop_max_vertices
         if self.mode == MODE_LEARNED:
             from . import learned
             m = self.model if self.model is not None else learned.default_model()
