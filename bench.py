@@ -108,7 +108,8 @@ def batched_c5(L, mode, reps=3, only_uniform=False):
         [W.partial_mask(n, 0.1, 1000 + i) for i, n in enumerate(sizes)]).astype(np.uint8))
     xt = torch.from_numpy(xyz).to("cuda:0")
     mt = torch.from_numpy(mask).to("cuda:0")
-    opt = BuildOptions(mode=mode).to_c()
+    from meshless_voro_amd.autotune import tuned_options
+    opt = tuned_options(int(sum(sizes)), True, mode, kind="batched").to_c()
     hs = (C.c_void_p * P)()
 
     def batched(use_mask):
@@ -186,7 +187,7 @@ def clustered_4m(L, modes, reps=2):
     res, dig = {}, {}
     for name, mode in modes:
         print(f"bench: clustered 4M, mode {name}", file=sys.stderr, flush=True)
-        h, _ = build_device(L, xt, n, mode)
+        h, _ = build_device(L, xt, n, mode, kind="clustered")
         dig[name] = connectivity_digest(L, h)
         L.mv_free(h)
         torch.cuda.synchronize()
@@ -194,7 +195,7 @@ def clustered_4m(L, modes, reps=2):
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            h, st = build_device(L, xt, n, mode)
+            h, st = build_device(L, xt, n, mode, kind="clustered")
             L.mv_free(h)
         e1.record()
         torch.cuda.synchronize()
@@ -250,7 +251,7 @@ def host_to_host(L, x, modes, reps=3):
     return res
 
 
-def build_device(L, xt, n, mode, periodic=True):
+def build_device(L, xt, n, mode, periodic=True, kind="uniform"):
     from meshless_voro_amd import _native
     from meshless_voro_amd.voronoi import BuildOptions, VoronoiError
     import numpy as np
@@ -258,7 +259,7 @@ def build_device(L, xt, n, mode, periodic=True):
     a = np.zeros(3)
     w = np.ones(3)
     # launch and layout parameters from the autotuner's table (defaults if none)
-    opt = tuned_options(n, periodic, mode).to_c()
+    opt = tuned_options(n, periodic, mode, kind=kind).to_c()
     h = C.c_void_p()
     st = L.mv_build_device(C.c_void_p(xt.data_ptr()), n, None, a.ctypes.data, w.ctypes.data,
                            3, int(periodic), C.byref(opt), C.byref(h))

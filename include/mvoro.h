@@ -80,6 +80,12 @@ typedef struct mv_options {
                                    wide kernel (device-memory arrays without
                                    fixed capacity); 0 = the kernel's 255 and
                                    1024.  Lower values are for testing. */
+  uint32_t cells_per_wave;      /* main lists: 1 cell per wavefront (0 or 1,
+                                   the default) or 64 (one cell per lane) */
+  uint32_t first_list_cap;      /* learned mode: longest first-pass list,
+                                   multiple of 8 in [16, 1024]; 0 = 256 */
+  uint32_t knn_min_tile;        /* neighbour gathers: smallest LDS tile
+                                   (512, 1024 or 2048 entries); 0 = 512 */
   uint32_t reserved;
 } mv_options;
 

@@ -662,12 +662,16 @@ constexpr uint32_t WIDE_BLOCKS = 256;
 constexpr uint32_t BIG_KNN_BLOCKS = 1024;
 constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
 
+// The list kernel for lists of at most L entries: the smallest LDS tile
+// (512, 1024, 2048 gathered entries) that holds twice L and at least
+// min_tile, else the long-list kernel.
 void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems &PR,
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
                 uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
-                hipStream_t s) {
+                hipStream_t s, uint32_t min_tile = 0) {
   if (cnt == 0) return;
+  if (!big && min_tile > 512 && L <= 512) L = min_tile >= 2048 ? 1024u : 512u;
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
@@ -1019,7 +1023,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
                        0, s, dm, F, nwork, Y);
     float *R0 = need(TT.get<float>(nwork));
     const float kscale = opt.model->kscale > 0.f ? opt.model->kscale : 1.f;
-    lbound = std::min<uint32_t>(256, kmax);
+    // first-pass lists are cut to first_list_cap (a multiple of 8 in
+    // [16, 1024]; 0 = 256)
+    const uint32_t cap0 = opt.first_list_cap ? opt.first_list_cap : 256u;
+    lbound = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(cap0, 16u), 1024u) & ~7u,
+                                kmax);
     const float rscale = opt.model->rscale > 0.f ? opt.model->rscale : 0.f;
     hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
                        std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
@@ -1102,7 +1110,10 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   // work-queue heads of the two wave clip launches and the count of cells
   // handed from the first to the second
   unsigned int *queue = need(TT.get<unsigned int>(3));
-  const bool exact_order = (opt.flags & MV_FLAG_EXACT_ORDER) != 0;
+  // the reference's summation order, or 64 cells per wavefront: the
+  // thread-per-cell kernel for the main lists
+  const bool exact_order =
+      (opt.flags & MV_FLAG_EXACT_ORDER) != 0 || opt.cells_per_wave == 64;
   const uint32_t lanes_main = opt.lanes ? opt.lanes : (uint32_t)device_cus() * 512u;
   const uint32_t lanes_big = 8192;
   struct LaneBufs {
@@ -1204,7 +1215,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         (tier == 2 ? kd_wide : big ? kd_big : kd_main) = kdist;
         Timer tkk(s, timed);
         launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
-                   s);
+                   s, opt.knn_min_tile);
         ms_knn_k += tkk.stop();
         n_knn++;
         q_knn += cnt;

@@ -58,7 +58,8 @@ class BuildOptions:
     def __init__(self, mode: int = MODE_PLAIN, k0: int = 27, k_max: int = 4194304,
                  lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
                  model=None, clip_blocks_per_cu: int = 0, length_order: bool = True,
-                 coop_max_planes: int = 0, coop_max_vertices: int = 0):
+                 coop_max_planes: int = 0, coop_max_vertices: int = 0,
+                 cells_per_wave: int = 1, first_list_cap: int = 0, knn_min_tile: int = 0):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
         self.exact_order = exact_order
@@ -67,6 +68,11 @@ class BuildOptions:
         # capacities of the large-cell kernel before a cell moves to the wide
         # kernel (0: the kernel's own; lower values force the wide path)
         self.coop_max_planes, self.coop_max_vertices = coop_max_planes, coop_max_vertices
+        # main lists: 1 cell per wavefront or 64 (one per lane); learned
+        # first-pass list cap (0: 256); smallest gather LDS tile (0: 512)
+        self.cells_per_wave = cells_per_wave
+        self.first_list_cap = first_list_cap
+        self.knn_min_tile = knn_min_tile
         # learned mode: a meshless_voro_amd.learned.Model (None: the
         # committed checkpoint)
         self.model = model
@@ -81,6 +87,9 @@ class BuildOptions:
         o.clip_blocks_per_cu = self.clip_blocks_per_cu
         o.coop_max_planes = self.coop_max_planes
         o.coop_max_vertices = self.coop_max_vertices
+        o.cells_per_wave = self.cells_per_wave
+        o.first_list_cap = self.first_list_cap
+        o.knn_min_tile = self.knn_min_tile
         if self.mode == MODE_LEARNED:
             from . import learned
             m = self.model if self.model is not None else learned.default_model()
