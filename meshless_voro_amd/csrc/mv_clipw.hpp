@@ -29,27 +29,30 @@ namespace mv {
 
 constexpr int WCAP = 64;  // vertices and planes per cell on this path
 // decomposition: lane 3 i + k takes plane k of the chunk's vertex i, so a
-// chunk covers 16 vertices (48 lanes)
-constexpr int WCH = 16;
+// chunk covers 14 vertices (42 lanes)
+constexpr int WCH = 14;
 
+// 8 KiB per wavefront: five blocks of four wavefronts fill a CU's 160 KiB.
+// A plane's point p (the midpoint of the generator and its neighbour, or
+// the box face's point) is not kept: the few steps that read it rebuild it
+// from the reference word, bit for bit (w_plane_point).
 struct WaveLds {
   double4 pa[WCAP];      // plane: n.xyz, d
-  double4 pb[WCAP];      // plane: p.xyz, reference word
+  uint64_t pref[WCAP];   // plane: reference word
   union {
-    // the 64 stream entries being consumed (lane t wrote entry t): plane
-    // p and reference word, plane n and d, neighbour distance and the
-    // plane's rounding bound
+    // the 64 stream entries being consumed (lane t wrote entry t): the
+    // reference word, plane n and d, neighbour distance and the plane's
+    // rounding bound
     // and, while a clip's boundary is found, the directed edges of the
     // removed vertices' dual triangles (bit y of emat[x]: edge x -> y)
     struct {
-      double4 pr[64];
+      uint64_t ref[64];
       double4 nd[64];
       double2 de[64];
       uint64_t emat[64];
       uint64_t rmat[64];  // bit x of rmat[y]: edge x -> y
     } e;
-    // decomposition: per (tet, vertex) of a chunk of WCH vertices (WaveLds
-    // is 10 KiB: four blocks of four wavefronts fit a CU's 160 KiB)
+    // decomposition: per (tet, vertex) of a chunk of WCH vertices
     struct {
       double4 fcf[6][WCH];  // area, area * (v0 + v1 + v2)
       double4 fcv[6][WCH];  // volume, volume * (v0 + v1 + v2 + gen)
@@ -147,14 +150,33 @@ __device__ __forceinline__ void lds_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The point p of stored plane j as HalfSpace::new received it
+// (half_space.rs:24-34): a box face's point (the face whose normal the
+// plane has), else 0.5 (q + neighbour) with the neighbour's image shift,
+// the same operations as when the plane was made.
+__device__ __forceinline__ d3 w_plane_point(const WaveLds &S, const WaveArgs &A,
+                                            const BoxParams &B, d3 q, int j) {
+  const uint64_t ref = S.pref[j];
+  if (ref == BOX_REF) {
+    const double4 a = S.pa[j];
+    int k = 0;
+    for (int c = 1; c < 6; c++)
+      if (a.x == B.pn[c][0] && a.y == B.pn[c][1] && a.z == B.pn[c][2]) k = c;
+    return mk3(B.pp[k][0], B.pp[k][1], B.pp[k][2]);
+  }
+  d3 ngb = xyz(A.spos[pk_slot(ref)]);
+  d3 s;
+  if (image_shift(pk_img(ref), B.width, &s)) ngb = add3(ngb, s);
+  return scl3(0.5, add3(q, ngb));
+}
+
 // HalfSpace::right_loc of a stored plane (half_space.rs:53-65)
 __device__ __forceinline__ d3 w_right_loc(const WaveLds &S, const WaveArgs &A,
                                           const BoxParams &B, d3 q, int j) {
-  double4 b = S.pb[j];
-  uint64_t ref = (uint64_t)__double_as_longlong(b.w);
+  const uint64_t ref = S.pref[j];
   if (ref == BOX_REF) {
-    double4 a = S.pa[j];
-    d3 pr = plane_project_onto(xyz(a), xyz(b), q);
+    const double4 a = S.pa[j];
+    d3 pr = plane_project_onto(xyz(a), w_plane_point(S, A, B, q, j), q);
     return sub3(scl3(2., pr), q);
   }
   d3 l = xyz(A.spos[pk_slot(ref)]);
@@ -200,6 +222,13 @@ __device__ __forceinline__ bool w_vertex_np(const WaveLds &S, d3 q, int dim,
   return true;
 }
 
+// The position a lane's new vertex takes: with copies kept (at most 32
+// vertices after the clip) both halves hold position lane & 31, else the
+// lane's own index.
+__device__ __forceinline__ int new_pos(bool dup, int nv_after, int lane) {
+  return dup && nv_after <= 32 ? (lane & 31) : lane;
+}
+
 // wave-uniform SimpleCycle ops on the pointer register `cyc`
 __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
   cyc = lane == at ? val : cyc;
@@ -207,14 +236,15 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
 
 // KEEP: also append every completed cell to the kept-cell store.  EXACT:
 // decide near-degenerate tests with the exact in-sphere predicate; without
-// it (its call holds ~30 VGPRs) the kernel fits 128 VGPRs, i.e. four
-// wavefronts per SIMD, and hands such cells (rare outside lattices) to the
-// EXACT instantiation through A.exact_list.
+// it (its call holds ~30 VGPRs) the kernel fits 96 VGPRs, i.e. five
+// wavefronts per SIMD (as many as the 8 KiB of LDS per wavefront allow), and
+// hands such cells (rare outside lattices) to the EXACT instantiation
+// through A.exact_list.
 template <bool KEEP, bool EXACT>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(EXACT || KEEP ? 3 : 4)))
+__attribute__((amdgpu_waves_per_eu(EXACT || KEEP ? 3 : 5)))
 k_clip_wave(WaveArgs A) {
-  __shared__ WaveLds lds_all[4];  // 4 x 10 KiB
+  __shared__ WaveLds lds_all[4];  // 4 x 8 KiB
   WaveLds &S = lds_all[threadIdx.x >> 6];
   const int lane = (int)__lane_id();
   unsigned long long my_exact = 0;
@@ -239,6 +269,9 @@ k_clip_wave(WaveArgs A) {
     const d3 q = xyz(qq);
     const uint64_t idx = (uint64_t)__double_as_longlong(qq.w);
     const uint32_t cnt = A.kcnt[w];
+    // read now, used when the list runs out: its latency hides behind the
+    // cell's clipping instead of stalling the wave at the end
+    const double kdist_w = A.kdist[w];
     const uint64_t* list = A.kpk + (A.kofs ? A.kofs[w] : (uint64_t)w * A.L);
     uint8_t state = CS_DONE;
     uint32_t err = 0, npop = 0, nexact = 0;
@@ -246,11 +279,16 @@ k_clip_wave(WaveArgs A) {
     // ---- ConvexCell::init (convex_cell.rs:294-312)
     if (lane < 6) {
       S.pa[lane] = make_double4(B.pn[lane][0], B.pn[lane][1], B.pn[lane][2], B.pd[lane]);
-      S.pb[lane] = make_double4(B.pp[lane][0], B.pp[lane][1], B.pp[lane][2],
-                                __longlong_as_double((long long)BOX_REF));
+      S.pref[lane] = BOX_REF;
     }
     lds_fence();
     int np = 6, nv = 8, cstart = 0, clen = 0;
+    // dup: while the cell has at most 32 vertices, lanes 32..63 hold copies
+    // of lanes 0..31 (vertex position p on lanes p and p + 32), so a scan
+    // pass tests two list entries at once.  Every step that moves or adds
+    // vertices does the same on both halves until a clip leaves more than
+    // 32; from then on the cell uses one lane per position up to 64.
+    bool dup = true;
     int cyc = lane;  // ptrs[p] of SimpleCycle::new
     // vs = 2 |v - q| as the reference rounds it (2 sqrt(r2)): the safety
     // radius 2 sqrt(max r2) is the maximum of vs over the vertices, sqrt
@@ -264,7 +302,7 @@ k_clip_wave(WaveArgs A) {
                               0x00000204u, 0x00030004u, 0x00010402u, 0x00010304u};
       uint32_t du = DU[lane & 7];
       bool ok = true;
-      if (lane < 8) {
+      if ((lane & 31) < 8) {
         d3 l;
         double r2;
         ok = w_vertex(S, q, dim, du & 0xFF, (du >> 8) & 0xFF, (du >> 16) & 0xFF, &l, &r2);
@@ -310,7 +348,7 @@ k_clip_wave(WaveArgs A) {
         // HalfSpace::new (half_space.rs:24-34)
         perrb = 1e-13 * (1. + dot3(abs3(pn), abs3(pp)));
         pd = dot3(pn, pp);
-        S.u.e.pr[lane] = make_double4(pp.x, pp.y, pp.z, __longlong_as_double((long long)ppk));
+        S.u.e.ref[lane] = ppk;
         S.u.e.nd[lane] = make_double4(pn.x, pn.y, pn.z, pd);
         S.u.e.de[lane] = make_double2(pdist, perrb);
       }
@@ -359,6 +397,38 @@ k_clip_wave(WaveArgs A) {
           return ((wave_ballot(fabs(c) < errb) | wave_ballot(c < 0.)) & act_m) != 0ull;
         };
         bool stop_hit = false;
+        if (dup) {
+          // the low half tests entry t, the high half entry t + 1 (the
+          // reference's order: entry t's stop and clip tests decide before
+          // entry t + 1 is looked at); the next pair is read ahead
+          const int hi = lane >> 5;
+          double4 e = S.u.e.nd[min(t + hi, 63)];
+          double2 de = S.u.e.de[min(t + hi, 63)];
+          for (;;) {
+            if (t == tbad) break;
+            const bool two = t + 1 < nb && t + 1 != tbad;
+            const double4 e2 = S.u.e.nd[min(t + 2 + hi, 63)];
+            const double2 de2 = S.u.e.de[min(t + 2 + hi, 63)];
+            const uint64_t ns = wave_ballot(!(vs < de.x));
+            const double c = dot3(xyz(e), vl) - e.w;
+            const uint64_t tm = wave_ballot(fabs(c) < de.y) | wave_ballot(c < 0.);
+            if (!(ns & act_m)) {
+              stop_hit = true;
+              break;
+            }
+            if (tm & act_m) break;
+            if (++t >= nb || !two) break;
+            if (!((ns >> 32) & act_m)) {
+              stop_hit = true;
+              break;
+            }
+            if ((tm >> 32) & act_m) break;
+            if (++t >= nb) break;
+            e = e2;
+            de = de2;
+          }
+          // a pair cut short by the bad entry ends on it: t == tbad
+        } else {
         double4 ndA = S.u.e.nd[t];
         double2 deA = S.u.e.de[t];
         for (;;) {
@@ -380,6 +450,7 @@ k_clip_wave(WaveArgs A) {
           if (touches(ndB, deB.y)) break;
           if (++t >= nb) break;
         }
+        }
         if (t >= nb) break;
         if (t == tbad) {
           state = CS_ERROR;
@@ -399,7 +470,7 @@ k_clip_wave(WaveArgs A) {
         const double d = nd.w;
         const bool near = act && fabs(cv) < errb;
         bool clipped = act && !near && cv < 0.;
-        const uint64_t ref = (uint64_t)__double_as_longlong(S.u.e.pr[t].w);
+        const uint64_t ref = S.u.e.ref[t];
         if (!EXACT && wave_ballot(near)) {
           state = CS_EXACT;
           break;
@@ -435,23 +506,29 @@ k_clip_wave(WaveArgs A) {
         const int num_v = __popcll(kept_m);  // < nv: M is not empty
         const uint64_t low_m = (1ull << num_v) - 1ull;
         const uint64_t holes = M & low_m, backk = kept_m & ~low_m;
-        const uint64_t below_me = (1ull << lane) - 1ull;  // lane < 64
-        const bool is_h = (holes >> lane) & 1ull, is_b = (backk >> lane) & 1ull;
+        // pos: the vertex position this lane holds; with copies, the high
+        // half does the same moves 32 lanes up
+        const int pos = dup ? (lane & 31) : lane;
+        const int hbase = lane - pos;
+        const uint64_t below_me = (1ull << pos) - 1ull;  // pos < 64
+        const bool is_h = (holes >> pos) & 1ull, is_b = (backk >> pos) & 1ull;
         const int jh = __popcll(holes & below_me) + 1;  // rank of a hole
         const int jb = __popcll(backk & ~below_me);     // rank of a b, from the top
-        // lane j-1 receives b_j and h_j (other lanes send to lane 63, which
-        // no rank reaches: there are at most 32 pairs)
-        const int b_of = perm_i(lane, is_b ? jb - 1 : 63);
-        const int h_of = perm_i(lane, is_h ? jh - 1 : 63);
-        const int h_j = __shfl(h_of, max(jb - 1, 0), 64);
-        const int b_jm1 = __shfl(b_of, max(jh - 2, 0), 64);
-        int dst = lane;
+        // lane j-1 (of the lane's half) receives b_j and h_j (other lanes
+        // send to lane 63, which no rank reaches: there are at most 32
+        // pairs, 16 with copies)
+        const int b_of = perm_i(pos, is_b ? hbase + jb - 1 : 63);
+        const int h_of = perm_i(pos, is_h ? hbase + jh - 1 : 63);
+        const int h_j = __shfl(h_of, hbase + max(jb - 1, 0), 64);
+        const int b_jm1 = __shfl(b_of, hbase + max(jh - 2, 0), 64);
+        int dst = pos;
         if (is_b)
           dst = h_j;
         else if (is_h)
           dst = (jh == 1 ? nv : b_jm1) - 1;
-        else if ((M >> lane) & 1ull)
-          dst = lane == num_v ? (backk ? (int)__builtin_ctzll(backk) : nv) - 1 : lane - 1;
+        else if ((M >> pos) & 1ull)
+          dst = pos == num_v ? (backk ? (int)__builtin_ctzll(backk) : nv) - 1 : pos - 1;
+        dst += hbase;
         vx = perm_d(vx, dst);
         vy = perm_d(vy, dst);
         vz = perm_d(vz, dst);
@@ -466,7 +543,7 @@ k_clip_wave(WaveArgs A) {
         const int p_idx = np;
         if (lane == 0) {
           S.pa[p_idx] = nd;
-          S.pb[p_idx] = S.u.e.pr[t];
+          S.pref[p_idx] = ref;
         }
         np += 1;
         // compute_boundary (convex_cell.rs:421-441) over positions
@@ -522,7 +599,7 @@ k_clip_wave(WaveArgs A) {
             }
           }
           if (fast) {
-            const int k = lane - num_v;
+            const int k = new_pos(dup, num_v + clen, lane) - num_v;
             const int q4 = clen == 4 ? p3 : p0;
             vc = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
             vn = k == 0 ? p1 : k == 1 ? p2 : k == 2 ? q4 : p0;
@@ -559,13 +636,14 @@ k_clip_wave(WaveArgs A) {
           fast = !wave_ballot(nbo > 1 || dupe) && 2 * n_node == nf_t + n_edge + 2;
           if (fast) {
             const int a0 = rl_i(vd, num_v) & 0xFF;
+            const int vp = new_pos(dup, num_v + n_edge, lane);
             int cur = a0;
             // a successor at every step, back at a0 after exactly n_edge
             bool closed = false;
             for (int r = 0; r < n_edge; r++) {
               const int nx = rl_i(nxt, cur);
               if (nx == 0xFF) break;
-              const bool at = lane == num_v + r;
+              const bool at = vp == num_v + r;
               vc = at ? cur : vc;
               vn = at ? nx : vn;
               if (nx == a0) {
@@ -637,7 +715,7 @@ k_clip_wave(WaveArgs A) {
               if (ok) {
                 if (ix > r) {
                   const int dr = rl_i(vd, r);
-                  vd = lane == r ? du : (lane == ix ? dr : vd);
+                  vd = pos == r ? du : (pos == ix ? dr : vd);
                 }
                 break;
               }
@@ -646,10 +724,11 @@ k_clip_wave(WaveArgs A) {
           }
           if (state != CS_DONE) break;
           int cur = cstart;
+          const int vp = new_pos(dup, num_v + clen, lane);
           for (int r = 0; r < clen; r++) {
             const int nx = rl_i(cyc, cur);
-            vc = lane == num_v + r ? cur : vc;
-            vn = lane == num_v + r ? nx : vn;
+            vc = vp == num_v + r ? cur : vc;
+            vn = vp == num_v + r ? nx : vn;
             cur = nx;
           }
           cyc = lane;  // SimpleCycle starts the next clip from the identity
@@ -659,7 +738,11 @@ k_clip_wave(WaveArgs A) {
           state = CS_CAPACITY;
           break;
         }
-        const bool mk = lane >= nv && lane < nv + clen;
+        // the copies survive the clip when the cell keeps at most 32
+        // vertices; otherwise positions 32 and up take the high lanes
+        const bool dup2 = dup && nv + clen <= 32;
+        const int vp = new_pos(dup, nv + clen, lane);
+        const bool mk = vp >= nv && vp < nv + clen;
         bool ok = true;
         double newr = 0.;
         if (mk) {
@@ -684,6 +767,7 @@ k_clip_wave(WaveArgs A) {
           break;
         }
         nv += clen;
+        dup = dup2;
         if (EXACT && !KEEP && np >= WCAP) {
           // the plane list is full: drop the planes no vertex refers to
           // (they build no face and enter no later vertex), keeping the
@@ -704,17 +788,18 @@ k_clip_wave(WaveArgs A) {
           const int np_new = __popcll(live);
           if (np_new < np) {
             auto renum = [&](int x) { return __popcll(live & ((1ull << x) - 1ull)); };
-            if (act) vd = renum(t0) | (renum(t1) << 8) | (renum(t2) << 16);
+            if ((dup ? (lane & 31) : lane) < nv)
+              vd = renum(t0) | (renum(t1) << 8) | (renum(t2) << 16);
             // one double of every plane at a time (few registers live)
             const bool mv_pl = lane < np && ((live >> lane) & 1ull);
             const int to = renum(lane);
 #pragma unroll 1
-            for (int c = 0; c < 8; c++) {
-              double *src = c < 4 ? &S.pa[lane].x : &S.pb[lane].x;
-              double *dst = c < 4 ? &S.pa[to].x : &S.pb[to].x;
-              const double v = mv_pl ? src[c & 3] : 0.;
+            for (int c = 0; c < 5; c++) {
+              double *src = c < 4 ? &S.pa[lane].x + c : reinterpret_cast<double *>(&S.pref[lane]);
+              double *dst = c < 4 ? &S.pa[to].x + c : reinterpret_cast<double *>(&S.pref[to]);
+              const double v = mv_pl ? *src : 0.;
               lds_fence();
-              if (mv_pl) dst[c & 3] = v;
+              if (mv_pl) *dst = v;
               lds_fence();
             }
             np = np_new;
@@ -732,7 +817,7 @@ k_clip_wave(WaveArgs A) {
       // comes with the list), else the whole stream was used
       if ((uint64_t)cnt >= total_stream)
         npop = cnt;
-      else if (sr < A.kdist[w])
+      else if (sr < kdist_w)
         npop = cnt + 1;
       else
         state = CS_NEED_MORE;
@@ -749,7 +834,7 @@ k_clip_wave(WaveArgs A) {
       bool cons = false;
       if (lane < np) {
         const double4 a = S.pa[lane];
-        const uint64_t ref = (uint64_t)__double_as_longlong(S.pb[lane].w);
+        const uint64_t ref = S.pref[lane];
         cons = true;
         if (dim == 1 && !(a.y == 0. && a.z == 0.)) cons = false;
         if (dim == 2 && !(a.z == 0.)) cons = false;
@@ -763,6 +848,11 @@ k_clip_wave(WaveArgs A) {
       // (lane c sums component c) and the faces (lane p = plane p) then
       // accumulate them in the reference's order: vertex order, tet order
       // within a vertex
+      // the projection of q onto plane `lane` (the face's first tet point),
+      // handed to the tet lanes by lane permutes
+      d3 pfl = mk3(0., 0., 0.);
+      if (lane < np)
+        pfl = plane_project_onto(xyz(S.pa[lane]), w_plane_point(S, A, B, q, lane), q);
       double cacc = 0.;
       const double *cterm =
           reinterpret_cast<const double *>(&S.u.t.fcv[0][0]) + (lane & 3);
@@ -793,9 +883,8 @@ k_clip_wave(WaveArgs A) {
         const int psrc = min(kk == 0 ? lane + 2 : lane - 1, 63);
         const d3 prev = mk3(__shfl(pedge.x, psrc, 64), __shfl(pedge.y, psrc, 64),
                             __shfl(pedge.z, psrc, 64));
+        const d3 pface = mk3(__shfl(pfl.x, pk, 64), __shfl(pfl.y, pk, 64), __shfl(pfl.z, pk, 64));
         if (mine) {
-          const double4 ak = S.pa[pk], bk = S.pb[pk];
-          const d3 pface = plane_project_onto(xyz(ak), xyz(bk), q);
 #pragma unroll
           for (int h = 0; h < 2; h++) {
             const d3 v0 = h == 0 ? pface : pedge;
@@ -875,7 +964,7 @@ k_clip_wave(WaveArgs A) {
       } else if ((facemask >> lane) & 1ull) {
         const uint64_t o = base + (uint64_t)__popcll(facemask & ((1ull << lane) - 1ull));
         const double4 a = S.pa[lane];
-        const uint64_t ref = (uint64_t)__double_as_longlong(S.pb[lane].w);
+        const uint64_t ref = S.pref[lane];
         const double fnrm = fa > 0. ? 1. / (3. * fa) : 0.;
         A.st_cell[o] = (uint32_t)idx;
         A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
@@ -905,7 +994,8 @@ k_clip_wave(WaveArgs A) {
       } else {
         if (lane < np) {
           A.cs.pl_a[bp + lane] = S.pa[lane];
-          A.cs.pl_b[bp + lane] = S.pb[lane];
+          const d3 pp = w_plane_point(S, A, B, q, lane);
+          A.cs.pl_b[bp + lane] = make_double4(pp.x, pp.y, pp.z, __longlong_as_double((long long)S.pref[lane]));
         }
         if (lane < nv)
           A.cs.vx[bv + lane] = make_double4(vx, vy, vz, __longlong_as_double((long long)vd));
