@@ -204,24 +204,6 @@ __device__ __forceinline__ bool w_vertex(const WaveLds &S, d3 q, int dim,
   return true;
 }
 
-// Vertex::from_dual for (i, j, new plane) with the new plane in registers
-__device__ __forceinline__ bool w_vertex_np(const WaveLds &S, d3 q, int dim,
-                                            int i, int j, d3 n, double d,
-                                            d3 *loc, double *r2) {
-  double4 a = S.pa[i], b = S.pa[j];
-  if (!intersect_planes(xyz(a), a.w, xyz(b), b.w, n, d, loc)) return false;
-  d3 dl = *loc;
-  if (dim == 1) {
-    dl.y = 0.;
-    dl.z = 0.;
-  } else if (dim == 2) {
-    dl.z = 0.;
-  }
-  d3 df = sub3(q, dl);
-  *r2 = dot3(df, df);
-  return true;
-}
-
 // The position a lane's new vertex takes: with copies kept (at most 32
 // vertices after the clip) both halves hold position lane & 31, else the
 // lane's own index.
@@ -555,7 +537,9 @@ k_clip_wave(WaveArgs A) {
         // a0, the lanes find the boundary edges at once; otherwise the
         // reference's walk runs step by step.  Either way the new vertices
         // (cur, next, p_idx) follow in cycle order from the start.
-        int vc = 0, vn = 0;
+        // the new vertex at position num_v + r: planes (cur, next) of the
+        // boundary's step r, packed cur | next << 8
+        int vcn = 0;
         bool fast = false;
         const int nrem = nv - num_v;
         if (nrem <= 2) {
@@ -601,8 +585,9 @@ k_clip_wave(WaveArgs A) {
           if (fast) {
             const int k = new_pos(dup, num_v + clen, lane) - num_v;
             const int q4 = clen == 4 ? p3 : p0;
-            vc = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
-            vn = k == 0 ? p1 : k == 1 ? p2 : k == 2 ? q4 : p0;
+            const int vc = k == 0 ? p0 : k == 1 ? p1 : k == 2 ? p2 : p3;
+            const int vn = k == 0 ? p1 : k == 1 ? p2 : k == 2 ? q4 : p0;
+            vcn = vc | (vn << 8);
           }
         }
         if (!fast) {
@@ -644,8 +629,7 @@ k_clip_wave(WaveArgs A) {
               const int nx = rl_i(nxt, cur);
               if (nx == 0xFF) break;
               const bool at = vp == num_v + r;
-              vc = at ? cur : vc;
-              vn = at ? nx : vn;
+              vcn = at ? cur | (nx << 8) : vcn;
               if (nx == a0) {
                 closed = r + 1 == n_edge;
                 break;
@@ -727,8 +711,7 @@ k_clip_wave(WaveArgs A) {
           const int vp = new_pos(dup, num_v + clen, lane);
           for (int r = 0; r < clen; r++) {
             const int nx = rl_i(cyc, cur);
-            vc = vp == num_v + r ? cur : vc;
-            vn = vp == num_v + r ? nx : vn;
+            vcn = vp == num_v + r ? cur | (nx << 8) : vcn;
             cur = nx;
           }
           cyc = lane;  // SimpleCycle starts the next clip from the identity
@@ -745,23 +728,63 @@ k_clip_wave(WaveArgs A) {
         const bool mk = vp >= nv && vp < nv + clen;
         bool ok = true;
         double newr = 0.;
-        if (mk) {
-          d3 l;
-          double r2;
-          ok = w_vertex_np(S, q, dim, vc, vn, n, d, &l, &r2);
-          vx = l.x;
-          vy = l.y;
-          vz = l.z;
-          vs = 2. * sqrt(r2);
-          vd = (int)pack_dual(vc, vn, p_idx);
-          newr = r2;
+        for (int q0 = 0; q0 < clen; q0 += 16) {
+          // Vertex::from_dual on lane quads: quad i (lanes 4i..4i+3) builds
+          // new vertex i, lane 4i+c its coordinate c with the operations
+          // intersect_planes applies to that coordinate (cross3's component
+          // c is a[c+1] b[c+2] - b[c+1] a[c+2]), every lane the determinant;
+          // lane 4i then sums |q - v|^2 in dot3's order.  The vertex lanes
+          // collect the results.
+          // the lane index through an empty asm, so that the lane constants
+          // below are recomputed here rather than kept live (and spilled)
+          // across the clip loop
+          int ql = lane;
+          asm volatile("" : "+v"(ql));
+          const int qi = q0 + (ql >> 2), qc = ql & 3;
+          const int qsrc = min(nv + qi, 63);
+          // planes of lanes outside the new range are stale but < 64
+          const int pab = __shfl(vcn, qsrc, 64);
+          const int ia = pab & 63, ib = (pab >> 8) & 63;
+          const bool live = qi < clen && qc < 3;
+          // components c+1, c+2 (lane 4i+3 reads valid, unused ones)
+          const int c1 = qc >= 2 ? qc - 2 : qc + 1, c2 = qc == 0 ? 2 : qc - 1;
+          const double *ra = &S.pa[ia].x, *rb = &S.pa[ib].x, *rp = &S.u.e.nd[t].x;
+          const double a1 = ra[c1], a2 = ra[c2], b1 = rb[c1], b2 = rb[c2];
+          const double p1 = rp[c1], p2 = rp[c2];
+          const double x12 = b1 * p2 - p1 * b2;  // (n_b x n_p)_c
+          const double x20 = p1 * a2 - a1 * p2;  // (n_p x n_a)_c
+          const double x01 = a1 * b2 - b1 * a2;  // (n_a x n_b)_c
+          // det = n_p . (n_a x n_b), the three components from the quad
+          const double det = (n.x * dpp_d<0x00>(x01) + n.y * dpp_d<0x55>(x01)) +
+                             n.z * dpp_d<0xAA>(x01);
+          const double lc = ((ra[3] * x12 + rb[3] * x20) + d * x01) / det;
+          const bool zc = (dim == 1 && qc > 0) || (dim == 2 && qc == 2);
+          const double df = (qc == 0 ? q.x : qc == 1 ? q.y : q.z) - (zc ? 0. : lc);
+          const double sq = df * df;
+          const double r2 = (sq + dpp_d<0x55>(sq)) + dpp_d<0xAA>(sq);
+          const double sv = 2. * sqrt(r2);
+          ok &= !(live && det == 0.);
+          if (live && qc == 0 && isnan(r2)) newr = r2;
+          // vertex lanes of this group of 16 take their results
+          const int k = vp - nv - q0;
+          const bool mine = mk && k >= 0 && k < 16;
+          const int g = 4 * min(max(k, 0), 15);
+          const double nx = __shfl(lc, g, 64), ny = __shfl(lc, g + 1, 64),
+                       nz = __shfl(lc, g + 2, 64), ns = __shfl(sv, g, 64);
+          if (mine) {
+            vx = nx;
+            vy = ny;
+            vz = nz;
+            vs = ns;
+          }
         }
+        if (mk) vd = vcn | (p_idx << 16);
         if (wave_ballot(!ok)) {
           state = CS_ERROR;
           err = E_DET_ZERO;
           break;
         }
-        if (wave_ballot(mk && isnan(newr))) {
+        if (wave_ballot(isnan(newr))) {
           state = CS_ERROR;
           err = E_NAN;
           break;
