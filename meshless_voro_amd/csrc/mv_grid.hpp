@@ -757,6 +757,34 @@ __device__ __forceinline__ void bitonic_sort_reg(double (&key)[H], uint64_t (&pk
   if constexpr (K < N) bitonic_sort_reg<2 * K, N, H>(key, pk);
 }
 
+// The same network on one 64-bit word per lane (the ascending order of
+// the words): a squared distance's bits with its low 7 bits replaced by the
+// entry's buffer slot.  Keys are >= +0, so their bit patterns order like the
+// keys; the words are distinct, and two entries whose keys agree above the
+// low 7 bits come out in slot order, which the caller checks against the
+// exact (key, word) order afterwards.
+template <int K, int J>
+__device__ __forceinline__ void bitonic_stage_u64(uint64_t &v) {
+  const uint32_t lane = __lane_id();
+  const int lo = xchg_i<J>((int)(v & 0xffffffffull)), hi = xchg_i<J>((int)(v >> 32));
+  const uint64_t o = ((uint64_t)(unsigned int)hi << 32) | (uint64_t)(unsigned int)lo;
+  const bool keep_min = ((lane & J) == 0) == ((lane & K) == 0);
+  if (keep_min != (v < o)) v = o;
+}
+template <int K, int J>
+__device__ __forceinline__ void bitonic_merge_u64(uint64_t &v) {
+  bitonic_stage_u64<K, J>(v);
+  if constexpr (J > 1) bitonic_merge_u64<K, J / 2>(v);
+}
+template <int K>
+__device__ __forceinline__ void bitonic_sort_u64(uint64_t &v) {
+  bitonic_merge_u64<K, K / 2>(v);
+  if constexpr (K < 64) bitonic_sort_u64<2 * K>(v);
+}
+__device__ __forceinline__ uint64_t packed_entry(const double *key, uint32_t s) {
+  return ((uint64_t)__double_as_longlong(key[s]) & ~0x7Full) | (uint64_t)s;
+}
+
 template <int CAP, int WPB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
@@ -800,7 +828,50 @@ __global__ void __launch_bounds__(256)
     }
     const uint32_t outn = min(cnt, Lw);
     uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
-    if (cnt <= 64) {
+    // up to 128 entries: the packed network, then the exact order of
+    // consecutive entries checked (order[] after the buffer's entries); on
+    // a violation (keys equal above their low 7 bits, lattices) the exact
+    // networks below sort the same buffer again
+    bool done = false;
+    if (cnt <= 128) {
+      uint64_t va = lane < cnt ? packed_entry(buf.key, lane) : ~0ull;
+      uint64_t vb = lane + 64 < cnt ? packed_entry(buf.key, lane + 64) : ~0ull;
+      bitonic_sort_u64<2>(va);
+      uint32_t posa = lane, posb = 64u + lane;
+      if (cnt > 64) {
+        bitonic_sort_u64<2>(vb);
+        // final place: own rank plus the smaller words of the other half
+        uint32_t ra = 0, rb = 0;
+#pragma unroll
+        for (uint32_t st = 64; st > 0; st >>= 1) {
+          const uint32_t ia = min(ra + st - 1, 63u), ib = min(rb + st - 1, 63u);
+          const uint64_t vbi = (uint64_t)__shfl((unsigned long long)vb, (int)ia, 64);
+          const uint64_t vai = (uint64_t)__shfl((unsigned long long)va, (int)ib, 64);
+          if (ra + st - 1 < 64 && vbi < va) ra += st;
+          if (rb + st - 1 < 64 && vai < vb) rb += st;
+        }
+        posa = lane + ra;
+        posb = lane + rb;
+      }
+      uint32_t *order = reinterpret_cast<uint32_t *>(buf.pk + 128);
+      if (posa < cnt) order[posa] = (uint32_t)(va & 0x7F);
+      if (posb < cnt) order[posb] = (uint32_t)(vb & 0x7F);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      bool bad = false;
+      for (uint32_t p = lane; p + 1 < cnt; p += 64) {
+        const uint32_t s0 = order[p], s1 = order[p + 1];
+        bad |= entry_less(buf.key[s1], buf.pk[s1], buf.key[s0], buf.pk[s0]);
+      }
+      if (!wave_ballot(bad)) {
+        done = true;
+        for (uint32_t p = lane; p < outn; p += 64) out[p] = buf.pk[order[p]];
+        if (lane == 0 && outn < cnt) kdist[w] = entry_dist(spos, box.width, q, buf.pk[order[outn]]);
+      }
+    }
+    if (done) {
+    } else if (cnt <= 64) {
       // bitonic network across the lanes, one entry per lane
       double key[1] = {lane < cnt ? buf.key[lane] : INFINITY};
       uint64_t pk[1] = {lane < cnt ? buf.pk[lane] : ~0ull};
