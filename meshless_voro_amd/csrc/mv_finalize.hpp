@@ -82,6 +82,29 @@ __global__ void k_sort_links(const uint64_t *__restrict__ link_off,
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   uint64_t b = link_off[c], e = b + link_in[c];
+  const uint32_t m = (uint32_t)(e - b);
+  if (m <= 1) return;
+  if (m <= 16) {
+    // a short run (the common case, ~8): every load issued at once, an
+    // insertion sort in registers, the run stored back
+    uint64_t r[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) r[i] = i < m ? links[b + i] : ~0ull;
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+#pragma unroll
+      for (int j = i; j > 0; j--) {
+        const uint64_t lo = r[j - 1] < r[j] ? r[j - 1] : r[j];
+        const uint64_t hi = r[j - 1] < r[j] ? r[j] : r[j - 1];
+        r[j - 1] = lo;
+        r[j] = hi;
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++)
+      if (i < m) links[b + i] = r[i];
+    return;
+  }
   for (uint64_t i = b + 1; i < e; i++) {
     uint64_t v = links[i];
     uint64_t j = i;
