@@ -492,24 +492,24 @@ k_clip_wave(WaveArgs A) {
         // half does the same moves 32 lanes up
         const int pos = dup ? (lane & 31) : lane;
         const int hbase = lane - pos;
-        const uint64_t below_me = (1ull << pos) - 1ull;  // pos < 64
-        const bool is_h = (holes >> pos) & 1ull, is_b = (backk >> pos) & 1ull;
-        const int jh = __popcll(holes & below_me) + 1;  // rank of a hole
-        const int jb = __popcll(backk & ~below_me);     // rank of a b, from the top
-        // lane j-1 (of the lane's half) receives b_j and h_j (other lanes
-        // send to lane 63, which no rank reaches: there are at most 32
-        // pairs, 16 with copies)
-        const int b_of = perm_i(pos, is_b ? hbase + jb - 1 : 63);
-        const int h_of = perm_i(pos, is_h ? hbase + jh - 1 : 63);
-        const int h_j = __shfl(h_of, hbase + max(jb - 1, 0), 64);
-        const int b_jm1 = __shfl(b_of, hbase + max(jh - 2, 0), 64);
+        // every lane's rule but the pairs', then the pairs (h_j, b_j) one by
+        // one from the scalar masks (a compare and a select per lane each)
         int dst = pos;
-        if (is_b)
-          dst = h_j;
-        else if (is_h)
-          dst = (jh == 1 ? nv : b_jm1) - 1;
-        else if ((M >> pos) & 1ull)
+        if ((M >> pos) & 1ull)
           dst = pos == num_v ? (backk ? (int)__builtin_ctzll(backk) : nv) - 1 : pos - 1;
+        {
+          uint64_t hr = holes, br = backk;
+          int prev_b = nv;
+          while (hr) {
+            const int h = (int)__builtin_ctzll(hr);
+            const int b = 63 - (int)__builtin_clzll(br);
+            dst = pos == b ? h : dst;
+            dst = pos == h ? prev_b - 1 : dst;
+            prev_b = b;
+            hr &= hr - 1ull;
+            br &= ~(1ull << b);
+          }
+        }
         dst += hbase;
         vx = perm_d(vx, dst);
         vy = perm_d(vy, dst);
