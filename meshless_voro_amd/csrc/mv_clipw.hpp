@@ -185,25 +185,6 @@ __device__ __forceinline__ d3 w_right_loc(const WaveLds &S, const WaveArgs &A,
   return l;
 }
 
-// Vertex::from_dual with the planes in LDS
-__device__ __forceinline__ bool w_vertex(const WaveLds &S, d3 q, int dim,
-                                         int i, int j, int k, d3 *loc,
-                                         double *r2) {
-  double4 a = S.pa[i], b = S.pa[j], e = S.pa[k];
-  if (!intersect_planes(xyz(a), a.w, xyz(b), b.w, xyz(e), e.w, loc))
-    return false;
-  d3 dl = *loc;
-  if (dim == 1) {
-    dl.y = 0.;
-    dl.z = 0.;
-  } else if (dim == 2) {
-    dl.z = 0.;
-  }
-  d3 df = sub3(q, dl);
-  *r2 = dot3(df, df);
-  return true;
-}
-
 // The position a lane's new vertex takes: with copies kept (at most 32
 // vertices after the clip) both halves hold position lane & 31, else the
 // lane's own index.
@@ -283,18 +264,26 @@ k_clip_wave(WaveArgs A) {
       const uint32_t DU[8] = {0x00000502u, 0x00000305u, 0x00020501u, 0x00030105u,
                               0x00000204u, 0x00030004u, 0x00010402u, 0x00010304u};
       uint32_t du = DU[lane & 7];
-      bool ok = true;
+      // the positions are the box's (Vertex::from_dual of the box planes,
+      // computed once per problem with the same operations); only the
+      // distance to q is the cell's own
       if ((lane & 31) < 8) {
-        d3 l;
-        double r2;
-        ok = w_vertex(S, q, dim, du & 0xFF, (du >> 8) & 0xFF, (du >> 16) & 0xFF, &l, &r2);
-        vx = l.x;
-        vy = l.y;
-        vz = l.z;
-        vs = 2. * sqrt(r2);
+        const double *cp = B.corner[lane & 7];
+        vx = cp[0];
+        vy = cp[1];
+        vz = cp[2];
+        d3 dl = mk3(vx, vy, vz);
+        if (dim == 1) {
+          dl.y = 0.;
+          dl.z = 0.;
+        } else if (dim == 2) {
+          dl.z = 0.;
+        }
+        const d3 df = sub3(q, dl);
+        vs = 2. * sqrt(dot3(df, df));
         vd = (int)du;
       }
-      if (wave_ballot(!ok)) {
+      if (!B.corner_ok) {
         state = CS_ERROR;
         err = E_DET_ZERO;
       }
@@ -879,9 +868,13 @@ k_clip_wave(WaveArgs A) {
       double cacc = 0.;
       const double *cterm =
           reinterpret_cast<const double *>(&S.u.t.fcv[0][0]) + (lane & 3);
-      const int sl = lane / 3, kk = lane - 3 * sl;
       bool ok = true, has = false;
       for (int c0 = 0; c0 < nv && ok; c0 += WCH) {
+        // the lane index through an empty asm: its constants are recomputed
+        // in each chunk instead of held (and spilled) across the cell
+        int dl_ = lane;
+        asm volatile("" : "+v"(dl_));
+        const int sl = dl_ / 3, kk = dl_ - 3 * sl;
         const int c1 = min(nv, c0 + WCH);
         const bool mine = c0 + sl < c1;
         // the vertex of this lane, from the lane that holds it
@@ -985,7 +978,11 @@ k_clip_wave(WaveArgs A) {
       if (base + nf > A.st_cap) {
         if (lane == 0) atomicOr(A.overflow, 1ull);
       } else if ((facemask >> lane) & 1ull) {
-        const uint64_t o = base + (uint64_t)__popcll(facemask & ((1ull << lane) - 1ull));
+        // the faces of lower lanes (mbcnt: no per-lane mask register)
+        const uint64_t o =
+            base + (uint64_t)__builtin_amdgcn_mbcnt_hi(
+                       (uint32_t)(facemask >> 32),
+                       __builtin_amdgcn_mbcnt_lo((uint32_t)facemask, 0u));
         const double4 a = S.pa[lane];
         const uint64_t ref = S.pref[lane];
         const double fnrm = fa > 0. ? 1. / (3. * fa) : 0.;

@@ -26,6 +26,11 @@ struct BoxParams {
   double A[3];      // snap anchor
   double iw[3];     // 1 / (3 width')
   double width[3];  // period of the neighbour images (user width)
+  // ConvexCell::init's eight vertices (the box corners, one per dual of the
+  // initial cell), computed once on the host by intersect_planes; every
+  // cell of the problem starts from them.  corner_ok: no determinant zero.
+  double corner[8][3];
+  int corner_ok;
   int dim;
   int periodic;
 };

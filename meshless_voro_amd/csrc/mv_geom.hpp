@@ -14,31 +14,31 @@ struct d3 {
   double x, y, z;
 };
 
-__device__ __forceinline__ d3 mk3(double x, double y, double z) {
+__host__ __device__ __forceinline__ d3 mk3(double x, double y, double z) {
   d3 r;
   r.x = x;
   r.y = y;
   r.z = z;
   return r;
 }
-__device__ __forceinline__ d3 add3(d3 a, d3 b) {
+__host__ __device__ __forceinline__ d3 add3(d3 a, d3 b) {
   return mk3(a.x + b.x, a.y + b.y, a.z + b.z);
 }
-__device__ __forceinline__ d3 sub3(d3 a, d3 b) {
+__host__ __device__ __forceinline__ d3 sub3(d3 a, d3 b) {
   return mk3(a.x - b.x, a.y - b.y, a.z - b.z);
 }
-__device__ __forceinline__ d3 scl3(double s, d3 a) {
+__host__ __device__ __forceinline__ d3 scl3(double s, d3 a) {
   return mk3(s * a.x, s * a.y, s * a.z);
 }
-__device__ __forceinline__ d3 div3(d3 a, double s) {
+__host__ __device__ __forceinline__ d3 div3(d3 a, double s) {
   return mk3(a.x / s, a.y / s, a.z / s);
 }
 // glam DVec3::dot: (x*x' + y*y') + z*z'
-__device__ __forceinline__ double dot3(d3 a, d3 b) {
+__host__ __device__ __forceinline__ double dot3(d3 a, d3 b) {
   return (a.x * b.x + a.y * b.y) + a.z * b.z;
 }
 // glam DVec3::cross
-__device__ __forceinline__ d3 cross3(d3 a, d3 b) {
+__host__ __device__ __forceinline__ d3 cross3(d3 a, d3 b) {
   return mk3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x,
              a.x * b.y - b.x * a.y);
 }
@@ -58,7 +58,7 @@ __device__ __forceinline__ d3 plane_project_onto(d3 n, d3 p, d3 point) {
 
 // intersect_planes (src/geometry.rs:68-76); d_i = p_i . n_i.
 // Returns false when the determinant is exactly zero (reference panics).
-__device__ __forceinline__ bool intersect_planes(d3 n0, double d0, d3 n1,
+__host__ __device__ __forceinline__ bool intersect_planes(d3 n0, double d0, d3 n1,
                                                  double d1, d3 n2, double d2,
                                                  d3 *out) {
   d3 c01 = cross3(n0, n1);

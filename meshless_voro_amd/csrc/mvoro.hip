@@ -601,6 +601,23 @@ BoxParams make_box(const double anchor_in[3], const double width_in[3],
   }
   b.dim = dim;
   b.periodic = periodic;
+  // the initial cell's vertices: duals (2,5,0) (5,3,0) (1,5,2) (5,1,3)
+  // (4,2,0) (4,0,3) (2,4,1) (4,3,1) (convex_cell.rs:294-312), with the same
+  // operations as on the device (this file is compiled without contraction)
+  static const int DU[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
+                               {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
+  b.corner_ok = 1;
+  for (int v = 0; v < 8; v++) {
+    const int i = DU[v][0], j = DU[v][1], k = DU[v][2];
+    d3 loc = mk3(0., 0., 0.);
+    if (!intersect_planes(mk3(b.pn[i][0], b.pn[i][1], b.pn[i][2]), b.pd[i],
+                          mk3(b.pn[j][0], b.pn[j][1], b.pn[j][2]), b.pd[j],
+                          mk3(b.pn[k][0], b.pn[k][1], b.pn[k][2]), b.pd[k], &loc))
+      b.corner_ok = 0;
+    b.corner[v][0] = loc.x;
+    b.corner[v][1] = loc.y;
+    b.corner[v][2] = loc.z;
+  }
   return b;
 }
 
