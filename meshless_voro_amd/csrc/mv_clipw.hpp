@@ -240,9 +240,15 @@ k_clip_wave(WaveArgs A) {
     uint32_t err = 0, npop = 0, nexact = 0;
 
     // ---- ConvexCell::init (convex_cell.rs:294-312)
-    if (lane < 6) {
-      S.pa[lane] = make_double4(B.pn[lane][0], B.pn[lane][1], B.pn[lane][2], B.pd[lane]);
-      S.pref[lane] = BOX_REF;
+    {
+      // (the lane through an empty asm: the address is recomputed per cell
+      // rather than held across the cell loop)
+      int il = lane;
+      asm volatile("" : "+v"(il));
+      if (il < 6) {
+        S.pa[il] = make_double4(B.pn[il][0], B.pn[il][1], B.pn[il][2], B.pd[il]);
+        S.pref[il] = BOX_REF;
+      }
     }
     lds_fence();
     int np = 6, nv = 8, cstart = 0, clen = 0;
@@ -260,10 +266,13 @@ k_clip_wave(WaveArgs A) {
     int vd = 0;
     {
       // init vertex duals (2,5,0) (5,3,0) (1,5,2) (5,1,3) (4,2,0) (4,0,3)
-      // (2,4,1) (4,3,1), packed 8 bits each
-      const uint32_t DU[8] = {0x00000502u, 0x00000305u, 0x00020501u, 0x00030105u,
-                              0x00000204u, 0x00030004u, 0x00010402u, 0x00010304u};
-      uint32_t du = DU[lane & 7];
+      // (2,4,1) (4,3,1), packed 8 bits each; the three components of vertex
+      // v are nibble v of three constants (no table in memory)
+      int dl0 = lane;
+      asm volatile("" : "+v"(dl0));
+      const uint32_t sh = 4u * (uint32_t)(dl0 & 7);
+      const uint32_t du = ((0x42445152u >> sh) & 15u) | (((0x34021535u >> sh) & 15u) << 8) |
+                          (((0x11303200u >> sh) & 15u) << 16);
       // the positions are the box's (Vertex::from_dual of the box planes,
       // computed once per problem with the same operations); only the
       // distance to q is the cell's own
