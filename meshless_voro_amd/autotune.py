@@ -49,7 +49,7 @@ from .voronoi import MODE_LEARNED, MODE_PLAIN, BuildOptions
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 SPACE = {
     "occupancy16": [16, 24, 32, 48, 64],
-    "clip_blocks_per_cu": [2, 3, 4, 6, 8],
+    "clip_blocks_per_cu": [2, 3, 4, 5, 6, 8],
     "cells_per_wave": [1, 64],
     "knn_min_tile": [512, 1024, 2048],
     "length_order": [True, False],
