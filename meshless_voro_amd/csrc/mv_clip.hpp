@@ -48,6 +48,7 @@ struct ClipArgs {
   uint32_t *st_cell;
   int64_t *st_right;
   uint8_t *st_img;
+  uint32_t *st_lslot;    // a linked face's slot among its right cell's links
   double *st_area;
   double *st_cen;
   double *st_norm;
@@ -754,7 +755,10 @@ __global__ void __launch_bounds__(256) k_clip(ClipArgs A) {
           A.st_cell[o] = (uint32_t)c.idx;
           A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
           // a face without periodic shift is also a link of its right cell
-          if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
+          // (the count's old value is the link's slot in that cell's range)
+          A.st_lslot[o] = ref != BOX_REF && pk_img(ref) == 13
+                              ? atomicAdd(&A.link_in[pk_idx(ref)], 1u)
+                              : 0u;
           A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
           A.st_area[o] = fa.x;
           A.st_cen[3 * o] = fa.y * fnrm;
@@ -829,7 +833,10 @@ __device__ __forceinline__ void coop_cell(const LS &S, const ClipArgs &A, uint32
         A.st_cell[o] = (uint32_t)c.idx;
         A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
         // a face without periodic shift is also a link of its right cell
-        if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
+        // (the count's old value is the link's slot in that cell's range)
+        A.st_lslot[o] = ref != BOX_REF && pk_img(ref) == 13
+                            ? atomicAdd(&A.link_in[pk_idx(ref)], 1u)
+                            : 0u;
         A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
         A.st_area[o] = fa.x;
         A.st_cen[3 * o] = fa.y * fnrm;

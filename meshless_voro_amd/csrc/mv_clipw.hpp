@@ -87,6 +87,7 @@ struct WaveArgs {
   uint32_t *st_cell;
   int64_t *st_right;
   uint8_t *st_img;
+  uint32_t *st_lslot;     // a linked face's slot among its right cell's links
   double *st_area, *st_cen, *st_norm;
   uint64_t st_cap;
   unsigned long long *st_counter;
@@ -998,7 +999,10 @@ k_clip_wave(WaveArgs A) {
         A.st_cell[o] = (uint32_t)idx;
         A.st_right[o] = ref == BOX_REF ? -1 : (int64_t)pk_idx(ref);
         // a face without periodic shift is also a link of its right cell
-        if (ref != BOX_REF && pk_img(ref) == 13) atomicAdd(&A.link_in[pk_idx(ref)], 1u);
+        // (the count's old value is the link's slot in that cell's range)
+        A.st_lslot[o] = ref != BOX_REF && pk_img(ref) == 13
+                            ? atomicAdd(&A.link_in[pk_idx(ref)], 1u)
+                            : 0u;
         A.st_img[o] = ref == BOX_REF ? 13 : (uint8_t)pk_img(ref);
         A.st_area[o] = fa;
         A.st_cen[3 * o] = fx * fnrm;

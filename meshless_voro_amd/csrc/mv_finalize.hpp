@@ -11,8 +11,9 @@
 // active one of a partial build), so a cell's links are the faces of lower
 // cells that name it (ascending in face index because ascending in left
 // cell), then its own faces.  The clip kernels count the first kind per
-// cell (link_in); k_place_faces puts a face's own link in place and claims a
-// slot for the other; k_sort_links orders the claimed slots.
+// cell (link_in), the count's old value being the face's slot among that
+// cell's links; k_place_faces puts both links in place; k_sort_links orders
+// each cell's claimed slots.
 #pragma once
 #include "mv_common.hpp"
 
@@ -23,6 +24,7 @@ struct FinalArgs {
   const uint32_t *st_cell;
   const int64_t *st_right;
   const uint8_t *st_img;
+  const uint32_t *st_lslot;  // slot of a linked face in its right cell's links
   const double *st_area, *st_cen, *st_norm;
   uint64_t n_staged;
   const uint64_t *cell_fbase;
@@ -36,7 +38,6 @@ struct FinalArgs {
   uint8_t *face_has_shift;
   const uint32_t *link_in;   // links per cell from lower cells' faces
   const uint64_t *link_off;  // exclusive scan of cell_nf + link_in
-  uint32_t *link_fill;       // slots claimed per cell (zeroed)
   uint64_t *links;
 };
 
@@ -63,8 +64,7 @@ __global__ void k_place_faces(FinalArgs A) {
   A.face_shift[3 * f + 1] = s.y;
   A.face_shift[3 * f + 2] = s.z;
   A.links[A.link_off[c] + A.link_in[c] + (o - A.cell_fbase[c])] = f;
-  if (right >= 0 && !has)
-    A.links[A.link_off[right] + atomicAdd(&A.link_fill[right], 1u)] = f;
+  if (right >= 0 && !has) A.links[A.link_off[right] + A.st_lslot[o]] = f;
 }
 
 __global__ void k_link_count(const uint32_t *__restrict__ cell_nf,

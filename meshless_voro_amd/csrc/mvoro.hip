@@ -1164,12 +1164,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   uint32_t *st_cell = nullptr;
   int64_t *st_right = nullptr;
   uint8_t *st_img = nullptr;
+  uint32_t *st_lslot = nullptr;
   double *st_area = nullptr, *st_cen = nullptr, *st_norm = nullptr;
   unsigned int *cap_err = need(TT.get<unsigned int>(1));
   for (int attempt = 0;; attempt++) {
     st_cell = need(TT.get<uint32_t>(st_cap));
     st_right = need(TT.get<int64_t>(st_cap));
     st_img = need(TT.get<uint8_t>(st_cap));
+    st_lslot = need(TT.get<uint32_t>(st_cap));
     st_area = need(TT.get<double>(st_cap));
     st_cen = need(TT.get<double>(3 * st_cap));
     st_norm = need(TT.get<double>(3 * st_cap));
@@ -1265,6 +1267,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.st_cell = st_cell;
           W.st_right = st_right;
           W.st_img = st_img;
+          W.st_lslot = st_lslot;
           W.st_area = st_area;
           W.st_cen = st_cen;
           W.st_norm = st_norm;
@@ -1342,6 +1345,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         A.st_cell = st_cell;
         A.st_right = st_right;
         A.st_img = st_img;
+        A.st_lslot = st_lslot;
         A.st_area = st_area;
         A.st_cen = st_cen;
         A.st_norm = st_norm;
@@ -1580,12 +1584,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->nl = NL;
   T->d_links = T->own<uint64_t>(NL ? NL : 1);
   if (F) {
-    uint32_t *lfill = need(TT.get<uint32_t>(n));
-    HIPCHK(hipMemsetAsync(lfill, 0, n * sizeof(uint32_t), s));
     FinalArgs FA;
     FA.st_cell = st_cell;
     FA.st_right = st_right;
     FA.st_img = st_img;
+    FA.st_lslot = st_lslot;
     FA.st_area = st_area;
     FA.st_cen = st_cen;
     FA.st_norm = st_norm;
@@ -1603,7 +1606,6 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     FA.face_has_shift = T->d_face_has_shift;
     FA.link_in = link_in;
     FA.link_off = T->d_cell_foff;
-    FA.link_fill = lfill;
     FA.links = T->d_links;
     hipLaunchKernelGGL(k_place_faces, dim3(nblk(F)), dim3(256), 0, s, FA);
     hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff, link_in, n,
