@@ -81,7 +81,11 @@ typedef struct mv_options {
                                    fixed capacity); 0 = the kernel's 255 and
                                    1024.  Lower values are for testing. */
   uint32_t cells_per_wave;      /* main lists: 1 cell per wavefront (0 or 1,
-                                   the default) or 64 (one cell per lane) */
+                                   the default), 2 (two cells per wavefront,
+                                   one per 32-lane half; cells over 32
+                                   vertices or planes continue on the
+                                   one-cell kernel) or 64 (one cell per
+                                   lane) */
   uint32_t first_list_cap;      /* learned mode: longest first-pass list,
                                    multiple of 8 in [16, 1024]; 0 = 256 */
   uint32_t knn_min_tile;        /* neighbour gathers: smallest LDS tile
@@ -230,6 +234,8 @@ typedef struct mv_stats {
   double ms_knn_kernels;        /* device time of the neighbour-list launches */
   double ms_model;              /* features + proposal inference */
   uint64_t wide_cells;          /* cells moved to the wide kernel */
+  uint64_t pair_handovers;      /* cells_per_wave = 2: cells continued on the
+                                   one-cell kernel from their saved state */
 } mv_stats;
 void mv_get_stats(const mv_tess *t, mv_stats *s);
 

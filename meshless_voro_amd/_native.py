@@ -93,7 +93,8 @@ class mv_stats(C.Structure):
                 ("clip_wave_cells", C.c_uint64), ("knn_queries", C.c_uint64),
                 ("knn_entries", C.c_uint64),
                 ("ms_clip_wave", C.c_double), ("ms_knn_kernels", C.c_double),
-                ("ms_model", C.c_double), ("wide_cells", C.c_uint64)]
+                ("ms_model", C.c_double), ("wide_cells", C.c_uint64),
+                ("pair_handovers", C.c_uint64)]
 
 
 class mv_device_view(C.Structure):

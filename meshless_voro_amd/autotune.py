@@ -10,8 +10,9 @@ uneven problems (half uniform, half in 8 clusters, mixed boundaries):
                       features scan fewer, fuller or more, emptier cells)
   clip_blocks_per_cu  blocks of 4 wavefronts per CU of the persistent
                       one-cell-per-wavefront clip launch
-  cells_per_wave      1 (a cell per wavefront, its vertices on the lanes) or
-                      64 (a cell per lane) for the main lists
+  cells_per_wave      1 (a cell per wavefront, its vertices on the lanes),
+                      2 (two cells per wavefront on 32-lane halves) or 64
+                      (a cell per lane) for the main lists
   knn_min_tile        smallest LDS tile of the neighbour gather (512, 1024,
                       2048 entries: fewer re-gathers against fewer
                       wavefronts per CU)
@@ -50,7 +51,7 @@ TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json"
 SPACE = {
     "occupancy16": [16, 24, 32, 48, 64],
     "clip_blocks_per_cu": [2, 3, 4, 5, 6, 8],
-    "cells_per_wave": [1, 64],
+    "cells_per_wave": [1, 2, 64],
     "knn_min_tile": [512, 1024, 2048],
     "length_order": [True, False],
     "first_list_cap": [128, 256, 512],

@@ -60,6 +60,19 @@ struct WaveLds {
   } u;
 };
 
+// A cell handed from the two-cells-per-wavefront kernel (k_clip_pair) to
+// this one: its half-spaces, its vertices (position, 2 |v - q|, packed
+// dual) and the next list entry to read, exactly as they stood before that
+// entry.  32 half-spaces and vertices at most.
+constexpr int PCAP = 32;
+struct Handover {
+  uint32_t np, nv, jnext, pad;
+  double4 pa[PCAP];     // n.xyz, d
+  uint64_t pref[PCAP];  // reference word
+  double4 v[PCAP];      // x, y, z, 2 |v - q|
+  int32_t vd[PCAP];     // packed dual
+};
+
 struct WaveArgs {
   const double4 *spos;
   const uint8_t *mask;
@@ -95,6 +108,14 @@ struct WaveArgs {
   unsigned long long *exact_counter;
   unsigned long long *overflow;  // bit 0: staging area full, bit 1: cell store full
   CellStore cs;                  // kept cells (cs.pl_a == nullptr: none)
+  // cells the pair kernel hands over (k_clip_pair writes them; a launch of
+  // k_clip_wave with resume != nullptr continues item i of sub from
+  // resume[i]); ho_cap bounds both
+  Handover *ho;
+  uint32_t *ho_list;
+  unsigned int *ho_cnt;
+  uint32_t ho_cap;
+  const Handover *resume;
 };
 
 __device__ __forceinline__ int rl_i(int v, int l) {
@@ -213,7 +234,8 @@ k_clip_wave(WaveArgs A) {
   const int lane = (int)__lane_id();
   unsigned long long my_exact = 0;
 
-  const uint32_t nwork = A.nsub ? *A.nsub : A.nwork;
+  const uint32_t nwork =
+      A.nsub ? (A.resume ? min(*A.nsub, A.ho_cap) : *A.nsub) : A.nwork;
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(A.queue, 1u);
   next = (uint32_t)rl_i((int)next, 0);
@@ -239,9 +261,16 @@ k_clip_wave(WaveArgs A) {
     const uint64_t* list = A.kpk + (A.kofs ? A.kofs[w] : (uint64_t)w * A.L);
     uint8_t state = CS_DONE;
     uint32_t err = 0, npop = 0, nexact = 0;
+    // a cell handed over by the pair kernel continues from its saved state
+    const Handover *R = A.resume ? A.resume + next : nullptr;
 
     // ---- ConvexCell::init (convex_cell.rs:294-312)
-    {
+    if (R) {
+      if (lane < (int)R->np) {
+        S.pa[lane] = R->pa[lane];
+        S.pref[lane] = R->pref[lane];
+      }
+    } else {
       // (the lane through an empty asm: the address is recomputed per cell
       // rather than held across the cell loop)
       int il = lane;
@@ -252,20 +281,29 @@ k_clip_wave(WaveArgs A) {
       }
     }
     lds_fence();
-    int np = 6, nv = 8, cstart = 0, clen = 0;
+    int np = R ? (int)R->np : 6, nv = R ? (int)R->nv : 8, cstart = 0, clen = 0;
     // dup: while the cell has at most 32 vertices, lanes 32..63 hold copies
     // of lanes 0..31 (vertex position p on lanes p and p + 32), so a scan
     // pass tests two list entries at once.  Every step that moves or adds
     // vertices does the same on both halves until a clip leaves more than
     // 32; from then on the cell uses one lane per position up to 64.
-    bool dup = true;
+    bool dup = R == nullptr;
     int cyc = lane;  // ptrs[p] of SimpleCycle::new
     // vs = 2 |v - q| as the reference rounds it (2 sqrt(r2)): the safety
     // radius 2 sqrt(max r2) is the maximum of vs over the vertices, sqrt
     // being correctly rounded and so monotone
     double vx = 0., vy = 0., vz = 0., vs = -INFINITY;
     int vd = 0;
-    {
+    if (R) {
+      if (lane < nv) {
+        const double4 v = R->v[lane];
+        vx = v.x;
+        vy = v.y;
+        vz = v.z;
+        vs = v.w;
+        vd = R->vd[lane];
+      }
+    } else {
       // init vertex duals (2,5,0) (5,3,0) (1,5,2) (5,1,3) (4,2,0) (4,0,3)
       // (2,4,1) (4,3,1), packed 8 bits each; the three components of vertex
       // v are nibble v of three constants (no table in memory)
@@ -303,14 +341,14 @@ k_clip_wave(WaveArgs A) {
       err = E_NAN;
     }
     if (state == CS_DONE && cnt == 0xFFFFFFFFu) state = CS_CAPACITY;
-    if (state == CS_DONE && (cnt == 0 || (uint64_t)pk_idx(list[0]) != idx)) {
+    if (!R && state == CS_DONE && (cnt == 0 || (uint64_t)pk_idx(list[0]) != idx)) {
       state = CS_ERROR;
       err = E_SELF_NOT_FIRST;
     }
 
     // ---- ConvexCell::build (convex_cell.rs:316-356)
     bool stopped = false;
-    for (uint32_t j0 = 1; state == CS_DONE && !stopped && j0 < cnt; j0 += 64) {
+    for (uint32_t j0 = R ? R->jnext : 1u; state == CS_DONE && !stopped && j0 < cnt; j0 += 64) {
       // prepare the planes of entries j0 .. j0+63, one per lane, and stage
       // them in LDS
       const uint32_t je = j0 + (uint32_t)lane;

@@ -330,10 +330,12 @@ __global__ void __launch_bounds__(1024) k_max_u32(const uint32_t *__restrict__ v
 // capacity flag, written where the host reads them
 __global__ void k_pass_words(const unsigned long long *__restrict__ counters,
                              const unsigned int *__restrict__ cap_err,
+                             const unsigned int *__restrict__ handed,
                              unsigned long long *__restrict__ out) {
   const int i = threadIdx.x;
   if (i < 4) out[i] = counters[i];
   if (i == 4) out[4] = *cap_err;
+  if (i == 5) out[5] = *handed;
 }
 
 // ---------------------------------------------------------------------

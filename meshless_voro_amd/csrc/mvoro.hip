@@ -25,6 +25,7 @@
 #include "mvoro.h"
 #include "mv_clip.hpp"
 #include "mv_clipw.hpp"
+#include "mv_clipp.hpp"
 #include "mv_common.hpp"
 #include "mv_faces.hpp"
 #include "mv_finalize.hpp"
@@ -1124,9 +1125,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   // wide-kernel capacities (planes, vertices per cell), doubled when a
   // cell runs out
   uint32_t wide_p = WIDE_P0, wide_v = WIDE_V0;
-  // work-queue heads of the two wave clip launches and the count of cells
-  // handed from the first to the second
-  unsigned int *queue = need(TT.get<unsigned int>(3));
+  // work-queue heads of the clip launches: [0] main, [1] cells handed to the
+  // exact kernel, [2] its queue, [3] the resume launch's queue, [4] cells the
+  // pair kernel handed over
+  unsigned int *queue = need(TT.get<unsigned int>(5));
+  HIPCHK(hipMemsetAsync(queue, 0, 5 * sizeof(unsigned int), s));
   // the reference's summation order, or 64 cells per wavefront: the
   // thread-per-cell kernel for the main lists
   const bool exact_order =
@@ -1152,7 +1155,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   };
   LaneBufs LM{};
   bool have_main = false;
-  uint64_t redone = 0, bigcells = 0, widecells = 0;
+  uint64_t redone = 0, bigcells = 0, widecells = 0, handovers = 0;
   uint32_t passes = 0;
   double ms_knn = 0, ms_clip = 0;
   // per-kernel figures (reset with each attempt)
@@ -1196,6 +1199,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     unsigned long long ovf_bits = 0;
     passes = 0;
     redone = 0;
+    handovers = 0;
     bigcells = 0;
     widecells = 0;
     ms_knn_k = ms_cw = 0;
@@ -1277,18 +1281,41 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           W.exact_counter = counters + 2;
           W.overflow = counters + 3;
           if (T->keep) W.cs = T->cs;
-          HIPCHK(hipMemsetAsync(queue, 0, 3 * sizeof(unsigned int), s));
+          HIPCHK(hipMemsetAsync(queue, 0, 5 * sizeof(unsigned int), s));
           const uint32_t bpc = opt.clip_blocks_per_cu ? opt.clip_blocks_per_cu : 4u;
           uint32_t blocks = std::min<uint32_t>((cnt + 3) / 4, (uint32_t)device_cus() * bpc);
           if (!T->keep) {
             W.exact_list = need(TT.get<uint32_t>(cnt));
             W.exact_cnt = queue + 1;
           }
+          // two cells per wavefront (cells that outgrow it continue on the
+          // wavefront kernel from their saved state)
+          const bool pair = !T->keep && opt.cells_per_wave == 2;
+          if (pair) {
+            const uint32_t hcap = std::min<uint32_t>(cnt, cnt / 4 + 256);
+            W.ho = need(TT.get<Handover>(hcap));
+            W.ho_list = need(TT.get<uint32_t>(hcap));
+            W.ho_cnt = queue + 4;
+            W.ho_cap = hcap;
+          }
           Timer tcw(s, timed);
-          if (T->keep)
+          if (T->keep) {
             hipLaunchKernelGGL((k_clip_wave<true, true>), dim3(blocks), dim3(256), 0, s, W);
-          else
+          } else if (pair) {
+            // four blocks of four wavefronts fill a CU's LDS
+            const uint32_t pblocks =
+                std::min<uint32_t>((cnt + 7) / 8, (uint32_t)device_cus() * std::min(bpc, 4u));
+            hipLaunchKernelGGL(k_clip_pair, dim3(pblocks), dim3(256), 0, s, W);
+            HIPCHK(hipGetLastError());
+            WaveArgs R = W;
+            R.sub = W.ho_list;
+            R.nsub = W.ho_cnt;
+            R.queue = queue + 3;
+            R.resume = W.ho;
+            hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, R);
+          } else {
             hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, W);
+          }
           HIPCHK(hipGetLastError());
           ms_cw += tcw.stop();
           n_cw++;
@@ -1407,10 +1434,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, 0, s, true);
       if (nbig + nwide) ms_clip += tbig.stop();
       HIPCHK(hipStreamSynchronize(s2));
-      unsigned long long cnts[5];
-      hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err, host_word());
+      unsigned long long cnts[6];
+      hipLaunchKernelGGL(k_pass_words, dim3(1), dim3(64), 0, s, counters, cap_err,
+                         (const unsigned int *)(queue + 4), host_word());
       HIPCHK(hipGetLastError());
-      read_words(s, cnts, 5);
+      read_words(s, cnts, 6);
+      handovers += cnts[5];
       const unsigned long long cap_err_h = cnts[4];
       // a cell that reached kmax entries without its security radius
       if (cap_err_h) throw BuildError{MV_E_CAPACITY};
@@ -1555,6 +1584,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->stats.cells_redone = redone;
   T->stats.big_cells = bigcells;
   T->stats.wide_cells = widecells;
+  T->stats.pair_handovers = handovers;
   T->stats.ms_knn = ms_knn + ms_model;
   T->stats.ms_clip = ms_clip;
   T->stats.clip_wave_launches = n_cw;

@@ -1,5 +1,5 @@
 """Launch and layout options change the time of a build, never its result:
-every tuned parameter (one cell per wavefront or per lane, the smallest
+every tuned parameter (one or two cells per wavefront or one per lane, the smallest
 gather tile, the learned first-pass cap, grid occupancy, clip blocks per
 CU, length order) gives the default build's arrays bit for bit, which equal
 the oracle's."""
@@ -17,7 +17,8 @@ ARRAYS = ("cell_volume", "cell_centroid", "cell_safety_radius", "cell_face_offse
           "cell_face_count", "cell_n_popped", "face_left", "face_right", "face_area",
           "face_centroid", "face_normal", "face_has_shift", "links")
 
-CASES = [dict(cells_per_wave=64), dict(knn_min_tile=1024), dict(knn_min_tile=2048),
+CASES = [dict(cells_per_wave=64), dict(cells_per_wave=2),
+         dict(mode=MODE_LEARNED, cells_per_wave=2), dict(knn_min_tile=1024), dict(knn_min_tile=2048),
          dict(occupancy16=16), dict(occupancy16=64), dict(clip_blocks_per_cu=2),
          dict(mode=MODE_LEARNED, first_list_cap=128),
          dict(mode=MODE_LEARNED, first_list_cap=512),
