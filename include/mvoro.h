@@ -466,6 +466,36 @@ mv_status mv_build_device_noalloc(const double *d_xyz, size_t n,
                                   uint32_t periodic, const mv_options *opt,
                                   mv_noalloc_out *out);
 
+/* ---------------------------------------------------------------------
+ * Captured builds.  No counterpart in the reference, which rebuilds from
+ * scratch on every call (src/voronoi.rs:151-238); for callers that rebuild
+ * the same problems again and again as their seeds move (a simulation
+ * step, a serving loop).  mv_graph_create runs one ordinary build of the
+ * problems (as mv_build_batch_device, P = 1 for a single build), plans
+ * every pass from it with headroom (cells per pass, list lengths and
+ * entries, faces) and records the whole build -- grid, neighbour lists,
+ * clipping, redo passes, faces and links -- as one hipGraph whose counts
+ * stay on the device.  mv_graph_launch replays it on the current contents
+ * of d_xyz (and d_mask): when the replay needed no more than was recorded,
+ * *replayed = 1 and the P handles view the graph's output arrays (the next
+ * launch overwrites them; they stay readable until then, and their memory
+ * until the graph and the handles are freed); otherwise (more cells to
+ * redo, longer lists, a seed outside the recorded grid, an error) the
+ * ordinary build runs instead, *replayed = 0, and the handles are its own.
+ * Either way the results are those of mv_build_batch_device on the same
+ * data, bit for bit.  d_xyz, d_mask, the problem layout and the options
+ * are fixed at creation; d_xyz and d_mask must stay valid until
+ * mv_graph_free.  Refused (MV_E_ARG): MV_FLAG_KEEP_CELLS,
+ * MV_FLAG_EXACT_ORDER, cells_per_wave other than 0 or 1, no seeds.
+ * Launches of one graph must not overlap. */
+typedef struct mv_graph mv_graph;
+mv_status mv_graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
+                          const uint8_t *d_mask, const double *anchors, const double *widths,
+                          const uint32_t *periodic, uint32_t dimensionality,
+                          const mv_options *opt, mv_graph **out);
+mv_status mv_graph_launch(mv_graph *g, mv_tess **out, uint32_t *replayed);
+void mv_graph_free(mv_graph *g);
+
 /* Test hook, no counterpart in the reference: the number of hipMalloc calls
  * the library has made in this process. */
 uint64_t mv_test_device_mallocs(void);

@@ -21,6 +21,7 @@ struct ClipArgs {
   Problems PR;           // boxes, grids (total stream) per problem
   const uint32_t *work;  // slots to build
   uint32_t nwork;
+  const unsigned int *nsub;  // the item count on the device (nwork bounds it)
   const uint64_t *kpk;   // neighbour lists, item-major
   const uint64_t *kofs;  // list offset per item (nullptr: w * L)
   const uint32_t *kcnt;
@@ -875,7 +876,8 @@ __global__ void __launch_bounds__(64) k_clip_coop(ClipArgs A) {
   S.lane = 0;
   S.lanes = 1;
   unsigned long long my_exact = 0;
-  for (uint32_t w = blockIdx.x; w < A.nwork; w += gridDim.x)
+  const uint32_t nw = A.nsub ? min(*A.nsub, A.nwork) : A.nwork;
+  for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x)
     coop_cell<MAXP, MAXV, KEEP>(S, A, w, my_exact);
   // every lane counted every exact test of its vertices
   for (int o = 32; o > 0; o >>= 1) my_exact += __shfl_xor(my_exact, o, 64);

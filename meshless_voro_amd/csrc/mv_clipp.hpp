@@ -120,7 +120,8 @@ k_clip_pair(WaveArgs A) {
   const int hb = lane & 32;
   const int sl = lane & 31;    // position within the half
 
-  const uint32_t nwork = A.nwork;
+  // nsub: the item count on the device (nwork bounds it)
+  const uint32_t nwork = A.nsub ? min(*A.nsub, A.nwork) : A.nwork;
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(A.queue, 2u);
   next = (uint32_t)rl_i((int)next, 0);

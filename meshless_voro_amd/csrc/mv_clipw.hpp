@@ -234,8 +234,9 @@ k_clip_wave(WaveArgs A) {
   const int lane = (int)__lane_id();
   unsigned long long my_exact = 0;
 
-  const uint32_t nwork =
-      A.nsub ? (A.resume ? min(*A.nsub, A.ho_cap) : *A.nsub) : A.nwork;
+  // nsub: the item count on the device, bounded by nwork (or by the
+  // hand-over capacity when resuming)
+  const uint32_t nwork = A.nsub ? min(*A.nsub, A.resume ? A.ho_cap : A.nwork) : A.nwork;
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(A.queue, 1u);
   next = (uint32_t)rl_i((int)next, 0);

@@ -27,6 +27,7 @@ struct FinalArgs {
   const uint32_t *st_lslot;  // slot of a linked face in its right cell's links
   const double *st_area, *st_cen, *st_norm;
   uint64_t n_staged;
+  const unsigned long long *n_staged_dev;  // the count on the device (n_staged bounds it)
   const uint64_t *cell_fbase;
   const uint64_t *face_off;  // exclusive scan of cell_nf
   const BoxParams *boxes;    // per problem (periodic width for the shifts)
@@ -43,7 +44,7 @@ struct FinalArgs {
 
 __global__ void k_place_faces(FinalArgs A) {
   size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= A.n_staged) return;
+  if (o >= A.n_staged || (A.n_staged_dev && o >= *A.n_staged_dev)) return;
   uint32_t c = A.st_cell[o];
   size_t f = A.face_off[c] + (o - A.cell_fbase[c]);
   int64_t right = A.st_right[o];
@@ -78,10 +79,14 @@ __global__ void k_link_count(const uint32_t *__restrict__ cell_nf,
 // order)
 __global__ void k_sort_links(const uint64_t *__restrict__ link_off,
                              const uint32_t *__restrict__ link_in, size_t n,
-                             uint64_t *__restrict__ links) {
+                             uint64_t *__restrict__ links,
+                             const unsigned long long *__restrict__ dNL) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   uint64_t b = link_off[c], e = b + link_in[c];
+  // dNL: the link total on the device (zero when the counts are not to be
+  // trusted)
+  if (dNL && e > *dNL) return;
   const uint32_t m = (uint32_t)(e - b);
   if (m <= 1) return;
   if (m <= 16) {
@@ -142,9 +147,14 @@ __global__ void k_problem_bases(const uint64_t *__restrict__ goff, uint32_t P,
                                 const uint64_t *__restrict__ face_off, uint64_t F,
                                 const uint64_t *__restrict__ link_off, uint64_t L,
                                 size_t n, uint64_t *__restrict__ foff,
-                                uint64_t *__restrict__ loff) {
+                                uint64_t *__restrict__ loff,
+                                const unsigned long long *__restrict__ dF,
+                                const unsigned long long *__restrict__ dL) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > P) return;
+  // totals on the device when given
+  if (dF) F = *dF;
+  if (dL) L = *dL;
   const uint64_t g = goff[p];
   foff[p] = g < n ? face_off[g] : F;
   loff[p] = g < n ? link_off[g] : L;
@@ -152,9 +162,10 @@ __global__ void k_problem_bases(const uint64_t *__restrict__ goff, uint32_t P,
 __global__ void k_localize_faces(uint64_t *__restrict__ face_left,
                                  int64_t *__restrict__ face_right, uint64_t nf,
                                  const uint64_t *__restrict__ goff,
-                                 const uint32_t *__restrict__ prob) {
+                                 const uint32_t *__restrict__ prob,
+                                 const unsigned long long *__restrict__ dn) {
   const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
+  if (f >= nf || (dn && f >= *dn)) return;
   const uint64_t base = goff[prob[face_left[f]]];
   face_left[f] -= base;
   if (face_right[f] >= 0) face_right[f] -= (int64_t)base;
@@ -164,11 +175,13 @@ __global__ void k_localize_links(uint64_t *__restrict__ link_off,
                                  const uint32_t *__restrict__ prob,
                                  const uint64_t *__restrict__ foff,
                                  const uint64_t *__restrict__ loff,
-                                 uint64_t *__restrict__ links) {
+                                 uint64_t *__restrict__ links,
+                                 const unsigned long long *__restrict__ dNL) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const uint32_t p = prob[c];
   const uint64_t b = link_off[c], e = b + link_cnt[c];
+  if (dNL && e > *dNL) return;
   for (uint64_t i = b; i < e; i++) links[i] -= foff[p];
   link_off[c] = b - loff[p];
 }

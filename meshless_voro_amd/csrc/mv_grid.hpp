@@ -795,8 +795,10 @@ __global__ void __launch_bounds__(256)
           uint32_t L, const uint32_t *__restrict__ Lper,
           const float *__restrict__ Rper, const uint64_t *__restrict__ kofs,
           uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
-          double *__restrict__ kdist) {
+          double *__restrict__ kdist, const uint32_t *__restrict__ dn) {
   __shared__ KnnLds<CAP> lds[WPB];
+  // dn: the item count on the device (nwork bounds it)
+  if (dn) nwork = min(*dn, nwork);
   const uint32_t lane = __lane_id();
   const uint32_t wib = threadIdx.x >> 6;
   KnnLds<CAP> &buf = lds[wib];
@@ -970,7 +972,8 @@ __global__ void __launch_bounds__(KB_THREADS)
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
               double *__restrict__ gkey2, uint64_t *__restrict__ gpk2,
-              uint32_t cap) {
+              uint32_t cap, const uint32_t *__restrict__ dn) {
+  if (dn) nwork = min(*dn, nwork);
   __shared__ uint32_t hist[KB_BINS];
   __shared__ double wkey[KB_WIN];
   __shared__ uint64_t wpk[KB_WIN];
@@ -1113,9 +1116,10 @@ __global__ void __launch_bounds__(KB_THREADS)
 __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
                             const double *__restrict__ sr, const double *__restrict__ kd,
                             float *__restrict__ Rout, uint32_t kmax,
-                            unsigned int *__restrict__ cap_err) {
+                            unsigned int *__restrict__ cap_err,
+                            const uint32_t *__restrict__ dn) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nwork) return;
+  if (w >= nwork || (dn && w >= *dn)) return;
   const uint32_t Lp = Lio[w];
   if (Lp >= kmax) atomicOr(cap_err, 1u);
   const double k = kd[w], r = sr[w];

@@ -29,6 +29,7 @@
 #include "mv_common.hpp"
 #include "mv_faces.hpp"
 #include "mv_finalize.hpp"
+#include "mv_graph.hpp"
 #include "mv_grid.hpp"
 #include "mv_learn.hpp"
 #include "mvoro_integrals.hpp"
@@ -63,6 +64,8 @@ struct Arena {
   unsigned long long *words = nullptr;  // device words the host reads totals from
 };
 thread_local Arena *t_arena = nullptr;
+// set while a captured build (mv_graph) is recorded
+thread_local bool t_graph = false;
 bool arena_active() { return t_arena != nullptr; }
 size_t arena_free() {
   return t_arena && t_arena->used < t_arena->size ? t_arena->size - t_arena->used : 0;
@@ -687,34 +690,35 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
                 const uint32_t *wl, uint32_t cnt, uint32_t L,
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
                 uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
-                hipStream_t s, uint32_t min_tile = 0) {
+                hipStream_t s, uint32_t min_tile = 0, const uint32_t *dn = nullptr,
+                uint32_t big_blocks = 0) {
   if (cnt == 0) return;
   if (!big && min_tile > 512 && L <= 512) L = min_tile >= 2048 ? 1024u : 512u;
   if (!big && L <= 256) {
     hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else if (!big && L <= 512) {
     hipLaunchKernelGGL((k_knn<1024, 2>), dim3((cnt + 1) / 2), dim3(128), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else if (!big && L <= 1024) {
     hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist);
+                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else {
     uint32_t cap = BIG_KNN_CAP;
     while (cap < 2 * (uint64_t)L && cap < (1u << 24)) cap <<= 1;
     const uint64_t per_block = 2ull * cap * (sizeof(double) + sizeof(uint64_t));
     // in the allocation-free build, half of what is left of its scratch
-    const uint64_t budget = arena_active() ? arena_free() / 2 : BIG_KNN_BYTES;
+    const uint64_t budget = arena_active() && !t_graph ? arena_free() / 2 : BIG_KNN_BYTES;
     const uint32_t max_blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(BIG_KNN_BLOCKS, budget / per_block));
-    const uint32_t blocks = std::min<uint32_t>(cnt, max_blocks);
+    const uint32_t blocks = std::min<uint32_t>(big_blocks ? big_blocks : cnt, max_blocks);
     double *gkey = need(TT.get<double>((size_t)blocks * cap));
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)blocks * cap));
     double *gkey2 = need(TT.get<double>((size_t)blocks * cap));
     uint64_t *gpk2 = need(TT.get<uint64_t>((size_t)blocks * cap));
     hipLaunchKernelGGL(k_knn_big, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start, PR,
                        wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, gkey2, gpk2,
-                       cap);
+                       cap, dn);
   }
   HIPCHK(hipGetLastError());
 }
@@ -911,6 +915,25 @@ void upload_model(const mv_model *m, DevModel *d, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
 }
 
+// What a build did, pass by pass (recorded for mv_graph_create's plan):
+// per tier (0 main, 1 large cells, 2 wide) the cells, the list-length bound
+// the list kernel was chosen for, and the list entries.
+struct PassRec {
+  uint32_t cnt[3] = {0, 0, 0};
+  uint32_t lbnd[3] = {0, 0, 0};
+  uint64_t total[3] = {0, 0, 0};
+};
+struct BuildTrace {
+  std::vector<PassRec> passes;
+  std::vector<BoxParams> boxes;
+  std::vector<GridParams> grids;
+  std::vector<uint32_t> coff;
+  size_t ncell = 0;
+  uint32_t lbound0 = 0;
+  uint64_t st_cap = 0, F = 0, NL = 0;
+};
+thread_local BuildTrace *t_trace = nullptr;
+
 // One build over P independent problems (P = 1: Voronoi::build /
 // build_partial).  Ts[p] receives problem p; Ts[0] owns the device memory,
 // which the other handles share.  All kernels run once over the
@@ -990,6 +1013,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   GridData D;
   const double occ = opt.grid_occupancy16 ? (double)opt.grid_occupancy16 / 16.0 : 2.0;
   build_grid(D, d_xyz, n, probs, dim, occ, TT, s);
+  if (t_trace) {
+    t_trace->boxes = D.boxes;
+    t_trace->grids = D.grids;
+    t_trace->coff = D.coff;
+    t_trace->ncell = D.ncell;
+  }
   const Problems PR = D.PR();
   const double4 *spos = D.spos;
   const uint32_t *cell_start = D.cell_start;
@@ -1200,6 +1229,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     passes = 0;
     redone = 0;
     handovers = 0;
+    if (t_trace) t_trace->passes.clear();
     bigcells = 0;
     widecells = 0;
     ms_knn_k = ms_cw = 0;
@@ -1207,6 +1237,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     q_knn = cells_cw = e_knn = 0;
     while (nw + nbig + nwide > 0) {
       passes++;
+      if (t_trace) t_trace->passes.emplace_back();
       double *nr_main = need(TT.get<double>(std::max<uint32_t>(nw, 1)));
       double *nr_big = need(TT.get<double>(std::max<uint32_t>(nbig, 1)));
       double *nr_wide = need(TT.get<double>(std::max<uint32_t>(nwide, 1)));
@@ -1236,6 +1267,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         uint32_t *kcnt = need(TT.get<uint32_t>(cnt));
         double *kdist = need(TT.get<double>(cnt));
         (tier == 2 ? kd_wide : big ? kd_big : kd_main) = kdist;
+        if (t_trace) {
+          PassRec &r = t_trace->passes.back();
+          r.cnt[tier] = cnt;
+          r.lbnd[tier] = lbnd;
+          r.total[tier] = total;
+        }
         Timer tkk(s, timed);
         launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
                    s, opt.knn_min_tile);
@@ -1513,13 +1550,16 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       constexpr uint32_t GROW = 4;
       if (nw_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
-                           (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err);
+                           (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err,
+                           (const uint32_t *)nullptr);
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
-                           (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err);
+                           (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err,
+                           (const uint32_t *)nullptr);
       if (nx_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nx_next)), dim3(256), 0, s, nx_next, cxl2,
-                           (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err);
+                           (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err,
+                           (const uint32_t *)nullptr);
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
@@ -1580,6 +1620,10 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     HIPCHK(hipMemsetAsync(cell_nf, 0, n * sizeof(uint32_t), s));
     HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
   }
+  if (t_trace) {
+    t_trace->lbound0 = lbound;
+    t_trace->st_cap = st_cap;
+  }
   T->stats.passes = passes;
   T->stats.cells_redone = redone;
   T->stats.big_cells = bigcells;
@@ -1612,9 +1656,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   hipLaunchKernelGGL(k_link_count, dim3(nb), dim3(256), 0, s, cell_nf, link_in, n, T->d_cell_fcnt);
   uint64_t NL = scan_excl<uint64_t, uint64_t>(T->d_cell_fcnt, T->d_cell_foff, n, s);
   T->nl = NL;
+  if (t_trace) {
+    t_trace->F = F;
+    t_trace->NL = NL;
+  }
   T->d_links = T->own<uint64_t>(NL ? NL : 1);
   if (F) {
     FinalArgs FA;
+    FA.n_staged_dev = nullptr;
     FA.st_cell = st_cell;
     FA.st_right = st_right;
     FA.st_img = st_img;
@@ -1639,7 +1688,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     FA.links = T->d_links;
     hipLaunchKernelGGL(k_place_faces, dim3(nblk(F)), dim3(256), 0, s, FA);
     hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff, link_in, n,
-                       T->d_links);
+                       T->d_links, (const unsigned long long *)nullptr);
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos, n,
@@ -1653,7 +1702,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     uint64_t *d_loff = need(TT.get<uint64_t>(P + 1));
     hipLaunchKernelGGL(k_problem_bases, dim3((P + 1 + 63) / 64), dim3(64), 0, s,
                        (const uint64_t *)D.d_goff, P, (const uint64_t *)face_off, F,
-                       (const uint64_t *)T->d_cell_foff, NL, n, d_foff, d_loff);
+                       (const uint64_t *)T->d_cell_foff, NL, n, d_foff, d_loff,
+                       (const unsigned long long *)nullptr, (const unsigned long long *)nullptr);
     HIPCHK(hipMemcpyAsync(foff.data(), d_foff, (P + 1) * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(loff.data(), d_loff, (P + 1) * sizeof(uint64_t),
@@ -1661,10 +1711,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     if (F)
       hipLaunchKernelGGL(k_localize_faces, dim3(nblk(F)), dim3(256), 0, s, T->d_face_left,
                          T->d_face_right, F, (const uint64_t *)D.d_goff,
-                         (const uint32_t *)D.prob);
+                         (const uint32_t *)D.prob, (const unsigned long long *)nullptr);
     hipLaunchKernelGGL(k_localize_links, dim3(nb), dim3(256), 0, s, T->d_cell_foff,
                        (const uint64_t *)T->d_cell_fcnt, n, (const uint32_t *)D.prob,
-                       (const uint64_t *)d_foff, (const uint64_t *)d_loff, T->d_links);
+                       (const uint64_t *)d_foff, (const uint64_t *)d_loff, T->d_links,
+                       (const unsigned long long *)nullptr);
     HIPCHK(hipGetLastError());
   } else {
     foff[1] = F;
@@ -2263,6 +2314,821 @@ mv_tess *voronoi_from_kept(mv_tess *t) {
 }  // namespace
 
 // ---------------------------------------------------------------------
+// captured builds (mv_graph_create / mv_graph_launch)
+// ---------------------------------------------------------------------
+// The build of a fixed set of problems recorded once as a hipGraph and
+// replayed on the current contents of the caller's seed array.  Its pass
+// structure and capacities come from a planning build on the seeds given at
+// creation, with headroom; every count a pass needs stays on the device, so
+// the recorded sequence never waits for the host.  A replay that needs more
+// than was recorded (see mv_graph.hpp) runs the ordinary build instead.
+struct mv_graph {
+  const double *d_xyz = nullptr;
+  const uint8_t *d_mask = nullptr;
+  size_t n = 0;
+  uint32_t P = 0, dim = 3;
+  std::vector<uint64_t> offsets;
+  std::vector<double> anchors, widths;
+  std::vector<uint32_t> periodic;
+  std::vector<ProblemSpec> probs;
+  mv_options opt{};
+  mv_model model{};
+  // the plan: passes, and per pass the cells, list-length bound and list
+  // entries each tier is recorded for
+  uint32_t npass = 0;
+  std::vector<uint32_t> capM, capB, LbM, LbB;
+  std::vector<uint64_t> EM, EB;
+  uint64_t st_cap = 0;
+  size_t ncell = 0;
+  uint32_t lbound0 = 0;
+  // device memory: constant tables, the status words and the arena the
+  // recorded build carves every other buffer from
+  std::shared_ptr<DevStore> store = std::make_shared<DevStore>();
+  uint64_t *d_goff = nullptr;
+  BoxParams *d_boxes = nullptr;
+  GridParams *d_grids = nullptr;
+  uint32_t *d_coff = nullptr;
+  DevModel *d_model = nullptr;
+  FeatParams *d_fpd = nullptr;
+  unsigned long long *d_status = nullptr;
+  size_t status_words = 0, arena_bytes = 0;
+  // outputs (inside the arena)
+  double *cell_loc = nullptr, *cell_cen = nullptr, *cell_vol = nullptr, *cell_sr = nullptr;
+  uint64_t *cell_foff = nullptr, *cell_fcnt = nullptr, *cell_idx = nullptr;
+  uint32_t *npop = nullptr;
+  uint64_t *face_left = nullptr;
+  int64_t *face_right = nullptr;
+  double *face_area = nullptr, *face_cen = nullptr, *face_norm = nullptr, *face_shift = nullptr;
+  uint8_t *face_has_shift = nullptr;
+  uint64_t *links = nullptr;
+  unsigned long long *h_status = nullptr;  // pinned
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  mv_stats plan_stats{};
+  ~mv_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (s) (void)hipStreamDestroy(s);
+    if (h_status) (void)hipHostFree(h_status);
+  }
+  template <typename T>
+  T *own(size_t k) {
+    T *p = dnew<T>(k);
+    if (!p) throw BuildError{MV_E_OOM};
+    store->dev.push_back(p);
+    return p;
+  }
+};
+
+namespace {
+
+// The recorded build: the ordinary build's kernels in its order, with the
+// counts on the device and the buffers sized by the plan.  Called while the
+// stream is captured, with the arena active (nothing is freed).
+void graph_body(mv_graph &G) {
+  hipStream_t s = G.s;
+  const size_t n = G.n;
+  const uint32_t P = G.P, dim = G.dim, NPG = G.npass;
+  const mv_options &opt = G.opt;
+  Temps TT;
+  auto fill32 = [&](void *p, size_t k, uint32_t v) {
+    if (k)
+      hipLaunchKernelGGL(k_fill_u32, dim3(nblk(k)), dim3(256), 0, s, (uint32_t *)p, (uint32_t)k,
+                         v);
+  };
+  auto fill8 = [&](void *p, size_t k, uint8_t v) {
+    if (k) hipLaunchKernelGGL(k_fill_u8, dim3(nblk(k)), dim3(256), 0, s, (uint8_t *)p, k, v);
+  };
+  auto fill64 = [&](void *p, size_t k, unsigned long long v) {
+    if (k)
+      hipLaunchKernelGGL(k_fill_u64, dim3(nblk(k)), dim3(256), 0, s, (unsigned long long *)p, k,
+                         v);
+  };
+  unsigned int *viol = need(TT.get<unsigned int>(1));
+  fill32(viol, 1, 0u);
+
+  // ---- grid (build_grid with the recorded grid parameters)
+  uint64_t maxn = 1;
+  for (uint32_t p = 0; p < P; p++) maxn = std::max<uint64_t>(maxn, G.offsets[p + 1] - G.offsets[p]);
+  const unsigned nb = nblk(n);
+  double4 *gpos = need(TT.get<double4>(n));
+  hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(256), 0, s, G.d_xyz, n, (int)dim, gpos);
+  uint32_t *prob = nullptr, *prob_of_slot = nullptr;
+  if (P > 1) {
+    prob = need(TT.get<uint32_t>(n));
+    prob_of_slot = need(TT.get<uint32_t>(n));
+    const uint32_t gx = (uint32_t)std::min<uint64_t>((maxn + 255) / 256, 64);
+    hipLaunchKernelGGL(k_problem_of, dim3(gx, P), dim3(256), 0, s, (const uint64_t *)G.d_goff,
+                       P, prob);
+  }
+  const uint32_t S = (uint32_t)((maxn + BBOX_CHUNK - 1) / BBOX_CHUNK);
+  double *bbp = need(TT.get<double>(6 * (size_t)S * P));
+  double *bbd = need(TT.get<double>(6 * (size_t)P));
+  hipLaunchKernelGGL(k_bbox_part, dim3(S, P), dim3(256), 0, s, (const double4 *)gpos,
+                     (const uint64_t *)G.d_goff, S, bbp);
+  hipLaunchKernelGGL(k_bbox_final, dim3((P + 63) / 64), dim3(64), 0, s, (const double *)bbp, S,
+                     P, bbd);
+  hipLaunchKernelGGL(k_graph_grid_check, dim3((P + 63) / 64), dim3(64), 0, s,
+                     (const double *)bbd, (const GridParams *)G.d_grids, P,
+                     (const uint64_t *)G.d_goff, viol);
+  const size_t ncell = G.ncell;
+  uint32_t *cellid = need(TT.get<uint32_t>(n));
+  uint32_t *counts = need(TT.get<uint32_t>(ncell + 1));
+  uint32_t *gfill = need(TT.get<uint32_t>(ncell));
+  uint32_t *cell_start = need(TT.get<uint32_t>(ncell + 1));
+  uint32_t *order = need(TT.get<uint32_t>(n));
+  double4 *spos = need(TT.get<double4>(n));
+  uint32_t *slot_of = need(TT.get<uint32_t>(n));
+  fill32(counts, ncell + 1, 0u);
+  fill32(gfill, ncell, 0u);
+  hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, s, (const double4 *)gpos, n,
+                     (const GridParams *)G.d_grids, (const uint32_t *)G.d_coff,
+                     (const uint32_t *)prob, cellid, counts);
+  scan_excl_dev<uint32_t, uint32_t>(counts, cell_start, ncell + 1, s);
+  hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, s, cellid, n, cell_start, gfill,
+                     order);
+  {
+    uint32_t *big_list = gfill;
+    unsigned int *big_cnt = counts;
+    uint64_t *ta = reinterpret_cast<uint64_t *>(spos);
+    uint64_t *tb = ta + n;
+    fill32(big_cnt, 1, 0u);
+    hipLaunchKernelGGL(k_grid_sort_cells, dim3(nblk(ncell)), dim3(256), 0, s, cell_start, ncell,
+                       (const double4 *)gpos, order, big_list, big_cnt);
+    hipLaunchKernelGGL(k_grid_sort_big, dim3(1024), dim3(GS_THREADS), 0, s,
+                       (const uint32_t *)cell_start, (const uint32_t *)big_list,
+                       (const unsigned int *)big_cnt, (const double4 *)gpos, order, ta, tb);
+  }
+  hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, (const double4 *)gpos,
+                     (const uint32_t *)order, n, (const uint32_t *)prob, spos, slot_of,
+                     prob_of_slot);
+  const Problems PR{G.d_boxes, G.d_grids, G.d_coff, prob_of_slot};
+
+  // ---- the first pass's cells (active slots in grid order) and lengths
+  uint32_t *cm = need(TT.get<uint32_t>(NPG + 1));   // main-tier cells per pass
+  uint32_t *cb = need(TT.get<uint32_t>(NPG + 1));   // large-cell tier
+  uint32_t *cbn = need(TT.get<uint32_t>(NPG + 1));  // ... of those, the redone ones
+  fill32(cm, NPG + 1, 0u);
+  fill32(cb, NPG + 1, 0u);
+  fill32(cbn, NPG + 1, 0u);
+  uint32_t *flag = need(TT.get<uint32_t>(n));
+  uint32_t *ident = need(TT.get<uint32_t>(n));
+  uint32_t *pos = need(TT.get<uint32_t>(n));
+  uint32_t *lists[5], *lens[5];  // [0]/[1] main, [2]/[3] large cells, [4] sort buffer
+  for (int k = 0; k < 5; k++) {
+    lists[k] = need(TT.get<uint32_t>(n));
+    lens[k] = need(TT.get<uint32_t>(n));
+  }
+  hipLaunchKernelGGL(k_flag_active, dim3(nb), dim3(256), 0, s, spos, n, G.d_mask, flag, ident);
+  scan_excl_dev<uint32_t, uint32_t>(flag, pos, n, s);
+  hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, s, ident, flag, pos, (uint32_t)n,
+                     lists[0]);
+  hipLaunchKernelGGL(k_graph_count, dim3(1), dim3(1), 0, s, (const uint32_t *)flag,
+                     (const uint32_t *)pos, (uint32_t)n, (const uint32_t *)nullptr, cm,
+                     (uint32_t)n, GV_ITEMS, viol);
+  hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lists[0], (const uint32_t *)cm,
+                     (uint32_t)n, 0u);
+  const uint32_t kmax =
+      std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : MV_K_MAX, MV_K_MAX), 2);
+  const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
+  const float *Rm = nullptr;
+  if (opt.mode == MV_MODE_LEARNED) {
+    float *F = need(TT.get<float>(n * NF));
+    float *hl = need(TT.get<float>(n));
+    float *Y = need(TT.get<float>(n * NO));
+    float *R0 = need(TT.get<float>(n));
+    hipLaunchKernelGGL(k_features<true>, dim3(nb), dim3(256), 0, s, spos, cell_start, PR,
+                       (const FeatParams *)G.d_fpd, lists[0], (uint32_t)n, F, hl);
+    hipLaunchKernelGGL(k_features<false>, dim3(nb), dim3(256), 0, s, spos, cell_start, PR,
+                       (const FeatParams *)G.d_fpd, lists[0], (uint32_t)n, F, hl);
+    const uint32_t tiles = (uint32_t)((n + 15) / 16);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
+                       0, s, (const DevModel *)G.d_model, (const float *)F, (uint32_t)n, Y);
+    const float kscale = G.model.kscale > 0.f ? G.model.kscale : 1.f;
+    const float rscale = G.model.rscale > 0.f ? G.model.rscale : 0.f;
+    hipLaunchKernelGGL(k_proposal, dim3(nb), dim3(256), 0, s, (const float *)Y,
+                       (const float *)hl, (uint32_t)n, std::min<uint32_t>(16, kmax), G.lbound0,
+                       kscale, rscale, lens[0], R0);
+    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lens[0],
+                       (const uint32_t *)cm, (uint32_t)n, 0u);
+    if (opt.flags & MV_FLAG_NO_LENGTH_ORDER) {
+      Rm = R0;
+    } else {
+      // longest first; the unused tail (length 0) sorts last
+      const uint32_t NB = std::min<uint32_t>(G.lbound0 / 8 + 1, LEN_MAX_BUCKETS);
+      uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
+      uint32_t *bst = need(TT.get<uint32_t>(NB + 1));
+      uint32_t *bfill = need(TT.get<uint32_t>(NB));
+      fill32(hist, NB + 1, 0u);
+      fill32(bfill, NB, 0u);
+      const uint32_t lblocks = (uint32_t)((n + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK);
+      hipLaunchKernelGGL(k_len_hist, dim3(lblocks), dim3(256), 0, s, lens[0], (uint32_t)n, NB,
+                         hist);
+      scan_excl_dev<uint32_t, uint32_t>(hist, bst, NB + 1, s);
+      float *Rs = need(TT.get<float>(n));
+      hipLaunchKernelGGL(k_len_scatter, dim3(lblocks), dim3(256), 0, s, lists[0], lens[0], R0,
+                         (uint32_t)n, NB, bst, bfill, lists[4], lens[4], Rs);
+      std::swap(lists[0], lists[4]);
+      std::swap(lens[0], lens[4]);
+      Rm = Rs;
+    }
+  } else {
+    fill32(lens[0], n, k0);
+    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lens[0],
+                       (const uint32_t *)cm, (uint32_t)n, 0u);
+  }
+
+  // ---- per-cell outputs, face staging, pass counters
+  G.cell_loc = need(TT.get<double>(3 * n));
+  G.cell_cen = need(TT.get<double>(3 * n));
+  G.cell_vol = need(TT.get<double>(n));
+  G.cell_sr = need(TT.get<double>(n));
+  G.cell_foff = need(TT.get<uint64_t>(n));
+  G.cell_fcnt = need(TT.get<uint64_t>(n));
+  G.cell_idx = need(TT.get<uint64_t>(n));
+  G.npop = need(TT.get<uint32_t>(n));
+  fill64(G.cell_cen, 3 * n, 0ull);
+  fill64(G.cell_vol, n, 0ull);
+  fill64(G.cell_sr, n, 0ull);
+  fill32(G.npop, n, 0u);
+  uint32_t *cell_nf = need(TT.get<uint32_t>(n));
+  uint64_t *cell_fbase = need(TT.get<uint64_t>(n));
+  uint32_t *link_in = need(TT.get<uint32_t>(n));
+  fill32(cell_nf, n, 0u);
+  fill64(cell_fbase, n, 0ull);
+  fill32(link_in, n, 0u);
+  const uint64_t st_cap = G.st_cap;
+  uint32_t *st_cell = need(TT.get<uint32_t>(st_cap));
+  int64_t *st_right = need(TT.get<int64_t>(st_cap));
+  uint8_t *st_img = need(TT.get<uint8_t>(st_cap));
+  uint32_t *st_lslot = need(TT.get<uint32_t>(st_cap));
+  double *st_area = need(TT.get<double>(st_cap));
+  double *st_cen = need(TT.get<double>(3 * st_cap));
+  double *st_norm = need(TT.get<double>(3 * st_cap));
+  unsigned long long *counters = need(TT.get<unsigned long long>(4));
+  fill64(counters, 4, 0ull);
+  fill64(counters + 1, 1, ~0ull);
+  unsigned int *cap_err = need(TT.get<unsigned int>(1));
+  fill32(cap_err, 1, 0u);
+  uint8_t *wstate = need(TT.get<uint8_t>(n));
+  uint8_t *bstate = need(TT.get<uint8_t>(n));
+  unsigned int *queue = need(TT.get<unsigned int>(5));
+  uint32_t *exact_list = need(TT.get<uint32_t>(n));
+  const uint32_t bpc = opt.clip_blocks_per_cu ? opt.clip_blocks_per_cu : 4u;
+  const uint32_t cus = (uint32_t)device_cus();
+
+  // ---- passes: the main tier (one cell per wavefront), the large-cell
+  // tier; cells not certified go to the next pass with longer lists, cells
+  // over the wavefront kernel's capacity to the large-cell tier
+  int im = 0, ib = 2;
+  const float *Rb = nullptr;
+  for (uint32_t p = 0; p < NPG; p++) {
+    const uint32_t capM = G.capM[p], capB = G.capB[p];
+    double *need_m = need(TT.get<double>(std::max<uint32_t>(capM, 1)));
+    double *need_b = need(TT.get<double>(std::max<uint32_t>(capB, 1)));
+    double *kd_m = nullptr, *kd_b = nullptr;
+    if (capM) {
+      fill8(wstate, capM, (uint8_t)CS_DONE);
+      fill32(queue, 5, 0u);
+      uint64_t *kofs = need(TT.get<uint64_t>(capM));
+      scan_excl_dev<uint32_t, uint64_t>(lens[im], kofs, capM, s);
+      hipLaunchKernelGGL(k_graph_entries, dim3(1), dim3(1), 0, s, (const uint32_t *)lens[im],
+                         (const uint64_t *)kofs, capM, (uint64_t)G.EM[p], cm + p, viol);
+      uint64_t *kpk = need(TT.get<uint64_t>(G.EM[p]));
+      uint32_t *kcnt = need(TT.get<uint32_t>(capM));
+      double *kdist = need(TT.get<double>(capM));
+      kd_m = kdist;
+      launch_knn(spos, cell_start, PR, lists[im], capM, G.LbM[p], lens[im], Rm, kofs, kpk, kcnt,
+                 kdist, false, TT, s, opt.knn_min_tile, cm + p, 0);
+      WaveArgs W;
+      memset(&W, 0, sizeof W);
+      W.spos = spos;
+      W.mask = G.d_mask;
+      W.PR = PR;
+      W.work = lists[im];
+      W.nwork = capM;
+      W.nsub = cm + p;
+      W.kpk = kpk;
+      W.kofs = kofs;
+      W.kcnt = kcnt;
+      W.kdist = kdist;
+      W.need_r = need_m;
+      W.queue = queue;
+      W.wstate = wstate;
+      W.cell_vol = G.cell_vol;
+      W.cell_cen = G.cell_cen;
+      W.cell_sr = G.cell_sr;
+      W.cell_nf = cell_nf;
+      W.link_in = link_in;
+      W.cell_fbase = cell_fbase;
+      W.cell_npop = G.npop;
+      W.st_cell = st_cell;
+      W.st_right = st_right;
+      W.st_img = st_img;
+      W.st_lslot = st_lslot;
+      W.st_area = st_area;
+      W.st_cen = st_cen;
+      W.st_norm = st_norm;
+      W.st_cap = st_cap;
+      W.st_counter = counters;
+      W.err_word = counters + 1;
+      W.exact_counter = counters + 2;
+      W.overflow = counters + 3;
+      W.exact_list = exact_list;
+      W.exact_cnt = queue + 1;
+      const uint32_t blocks = std::min<uint32_t>((capM + 3) / 4, cus * bpc);
+      hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, W);
+      WaveArgs X = W;
+      X.sub = exact_list;
+      X.nsub = queue + 1;
+      X.queue = queue + 2;
+      hipLaunchKernelGGL((k_clip_wave<false, true>), dim3(blocks), dim3(256), 0, s, X);
+    }
+    if (capB) {
+      fill8(bstate, capB, (uint8_t)CS_DONE);
+      uint64_t *kofs = need(TT.get<uint64_t>(capB));
+      scan_excl_dev<uint32_t, uint64_t>(lens[ib], kofs, capB, s);
+      hipLaunchKernelGGL(k_graph_entries, dim3(1), dim3(1), 0, s, (const uint32_t *)lens[ib],
+                         (const uint64_t *)kofs, capB, (uint64_t)G.EB[p], cb + p, viol);
+      uint64_t *kpk = need(TT.get<uint64_t>(G.EB[p]));
+      uint32_t *kcnt = need(TT.get<uint32_t>(capB));
+      double *kdist = need(TT.get<double>(capB));
+      kd_b = kdist;
+      launch_knn(spos, cell_start, PR, lists[ib], capB, G.LbB[p], lens[ib], Rb, kofs, kpk, kcnt,
+                 kdist, true, TT, s, 0, cb + p, std::min<uint32_t>(capB, BIG_KNN_BLOCKS));
+      ClipArgs A;
+      memset(&A, 0, sizeof A);
+      A.spos = spos;
+      A.mask = G.d_mask;
+      A.PR = PR;
+      A.work = lists[ib];
+      A.nwork = capB;
+      A.nsub = cb + p;
+      A.kpk = kpk;
+      A.kofs = kofs;
+      A.kcnt = kcnt;
+      A.kdist = kdist;
+      A.need_r = need_b;
+      A.lanes = std::min<uint32_t>(8192, capB);
+      A.wstate = bstate;
+      A.cell_vol = G.cell_vol;
+      A.cell_cen = G.cell_cen;
+      A.cell_sr = G.cell_sr;
+      A.cell_nf = cell_nf;
+      A.link_in = link_in;
+      A.cell_fbase = cell_fbase;
+      A.cell_npop = G.npop;
+      A.st_cell = st_cell;
+      A.st_right = st_right;
+      A.st_img = st_img;
+      A.st_lslot = st_lslot;
+      A.st_area = st_area;
+      A.st_cen = st_cen;
+      A.st_norm = st_norm;
+      A.st_cap = st_cap;
+      A.st_counter = counters;
+      A.err_word = counters + 1;
+      A.exact_counter = counters + 2;
+      A.overflow = counters + 3;
+      A.coop_maxp = opt.coop_max_planes;
+      A.coop_maxv = opt.coop_max_vertices;
+      hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>),
+                         dim3(std::max<uint32_t>(A.lanes, 1)), dim3(64), 0, s, A);
+    }
+    hipLaunchKernelGGL(k_graph_pass_check, dim3(1), dim3(1), 0, s,
+                       (const unsigned long long *)counters, (const unsigned int *)cap_err, viol);
+    if (p + 1 < NPG) {
+      const uint32_t capM2 = G.capM[p + 1], capB2 = G.capB[p + 1];
+      const int im2 = 1 - im, ib2 = 5 - ib;
+      double *rr_m = need(TT.get<double>(std::max<uint32_t>(capM2, 1)));
+      double *kk_m = need(TT.get<double>(std::max<uint32_t>(capM2, 1)));
+      double *rr_b = need(TT.get<double>(std::max<uint32_t>(capB2, 1)));
+      double *kk_b = need(TT.get<double>(std::max<uint32_t>(capB2, 1)));
+      float *Rm2 = need(TT.get<float>(std::max<uint32_t>(capM2, 1)));
+      float *Rb2 = need(TT.get<float>(std::max<uint32_t>(capB2, 1)));
+      auto take = [&](const uint32_t *w, const uint32_t *L, const uint8_t *st, uint32_t cnt,
+                      uint8_t want, const double *nr, const double *kd, const uint32_t *base,
+                      uint32_t *ow, uint32_t *oL, double *onr, double *okd, uint32_t ocap,
+                      uint32_t *dst) {
+        hipLaunchKernelGGL(k_flag_state, dim3(nblk(cnt)), dim3(256), 0, s, st, cnt, want, flag);
+        scan_excl_dev<uint32_t, uint32_t>(flag, pos, cnt, s);
+        hipLaunchKernelGGL(k_take_at, dim3(nblk(cnt)), dim3(256), 0, s, w, L, nr, kd,
+                           (const uint32_t *)flag, (const uint32_t *)pos, cnt, base, ocap, ow, oL,
+                           onr, okd);
+        hipLaunchKernelGGL(k_graph_count, dim3(1), dim3(1), 0, s, (const uint32_t *)flag,
+                           (const uint32_t *)pos, cnt, base, dst, ocap, GV_ITEMS, viol);
+      };
+      // not certified: longer lists in the same tier; over the wavefront
+      // kernel's capacity: the large-cell tier with the same list
+      if (capM)
+        take(lists[im], lens[im], wstate, capM, (uint8_t)CS_NEED_MORE, need_m, kd_m, nullptr,
+             lists[im2], lens[im2], rr_m, kk_m, capM2, cm + p + 1);
+      if (capB)
+        take(lists[ib], lens[ib], bstate, capB, (uint8_t)CS_NEED_MORE, need_b, kd_b, nullptr,
+             lists[ib2], lens[ib2], rr_b, kk_b, capB2, cbn + p + 1);
+      if (capM)
+        take(lists[im], lens[im], wstate, capM, (uint8_t)CS_CAPACITY, nullptr, nullptr,
+             cbn + p + 1, lists[ib2], lens[ib2], nullptr, nullptr, capB2, cb + p + 1);
+      else
+        hipLaunchKernelGGL(k_graph_count, dim3(1), dim3(1), 0, s, (const uint32_t *)flag,
+                           (const uint32_t *)pos, 0u, (const uint32_t *)(cbn + p + 1),
+                           cb + p + 1, capB2, GV_ITEMS, viol);
+      if (capB)
+        hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capB)), dim3(256), 0, s,
+                           (const uint8_t *)bstate, capB, (uint8_t)CS_CAPACITY, GV_WIDE, viol);
+      if (capM2) {
+        hipLaunchKernelGGL(k_redo_size, dim3(nblk(capM2)), dim3(256), 0, s, capM2, lens[im2],
+                           (const double *)rr_m, (const double *)kk_m, Rm2, kmax, cap_err,
+                           (const uint32_t *)(cm + p + 1));
+        hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lens[im2],
+                           (const uint32_t *)(cm + p + 1), capM2, 0u);
+        hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lists[im2],
+                           (const uint32_t *)(cm + p + 1), capM2, 0u);
+      }
+      if (capB2) {
+        hipLaunchKernelGGL(k_redo_size, dim3(nblk(capB2)), dim3(256), 0, s, capB2, lens[ib2],
+                           (const double *)rr_b, (const double *)kk_b, Rb2, kmax, cap_err,
+                           (const uint32_t *)(cbn + p + 1));
+        // moved cells keep their list; their gather starts from the density
+        hipLaunchKernelGGL(k_range_fill_f32, dim3(nblk(capB2)), dim3(256), 0, s, Rb2,
+                           (const uint32_t *)(cbn + p + 1), (const uint32_t *)(cb + p + 1),
+                           capB2, 0.f);
+        hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capB2)), dim3(256), 0, s, lens[ib2],
+                           (const uint32_t *)(cb + p + 1), capB2, 0u);
+        hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capB2)), dim3(256), 0, s, lists[ib2],
+                           (const uint32_t *)(cb + p + 1), capB2, 0u);
+      }
+      im = im2;
+      ib = ib2;
+      Rm = Rm2;
+      Rb = Rb2;
+    } else {
+      // the last recorded pass: every cell must be done
+      if (capM) {
+        hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capM)), dim3(256), 0, s,
+                           (const uint8_t *)wstate, capM, (uint8_t)CS_NEED_MORE, GV_LEFT, viol);
+        hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capM)), dim3(256), 0, s,
+                           (const uint8_t *)wstate, capM, (uint8_t)CS_CAPACITY, GV_LEFT, viol);
+      }
+      if (capB) {
+        hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capB)), dim3(256), 0, s,
+                           (const uint8_t *)bstate, capB, (uint8_t)CS_NEED_MORE, GV_LEFT, viol);
+        hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capB)), dim3(256), 0, s,
+                           (const uint8_t *)bstate, capB, (uint8_t)CS_CAPACITY, GV_WIDE, viol);
+      }
+    }
+  }
+
+  // ---- faces and links (flatten! + finalize), totals on the device
+  uint64_t *face_off = need(TT.get<uint64_t>(n));
+  scan_excl_dev<uint32_t, uint64_t>(cell_nf, face_off, n, s);
+  hipLaunchKernelGGL(k_link_count, dim3(nb), dim3(256), 0, s, cell_nf, link_in, n, G.cell_fcnt);
+  scan_excl_dev<uint64_t, uint64_t>(G.cell_fcnt, G.cell_foff, n, s);
+  unsigned long long *dF = need(TT.get<unsigned long long>(1));
+  unsigned long long *dNL = need(TT.get<unsigned long long>(1));
+  hipLaunchKernelGGL(k_graph_totals, dim3(1), dim3(1), 0, s, (const uint32_t *)cell_nf,
+                     (const uint64_t *)face_off, (const uint64_t *)G.cell_fcnt,
+                     (const uint64_t *)G.cell_foff, n, st_cap, viol, dF, dNL);
+  G.face_left = need(TT.get<uint64_t>(st_cap));
+  G.face_right = need(TT.get<int64_t>(st_cap));
+  G.face_area = need(TT.get<double>(st_cap));
+  G.face_cen = need(TT.get<double>(3 * st_cap));
+  G.face_norm = need(TT.get<double>(3 * st_cap));
+  G.face_shift = need(TT.get<double>(3 * st_cap));
+  G.face_has_shift = need(TT.get<uint8_t>(st_cap));
+  G.links = need(TT.get<uint64_t>(2 * st_cap));
+  FinalArgs FA;
+  memset(&FA, 0, sizeof FA);
+  FA.st_cell = st_cell;
+  FA.st_right = st_right;
+  FA.st_img = st_img;
+  FA.st_lslot = st_lslot;
+  FA.st_area = st_area;
+  FA.st_cen = st_cen;
+  FA.st_norm = st_norm;
+  FA.n_staged = st_cap;
+  FA.n_staged_dev = dF;
+  FA.cell_fbase = cell_fbase;
+  FA.face_off = face_off;
+  FA.boxes = G.d_boxes;
+  FA.prob = prob;
+  FA.face_left = G.face_left;
+  FA.face_right = G.face_right;
+  FA.face_area = G.face_area;
+  FA.face_centroid = G.face_cen;
+  FA.face_normal = G.face_norm;
+  FA.face_shift = G.face_shift;
+  FA.face_has_shift = G.face_has_shift;
+  FA.link_in = link_in;
+  FA.link_off = G.cell_foff;
+  FA.links = G.links;
+  hipLaunchKernelGGL(k_place_faces, dim3(nblk(st_cap)), dim3(256), 0, s, FA);
+  hipLaunchKernelGGL(k_sort_links, dim3(nb), dim3(256), 0, s, (const uint64_t *)G.cell_foff,
+                     (const uint32_t *)link_in, n, G.links, (const unsigned long long *)dNL);
+  hipLaunchKernelGGL(k_cell_records, dim3(nb), dim3(256), 0, s, (const double4 *)gpos, n,
+                     G.d_mask, (const uint64_t *)G.d_goff, (const uint32_t *)prob, G.cell_loc,
+                     G.cell_idx);
+  uint64_t *d_foff = nullptr, *d_loff = nullptr;
+  if (P > 1) {
+    d_foff = need(TT.get<uint64_t>(P + 1));
+    d_loff = need(TT.get<uint64_t>(P + 1));
+    hipLaunchKernelGGL(k_problem_bases, dim3((P + 1 + 63) / 64), dim3(64), 0, s,
+                       (const uint64_t *)G.d_goff, P, (const uint64_t *)face_off, st_cap,
+                       (const uint64_t *)G.cell_foff, 2 * st_cap, n, d_foff, d_loff,
+                       (const unsigned long long *)dF, (const unsigned long long *)dNL);
+    hipLaunchKernelGGL(k_localize_faces, dim3(nblk(st_cap)), dim3(256), 0, s, G.face_left,
+                       G.face_right, st_cap, (const uint64_t *)G.d_goff, (const uint32_t *)prob,
+                       (const unsigned long long *)dF);
+    hipLaunchKernelGGL(k_localize_links, dim3(nb), dim3(256), 0, s, G.cell_foff,
+                       (const uint64_t *)G.cell_fcnt, n, (const uint32_t *)prob,
+                       (const uint64_t *)d_foff, (const uint64_t *)d_loff, G.links,
+                       (const unsigned long long *)dNL);
+  }
+  hipLaunchKernelGGL(k_graph_status, dim3(nblk(P + 1)), dim3(256), 0, s,
+                     (const unsigned int *)viol, (const unsigned long long *)counters,
+                     (const unsigned long long *)dF, (const unsigned long long *)dNL,
+                     (const uint64_t *)d_foff, (const uint64_t *)d_loff, P, G.d_status);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(G.h_status, G.d_status, G.status_words * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s));
+}
+
+// Record graph_body on G.s with the arena `A` active.
+hipGraph_t record_graph(mv_graph &G, Arena &A) {
+  t_arena = &A;
+  t_graph = true;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(G.s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    t_arena = nullptr;
+    t_graph = false;
+    throw BuildError{MV_E_DEVICE};
+  }
+  try {
+    graph_body(G);
+  } catch (...) {
+    (void)hipStreamEndCapture(G.s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    t_arena = nullptr;
+    t_graph = false;
+    throw;
+  }
+  const hipError_t e = hipStreamEndCapture(G.s, &g);
+  t_arena = nullptr;
+  t_graph = false;
+  if (e != hipSuccess || !g) throw BuildError{MV_E_DEVICE};
+  return g;
+}
+
+// capacities from the planning build: headroom on every count a replay with
+// moved seeds may change, one spare pass
+void plan_graph(mv_graph &G, const BuildTrace &tr) {
+  const size_t n = G.n;
+  const uint32_t np = (uint32_t)tr.passes.size();
+  G.npass = std::max<uint32_t>(np + 1, 2);
+  G.capM.assign(G.npass, 0);
+  G.capB.assign(G.npass, 0);
+  G.LbM.assign(G.npass, 0);
+  G.LbB.assign(G.npass, 0);
+  G.EM.assign(G.npass, 0);
+  G.EB.assign(G.npass, 0);
+  auto grow = [&](uint64_t c, double f, uint64_t add) {
+    return (uint32_t)std::min<uint64_t>(n, (uint64_t)((double)c * f) + add);
+  };
+  const PassRec last = tr.passes.back();
+  for (uint32_t p = 0; p < G.npass; p++) {
+    if (p == 0) {
+      const PassRec &r = tr.passes[0];
+      G.capM[0] = (uint32_t)n;
+      G.LbM[0] = std::max<uint32_t>(r.lbnd[0], 2);
+      G.EM[0] = (uint64_t)((double)r.total[0] * 1.1) + 4ull * n + 4096;
+      continue;
+    }
+    if (p < np) {
+      // the cells this pass took in the planning build, and half again
+      const PassRec &r = tr.passes[p];
+      G.capM[p] = grow(r.cnt[0], 1.5, 1024);
+      G.capB[p] = grow(r.cnt[1], 1.5, 256);
+      G.LbM[p] = std::max<uint32_t>(r.lbnd[0], 2);
+      G.LbB[p] = std::max<uint32_t>(r.lbnd[1], 2048);
+      G.EM[p] = (uint64_t)((double)r.total[0] * 1.5) +
+                (uint64_t)G.capM[p] * std::min<uint32_t>(G.LbM[p], 64) + 4096;
+      G.EB[p] = (uint64_t)((double)r.total[1] * 1.5) + (uint64_t)G.capB[p] * 1024 + 65536;
+    } else {
+      // the spare pass: a few cells with lists up to four times the last
+      // recorded pass's
+      G.capM[p] = grow(0, 1.0, 1024);
+      G.capB[p] = grow(0, 1.0, 256);
+      G.LbM[p] = std::max<uint32_t>(4 * std::max<uint32_t>(last.lbnd[0], 2), 64);
+      G.LbB[p] = std::max<uint32_t>(4 * last.lbnd[1], 2048);
+      G.EM[p] = (uint64_t)G.capM[p] * G.LbM[p] + 4096;
+      G.EB[p] = (uint64_t)G.capB[p] * 1024 + 65536;
+    }
+  }
+  G.st_cap = std::max<uint64_t>(tr.F + tr.F / 4, 4096);
+  G.ncell = tr.ncell;
+  G.lbound0 = tr.lbound0;
+}
+
+mv_status graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
+                       const uint8_t *d_mask, const double *anchors, const double *widths,
+                       const uint32_t *periodic, uint32_t dim, const mv_options *opt,
+                       mv_graph **out) {
+  mv_options o;
+  mv_default_options(&o);
+  if (opt) o = *opt;
+  if (o.mode != MV_MODE_PLAIN && o.mode != MV_MODE_LEARNED) return MV_E_ARG;
+  if (o.mode == MV_MODE_LEARNED &&
+      (!o.model || o.model->magic != MV_MODEL_MAGIC || o.model->version != 1))
+    return MV_E_ARG;
+  if ((o.flags & (MV_FLAG_KEEP_CELLS | MV_FLAG_EXACT_ORDER)) || o.cells_per_wave > 1)
+    return MV_E_ARG;
+  if (offsets[P] == 0) return MV_E_ARG;
+  std::unique_ptr<mv_graph> G(new (std::nothrow) mv_graph());
+  if (!G) return MV_E_OOM;
+  G->d_xyz = d_xyz;
+  G->d_mask = d_mask;
+  G->n = offsets[P];
+  G->P = P;
+  G->dim = dim;
+  G->offsets.assign(offsets, offsets + P + 1);
+  G->anchors.assign(anchors, anchors + 3 * P);
+  G->widths.assign(widths, widths + 3 * P);
+  G->periodic.assign(P, 0u);
+  for (uint32_t p = 0; p < P; p++) G->periodic[p] = periodic && periodic[p] ? 1u : 0u;
+  if (o.mode == MV_MODE_LEARNED) G->model = *o.model;
+  o.model = o.mode == MV_MODE_LEARNED ? &G->model : nullptr;
+  G->opt = o;
+  // the planning build
+  BuildTrace tr;
+  {
+    std::vector<mv_tess *> hs(P, nullptr);
+    t_trace = &tr;
+    const mv_status st = run_build_batch(d_xyz, offsets, P, d_mask, anchors, widths, periodic,
+                                         dim, &G->opt, hs.data());
+    t_trace = nullptr;
+    if (st != MV_OK) return st;
+    G->plan_stats = hs[0]->stats;
+    for (mv_tess *h : hs) delete h;
+  }
+  if (tr.passes.empty()) return MV_E_ARG;
+  try {
+    plan_graph(*G, tr);
+    G->probs.resize(P);
+    for (uint32_t p = 0; p < P; p++) {
+      normalise_box(anchors + 3 * p, widths + 3 * p, dim, G->probs[p].anchor,
+                    G->probs[p].width);
+      G->probs[p].periodic = G->periodic[p];
+      G->probs[p].g0 = offsets[p];
+      G->probs[p].g1 = offsets[p + 1];
+    }
+    // the planning build's grids, widened to hold each problem's box (a
+    // seed anywhere in it stays inside); the cell counts do not change
+    std::vector<GridParams> grids = tr.grids;
+    for (uint32_t p = 0; p < P; p++) {
+      GridParams &g = grids[p];
+      double scale = 0.;
+      for (int c = 0; c < 3; c++) {
+        if (g.h[c] == 0.) continue;
+        const double lo = std::min(g.lo[c], G->probs[p].anchor[c]);
+        const double hi = std::max(g.lo[c] + (double)g.G[c] * g.h[c],
+                                   G->probs[p].anchor[c] + G->probs[p].width[c]);
+        g.lo[c] = lo;
+        g.h[c] = (hi - lo) / g.G[c];
+        g.inv_h[c] = (double)g.G[c] / (hi - lo);
+        scale = std::max(scale, std::max(std::fabs(lo), std::fabs(hi)));
+        scale = std::max(scale, std::fabs(G->probs[p].width[c]));
+      }
+      g.margin = std::max(g.margin, 1e-10 * (scale + 1e-300) + 1e-300);
+    }
+    hipStream_t s0 = 0;
+    G->d_goff = G->own<uint64_t>(P + 1);
+    G->d_boxes = G->own<BoxParams>(P);
+    G->d_grids = G->own<GridParams>(P);
+    G->d_coff = G->own<uint32_t>(P + 1);
+    HIPCHK(hipMemcpyAsync(G->d_goff, G->offsets.data(), (P + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, s0));
+    HIPCHK(hipMemcpyAsync(G->d_boxes, tr.boxes.data(), P * sizeof(BoxParams),
+                          hipMemcpyHostToDevice, s0));
+    HIPCHK(hipMemcpyAsync(G->d_grids, grids.data(), P * sizeof(GridParams),
+                          hipMemcpyHostToDevice, s0));
+    HIPCHK(hipMemcpyAsync(G->d_coff, tr.coff.data(), (P + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s0));
+    if (o.mode == MV_MODE_LEARNED) {
+      G->d_model = G->own<DevModel>(1);
+      upload_model(&G->model, G->d_model, s0);
+      const std::vector<FeatParams> fph = feat_params(G->probs, (int)dim);
+      G->d_fpd = G->own<FeatParams>(P);
+      HIPCHK(hipMemcpyAsync(G->d_fpd, fph.data(), P * sizeof(FeatParams), hipMemcpyHostToDevice,
+                            s0));
+    }
+    HIPCHK(hipStreamSynchronize(s0));
+    G->status_words = 4 + 2 * ((size_t)P + 1);
+    G->d_status = G->own<unsigned long long>(G->status_words);
+    if (hipHostMalloc((void **)&G->h_status, G->status_words * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
+      G->h_status = nullptr;
+      throw BuildError{MV_E_OOM};
+    }
+    (void)device_cus();
+    HIPCHK(hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking));
+    // record once against a counting arena (its addresses are only kernel
+    // arguments of a graph that never runs) to size the arena, then for real
+    {
+      Arena A;
+      A.base = reinterpret_cast<uint8_t *>(uintptr_t(1) << 44);
+      A.size = size_t(1) << 44;
+      hipGraph_t g0 = record_graph(*G, A);
+      (void)hipGraphDestroy(g0);
+      G->arena_bytes = A.used + 4096;
+    }
+    Arena A;
+    A.base = G->own<uint8_t>(G->arena_bytes);
+    A.size = G->arena_bytes;
+    G->graph = record_graph(*G, A);
+    if (A.exhausted) throw BuildError{MV_E_OOM};
+    HIPCHK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+  } catch (...) {
+    const mv_status st = status_of_exception();
+    (void)hipDeviceSynchronize();
+    (void)hipGetLastError();
+    return st;
+  }
+  *out = G.release();
+  return MV_OK;
+}
+
+mv_status graph_launch(mv_graph *G, mv_tess **out, uint32_t *replayed) {
+  const uint32_t P = G->P;
+  auto t0 = std::chrono::steady_clock::now();
+  try {
+    HIPCHK(hipGraphLaunch(G->exec, G->s));
+    HIPCHK(hipStreamSynchronize(G->s));
+  } catch (...) {
+    const mv_status st = status_of_exception();
+    (void)hipGetLastError();
+    return st;
+  }
+  const unsigned long long *st = G->h_status;
+  if (st[0] != 0ull || st[1] != ~0ull) {
+    // the replay needed more than was recorded (or met an error): the
+    // ordinary build
+    if (replayed) *replayed = 0;
+    return run_build_batch(G->d_xyz, G->offsets.data(), P, G->d_mask, G->anchors.data(),
+                           G->widths.data(), G->periodic.data(), G->dim, &G->opt, out);
+  }
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::vector<mv_tess *> Ts(P, nullptr);
+  for (uint32_t p = 0; p < P; p++) {
+    Ts[p] = new (std::nothrow) mv_tess();
+    if (!Ts[p]) {
+      for (mv_tess *t : Ts) delete t;
+      return MV_E_OOM;
+    }
+  }
+  const uint64_t F = st[2], NL = st[3];
+  for (uint32_t p = 0; p < P; p++) {
+    mv_tess *U = Ts[p];
+    const uint64_t g0 = G->offsets[p];
+    const uint64_t f0 = P > 1 ? st[4 + p] : 0, f1 = P > 1 ? st[4 + p + 1] : F;
+    const uint64_t l0 = P > 1 ? st[5 + P + p] : 0, l1 = P > 1 ? st[5 + P + p + 1] : NL;
+    U->parent = G->store;
+    memcpy(U->anchor, G->probs[p].anchor, sizeof U->anchor);
+    memcpy(U->width, G->probs[p].width, sizeof U->width);
+    U->dim = G->dim;
+    U->periodic = G->probs[p].periodic;
+    U->n = G->offsets[p + 1] - g0;
+    U->nf = f1 - f0;
+    U->nl = l1 - l0;
+    U->d_cell_loc = G->cell_loc + 3 * g0;
+    U->d_cell_cen = G->cell_cen + 3 * g0;
+    U->d_cell_vol = G->cell_vol + g0;
+    U->d_cell_sr = G->cell_sr + g0;
+    U->d_cell_foff = G->cell_foff + g0;
+    U->d_cell_fcnt = G->cell_fcnt + g0;
+    U->d_cell_idx = G->cell_idx + g0;
+    U->d_npop = G->npop + g0;
+    U->d_face_left = G->face_left + f0;
+    U->d_face_right = G->face_right + f0;
+    U->d_face_area = G->face_area + f0;
+    U->d_face_cen = G->face_cen + 3 * f0;
+    U->d_face_norm = G->face_norm + 3 * f0;
+    U->d_face_shift = G->face_shift + 3 * f0;
+    U->d_face_has_shift = G->face_has_shift + f0;
+    U->d_links = G->links + l0;
+    U->stats = G->plan_stats;
+    U->stats.passes = G->npass;
+    U->stats.ms_total = ms;
+    out[p] = U;
+  }
+  if (replayed) *replayed = 1;
+  return MV_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------
 extern "C" {
@@ -2305,6 +3171,27 @@ mv_status mv_build_batch_device(const double *d_xyz, const uint64_t *offsets, ui
   return run_build_batch(d_xyz, offsets, P, d_mask, anchors, widths, periodic, dimensionality,
                          opt, out);
 }
+
+mv_status mv_graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
+                          const uint8_t *d_mask, const double *anchors, const double *widths,
+                          const uint32_t *periodic, uint32_t dimensionality,
+                          const mv_options *opt, mv_graph **out) {
+  if (!out || !valid_batch(offsets, P, dimensionality, anchors, widths)) return MV_E_ARG;
+  *out = nullptr;
+  if (!d_xyz) return MV_E_ARG;
+  if (mv_status st = device_ready()) return st;
+  return graph_create(d_xyz, offsets, P, d_mask, anchors, widths, periodic, dimensionality, opt,
+                      out);
+}
+
+mv_status mv_graph_launch(mv_graph *g, mv_tess **out, uint32_t *replayed) {
+  if (!g || !out) return MV_E_ARG;
+  for (uint32_t p = 0; p < g->P; p++) out[p] = nullptr;
+  if (replayed) *replayed = 0;
+  return graph_launch(g, out, replayed);
+}
+
+void mv_graph_free(mv_graph *g) { delete g; }
 
 mv_status mv_build_batch(const double *xyz, const uint64_t *offsets, uint32_t P,
                          const uint8_t *mask, const double *anchors, const double *widths,

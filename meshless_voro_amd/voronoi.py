@@ -23,7 +23,7 @@ import numpy as np
 from . import _native
 
 __all__ = ["Dimensionality", "Voronoi", "VoronoiCell", "VoronoiFace", "VoronoiError",
-           "BuildOptions", "MODE_PLAIN", "MODE_LEARNED"]
+           "BuildOptions", "BuildGraph", "MODE_PLAIN", "MODE_LEARNED"]
 
 MODE_PLAIN = 0
 MODE_LEARNED = 1
@@ -278,6 +278,62 @@ def _build_batch_handles(problems, dimensionality, masks, options, flags):
     if st != 0:
         raise VoronoiError(st)
     return [C.c_void_p(hs[p]) for p in range(P)]
+
+
+class BuildGraph:
+    """A captured build (mv_graph_create / mv_graph_launch; no counterpart in
+    the reference): the build of P problems over a device array of seeds,
+    recorded once as a hipGraph and replayed on the array's current
+    contents.  `xyz` is a contiguous float64 tensor of shape (n, 3) on the
+    GPU (the caller may rewrite it in place between launches and must keep
+    it alive), `problems` a sequence of (offset, anchor, width, periodic)
+    per problem with offsets ascending from 0, `mask` an optional uint8 GPU
+    tensor of n entries.  launch() returns (list of Voronoi, replayed):
+    replayed is False when the replay needed more than was recorded and the
+    ordinary build ran instead; the results are the same either way."""
+
+    def __init__(self, xyz, problems, dimensionality=Dimensionality.ThreeD, mask=None,
+                 options: Optional[BuildOptions] = None):
+        self._h = None
+        self._xyz, self._mask = xyz, mask
+        n = int(xyz.shape[0])
+        P = len(problems)
+        self.P = P
+        offsets = np.zeros(P + 1, np.uint64)
+        offsets[:P] = [int(o) for o, _, _, _ in problems]
+        offsets[P] = n
+        anchors = np.ascontiguousarray(np.stack([_vec3(a) for _, a, _, _ in problems]))
+        widths = np.ascontiguousarray(np.stack([_vec3(w) for _, _, w, _ in problems]))
+        per = np.ascontiguousarray(np.array([int(bool(p)) for _, _, _, p in problems], np.uint32))
+        opt = (options or BuildOptions()).to_c()
+        h = C.c_void_p()
+        st = _native.lib().mv_graph_create(
+            C.c_void_p(xyz.data_ptr()), offsets.ctypes.data, P,
+            C.c_void_p(mask.data_ptr()) if mask is not None else None, anchors.ctypes.data,
+            widths.ctypes.data, per.ctypes.data, int(Dimensionality(dimensionality)),
+            C.byref(opt), C.byref(h))
+        if st != 0:
+            raise VoronoiError(st)
+        self._h = h
+
+    def launch(self):
+        hs = (C.c_void_p * self.P)()
+        rep = C.c_uint32(0)
+        st = _native.lib().mv_graph_launch(self._h, hs, C.byref(rep))
+        if st != 0:
+            raise VoronoiError(st)
+        return [Voronoi._from_handle(C.c_void_p(hs[p])) for p in range(self.P)], bool(rep.value)
+
+    def close(self):
+        if self._h is not None:
+            _native.lib().mv_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class VoronoiCell:
