@@ -109,7 +109,8 @@ def batched_c5(L, mode, reps=3, only_uniform=False):
     xt = torch.from_numpy(xyz).to("cuda:0")
     mt = torch.from_numpy(mask).to("cuda:0")
     from meshless_voro_amd.autotune import tuned_options
-    opt = tuned_options(int(sum(sizes)), True, mode, kind="batched").to_c()
+    bo = tuned_options(int(sum(sizes)), True, mode, kind="batched")
+    opt = bo.to_c()
     hs = (C.c_void_p * P)()
 
     def batched(use_mask):
@@ -150,6 +151,50 @@ def batched_c5(L, mode, reps=3, only_uniform=False):
             best = min(best, time.perf_counter() - t0)
         return best
 
+    def captured(use_mask, reps_g):
+        """The same batched build recorded once as a hipGraph
+        (mv_graph_create) and replayed (mv_graph_launch): creation time,
+        median replay time, and whether replays ran the recording (also after
+        every seed moved by ~1e-4)."""
+        from meshless_voro_amd import BuildGraph
+        prob_list = [(int(offsets[p]), [0., 0., 0.], [1., 1., 1.], bool(per[p])) for p in range(P)]
+        t0 = time.perf_counter()
+        g = BuildGraph(xt, prob_list, mask=mt if use_mask else None, options=bo)
+        t_create = time.perf_counter() - t0
+        flags = []
+
+        def replay():
+            rep = C.c_uint32(0)
+            st = L.mv_graph_launch(g._h, hs, C.byref(rep))
+            if st != 0:
+                raise VoronoiError(st)
+            flags.append(int(rep.value))
+            for p in range(P):
+                L.mv_free(hs[p])
+
+        replay()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps_g):
+            t0 = time.perf_counter()
+            replay()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        keep = xt.clone()
+        gen = torch.Generator(device="cuda:0")
+        gen.manual_seed(7)
+        xt.add_(1e-4 * torch.randn(xt.shape, generator=gen, device="cuda:0", dtype=xt.dtype))
+        xt.remainder_(1.0)
+        replay()
+        moved_ok = flags[-1]
+        moved_viol = g.last_violation()
+        xt.copy_(keep)
+        torch.cuda.synchronize()
+        g.close()
+        return {"graph_create_ms": 1e3 * t_create, "graph_ms": 1e3 * float(np.median(ts)),
+                "graph_replayed": sum(flags[:-1]) / max(len(flags) - 1, 1),
+                "graph_replayed_after_move": moved_ok, "graph_violation_after_move": moved_viol}
+
     n = int(offsets[-1])
     nm = int(mask.sum())
     out = {"problems": P, "seeds": n, "periodic_problems": int(per.sum())}
@@ -159,6 +204,8 @@ def batched_c5(L, mode, reps=3, only_uniform=False):
         out[name] = {"cells": cells, "batched_ms": 1e3 * tb, "sequential_ms": 1e3 * ts,
                      "batched_cells_per_s": cells / tb, "sequential_cells_per_s": cells / ts,
                      "batched_speedup": ts / tb}
+        print(f"bench:   captured {name}", file=sys.stderr, flush=True)
+        out[name].update(captured(use_mask, max(reps, 3)))
         print(f"bench:   {name}: {json.dumps(out[name])}", file=sys.stderr, flush=True)
     return out
 

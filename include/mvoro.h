@@ -495,6 +495,11 @@ mv_status mv_graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t
                           const mv_options *opt, mv_graph **out);
 mv_status mv_graph_launch(mv_graph *g, mv_tess **out, uint32_t *replayed);
 void mv_graph_free(mv_graph *g);
+/* Why the last launch did not replay (0: it did): bit 0 a seed outside the
+ * recorded grid, 1 more cells for a pass than recorded, 2 more list
+ * entries, 3 an error status or full face staging, 4 cells left after the
+ * last recorded pass, 5 a cell over the large-cell kernel's capacity. */
+uint32_t mv_graph_last_violation(const mv_graph *g);
 
 /* Test hook, no counterpart in the reference: the number of hipMalloc calls
  * the library has made in this process. */

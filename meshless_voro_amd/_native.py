@@ -134,7 +134,7 @@ EXPORTS = [
     "mv_model_predict", "mv_model_loss_grad", "mv_in_sphere_exact", "mv_kept_cells",
     "mv_with_faces", "mv_discard_faces", "mv_face_vertices", "mv_voronoi_from_integrator",
     "mv_test_run_shares", "mv_build_device_noalloc", "mv_test_device_mallocs",
-    "mv_graph_create", "mv_graph_launch", "mv_graph_free",
+    "mv_graph_create", "mv_graph_launch", "mv_graph_free", "mv_graph_last_violation",
 ]
 
 
@@ -171,6 +171,8 @@ def lib():
     L.mv_graph_launch.restype = C.c_int
     L.mv_graph_launch.argtypes = [vp, vp, C.POINTER(u32)]
     L.mv_graph_free.argtypes = [vp]
+    L.mv_graph_last_violation.restype = u32
+    L.mv_graph_last_violation.argtypes = [vp]
     L.mv_anchor.argtypes = [vp, vp]
     L.mv_width.argtypes = [vp, vp]
     for f in ("mv_dimensionality", "mv_periodic"):

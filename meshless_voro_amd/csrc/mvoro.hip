@@ -2362,13 +2362,19 @@ struct mv_graph {
   uint8_t *face_has_shift = nullptr;
   uint64_t *links = nullptr;
   unsigned long long *h_status = nullptr;  // pinned
-  hipStream_t s = nullptr;
+  // the recorded stream, and a second one the large-cell tier forks onto
+  hipStream_t s = nullptr, s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   mv_stats plan_stats{};
+  uint32_t last_violation = 0;  // violation bits of the last replay
   ~mv_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
     if (h_status) (void)hipHostFree(h_status);
   }
@@ -2397,8 +2403,8 @@ void graph_body(mv_graph &G) {
       hipLaunchKernelGGL(k_fill_u32, dim3(nblk(k)), dim3(256), 0, s, (uint32_t *)p, (uint32_t)k,
                          v);
   };
-  auto fill8 = [&](void *p, size_t k, uint8_t v) {
-    if (k) hipLaunchKernelGGL(k_fill_u8, dim3(nblk(k)), dim3(256), 0, s, (uint8_t *)p, k, v);
+  auto fill8 = [&](void *p, size_t k, uint8_t v, hipStream_t st) {
+    if (k) hipLaunchKernelGGL(k_fill_u8, dim3(nblk(k)), dim3(256), 0, st, (uint8_t *)p, k, v);
   };
   auto fill64 = [&](void *p, size_t k, unsigned long long v) {
     if (k)
@@ -2484,34 +2490,37 @@ void graph_body(mv_graph &G) {
   scan_excl_dev<uint32_t, uint32_t>(flag, pos, n, s);
   hipLaunchKernelGGL(k_compact, dim3(nb), dim3(256), 0, s, ident, flag, pos, (uint32_t)n,
                      lists[0]);
+  // the first pass's capacity: the planning build's active cells and more
+  const uint32_t cap0 = G.capM[0];
+  const unsigned nb0 = nblk(cap0);
   hipLaunchKernelGGL(k_graph_count, dim3(1), dim3(1), 0, s, (const uint32_t *)flag,
-                     (const uint32_t *)pos, (uint32_t)n, (const uint32_t *)nullptr, cm,
-                     (uint32_t)n, GV_ITEMS, viol);
-  hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lists[0], (const uint32_t *)cm,
-                     (uint32_t)n, 0u);
+                     (const uint32_t *)pos, (uint32_t)n, (const uint32_t *)nullptr, cm, cap0,
+                     GV_ITEMS, viol);
+  hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb0), dim3(256), 0, s, lists[0], (const uint32_t *)cm,
+                     cap0, 0u);
   const uint32_t kmax =
       std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : MV_K_MAX, MV_K_MAX), 2);
   const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
   const float *Rm = nullptr;
   if (opt.mode == MV_MODE_LEARNED) {
-    float *F = need(TT.get<float>(n * NF));
-    float *hl = need(TT.get<float>(n));
-    float *Y = need(TT.get<float>(n * NO));
-    float *R0 = need(TT.get<float>(n));
-    hipLaunchKernelGGL(k_features<true>, dim3(nb), dim3(256), 0, s, spos, cell_start, PR,
-                       (const FeatParams *)G.d_fpd, lists[0], (uint32_t)n, F, hl);
-    hipLaunchKernelGGL(k_features<false>, dim3(nb), dim3(256), 0, s, spos, cell_start, PR,
-                       (const FeatParams *)G.d_fpd, lists[0], (uint32_t)n, F, hl);
-    const uint32_t tiles = (uint32_t)((n + 15) / 16);
+    float *F = need(TT.get<float>((size_t)cap0 * NF));
+    float *hl = need(TT.get<float>(cap0));
+    float *Y = need(TT.get<float>((size_t)cap0 * NO));
+    float *R0 = need(TT.get<float>(cap0));
+    hipLaunchKernelGGL(k_features<true>, dim3(nb0), dim3(256), 0, s, spos, cell_start, PR,
+                       (const FeatParams *)G.d_fpd, lists[0], cap0, F, hl);
+    hipLaunchKernelGGL(k_features<false>, dim3(nb0), dim3(256), 0, s, spos, cell_start, PR,
+                       (const FeatParams *)G.d_fpd, lists[0], cap0, F, hl);
+    const uint32_t tiles = (cap0 + 15) / 16;
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)), dim3(256),
-                       0, s, (const DevModel *)G.d_model, (const float *)F, (uint32_t)n, Y);
+                       0, s, (const DevModel *)G.d_model, (const float *)F, cap0, Y);
     const float kscale = G.model.kscale > 0.f ? G.model.kscale : 1.f;
     const float rscale = G.model.rscale > 0.f ? G.model.rscale : 0.f;
-    hipLaunchKernelGGL(k_proposal, dim3(nb), dim3(256), 0, s, (const float *)Y,
-                       (const float *)hl, (uint32_t)n, std::min<uint32_t>(16, kmax), G.lbound0,
+    hipLaunchKernelGGL(k_proposal, dim3(nb0), dim3(256), 0, s, (const float *)Y,
+                       (const float *)hl, cap0, std::min<uint32_t>(16, kmax), G.lbound0,
                        kscale, rscale, lens[0], R0);
-    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lens[0],
-                       (const uint32_t *)cm, (uint32_t)n, 0u);
+    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb0), dim3(256), 0, s, lens[0],
+                       (const uint32_t *)cm, cap0, 0u);
     if (opt.flags & MV_FLAG_NO_LENGTH_ORDER) {
       Rm = R0;
     } else {
@@ -2522,21 +2531,20 @@ void graph_body(mv_graph &G) {
       uint32_t *bfill = need(TT.get<uint32_t>(NB));
       fill32(hist, NB + 1, 0u);
       fill32(bfill, NB, 0u);
-      const uint32_t lblocks = (uint32_t)((n + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK);
-      hipLaunchKernelGGL(k_len_hist, dim3(lblocks), dim3(256), 0, s, lens[0], (uint32_t)n, NB,
-                         hist);
+      const uint32_t lblocks = (cap0 + LEN_PER_BLOCK - 1) / LEN_PER_BLOCK;
+      hipLaunchKernelGGL(k_len_hist, dim3(lblocks), dim3(256), 0, s, lens[0], cap0, NB, hist);
       scan_excl_dev<uint32_t, uint32_t>(hist, bst, NB + 1, s);
-      float *Rs = need(TT.get<float>(n));
+      float *Rs = need(TT.get<float>(cap0));
       hipLaunchKernelGGL(k_len_scatter, dim3(lblocks), dim3(256), 0, s, lists[0], lens[0], R0,
-                         (uint32_t)n, NB, bst, bfill, lists[4], lens[4], Rs);
+                         cap0, NB, bst, bfill, lists[4], lens[4], Rs);
       std::swap(lists[0], lists[4]);
       std::swap(lens[0], lens[4]);
       Rm = Rs;
     }
   } else {
-    fill32(lens[0], n, k0);
-    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb), dim3(256), 0, s, lens[0],
-                       (const uint32_t *)cm, (uint32_t)n, 0u);
+    fill32(lens[0], cap0, k0);
+    hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb0), dim3(256), 0, s, lens[0],
+                       (const uint32_t *)cm, cap0, 0u);
   }
 
   // ---- per-cell outputs, face staging, pass counters
@@ -2588,8 +2596,67 @@ void graph_body(mv_graph &G) {
     double *need_m = need(TT.get<double>(std::max<uint32_t>(capM, 1)));
     double *need_b = need(TT.get<double>(std::max<uint32_t>(capB, 1)));
     double *kd_m = nullptr, *kd_b = nullptr;
+    // the large-cell tier (few cells, long lists and latency) on a stream of
+    // its own beside the main tier, as in the ordinary build
+    const bool fork = capB && capM;
+    hipStream_t sb = fork ? G.s2 : s;
+    if (fork) {
+      HIPCHK(hipEventRecord(G.ev_fork, s));
+      HIPCHK(hipStreamWaitEvent(G.s2, G.ev_fork, 0));
+    }
+    if (capB) {
+      fill8(bstate, capB, (uint8_t)CS_DONE, sb);
+      uint64_t *kofs = need(TT.get<uint64_t>(capB));
+      scan_excl_dev<uint32_t, uint64_t>(lens[ib], kofs, capB, sb);
+      hipLaunchKernelGGL(k_graph_entries, dim3(1), dim3(1), 0, sb, (const uint32_t *)lens[ib],
+                         (const uint64_t *)kofs, capB, (uint64_t)G.EB[p], cb + p, viol);
+      uint64_t *kpk = need(TT.get<uint64_t>(G.EB[p]));
+      uint32_t *kcnt = need(TT.get<uint32_t>(capB));
+      double *kdist = need(TT.get<double>(capB));
+      kd_b = kdist;
+      launch_knn(spos, cell_start, PR, lists[ib], capB, G.LbB[p], lens[ib], Rb, kofs, kpk, kcnt,
+                 kdist, true, TT, sb, 0, cb + p, std::min<uint32_t>(capB, BIG_KNN_BLOCKS));
+      ClipArgs A;
+      memset(&A, 0, sizeof A);
+      A.spos = spos;
+      A.mask = G.d_mask;
+      A.PR = PR;
+      A.work = lists[ib];
+      A.nwork = capB;
+      A.nsub = cb + p;
+      A.kpk = kpk;
+      A.kofs = kofs;
+      A.kcnt = kcnt;
+      A.kdist = kdist;
+      A.need_r = need_b;
+      A.lanes = std::min<uint32_t>(8192, capB);
+      A.wstate = bstate;
+      A.cell_vol = G.cell_vol;
+      A.cell_cen = G.cell_cen;
+      A.cell_sr = G.cell_sr;
+      A.cell_nf = cell_nf;
+      A.link_in = link_in;
+      A.cell_fbase = cell_fbase;
+      A.cell_npop = G.npop;
+      A.st_cell = st_cell;
+      A.st_right = st_right;
+      A.st_img = st_img;
+      A.st_lslot = st_lslot;
+      A.st_area = st_area;
+      A.st_cen = st_cen;
+      A.st_norm = st_norm;
+      A.st_cap = st_cap;
+      A.st_counter = counters;
+      A.err_word = counters + 1;
+      A.exact_counter = counters + 2;
+      A.overflow = counters + 3;
+      A.coop_maxp = opt.coop_max_planes;
+      A.coop_maxv = opt.coop_max_vertices;
+      hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>),
+                         dim3(std::max<uint32_t>(A.lanes, 1)), dim3(64), 0, sb, A);
+    }
     if (capM) {
-      fill8(wstate, capM, (uint8_t)CS_DONE);
+      fill8(wstate, capM, (uint8_t)CS_DONE, s);
       fill32(queue, 5, 0u);
       uint64_t *kofs = need(TT.get<uint64_t>(capM));
       scan_excl_dev<uint32_t, uint64_t>(lens[im], kofs, capM, s);
@@ -2645,56 +2712,9 @@ void graph_body(mv_graph &G) {
       X.queue = queue + 2;
       hipLaunchKernelGGL((k_clip_wave<false, true>), dim3(blocks), dim3(256), 0, s, X);
     }
-    if (capB) {
-      fill8(bstate, capB, (uint8_t)CS_DONE);
-      uint64_t *kofs = need(TT.get<uint64_t>(capB));
-      scan_excl_dev<uint32_t, uint64_t>(lens[ib], kofs, capB, s);
-      hipLaunchKernelGGL(k_graph_entries, dim3(1), dim3(1), 0, s, (const uint32_t *)lens[ib],
-                         (const uint64_t *)kofs, capB, (uint64_t)G.EB[p], cb + p, viol);
-      uint64_t *kpk = need(TT.get<uint64_t>(G.EB[p]));
-      uint32_t *kcnt = need(TT.get<uint32_t>(capB));
-      double *kdist = need(TT.get<double>(capB));
-      kd_b = kdist;
-      launch_knn(spos, cell_start, PR, lists[ib], capB, G.LbB[p], lens[ib], Rb, kofs, kpk, kcnt,
-                 kdist, true, TT, s, 0, cb + p, std::min<uint32_t>(capB, BIG_KNN_BLOCKS));
-      ClipArgs A;
-      memset(&A, 0, sizeof A);
-      A.spos = spos;
-      A.mask = G.d_mask;
-      A.PR = PR;
-      A.work = lists[ib];
-      A.nwork = capB;
-      A.nsub = cb + p;
-      A.kpk = kpk;
-      A.kofs = kofs;
-      A.kcnt = kcnt;
-      A.kdist = kdist;
-      A.need_r = need_b;
-      A.lanes = std::min<uint32_t>(8192, capB);
-      A.wstate = bstate;
-      A.cell_vol = G.cell_vol;
-      A.cell_cen = G.cell_cen;
-      A.cell_sr = G.cell_sr;
-      A.cell_nf = cell_nf;
-      A.link_in = link_in;
-      A.cell_fbase = cell_fbase;
-      A.cell_npop = G.npop;
-      A.st_cell = st_cell;
-      A.st_right = st_right;
-      A.st_img = st_img;
-      A.st_lslot = st_lslot;
-      A.st_area = st_area;
-      A.st_cen = st_cen;
-      A.st_norm = st_norm;
-      A.st_cap = st_cap;
-      A.st_counter = counters;
-      A.err_word = counters + 1;
-      A.exact_counter = counters + 2;
-      A.overflow = counters + 3;
-      A.coop_maxp = opt.coop_max_planes;
-      A.coop_maxv = opt.coop_max_vertices;
-      hipLaunchKernelGGL((k_clip_coop<MAXP_BIG, MAXV_BIG, false>),
-                         dim3(std::max<uint32_t>(A.lanes, 1)), dim3(64), 0, s, A);
+    if (fork) {
+      HIPCHK(hipEventRecord(G.ev_join, G.s2));
+      HIPCHK(hipStreamWaitEvent(s, G.ev_join, 0));
     }
     hipLaunchKernelGGL(k_graph_pass_check, dim3(1), dim3(1), 0, s,
                        (const unsigned long long *)counters, (const unsigned int *)cap_err, viol);
@@ -2899,9 +2919,10 @@ void plan_graph(mv_graph &G, const BuildTrace &tr) {
   for (uint32_t p = 0; p < G.npass; p++) {
     if (p == 0) {
       const PassRec &r = tr.passes[0];
-      G.capM[0] = (uint32_t)n;
+      // the active cells (all, or the masked ones) and a tenth more
+      G.capM[0] = grow(r.cnt[0], 1.1, 1024);
       G.LbM[0] = std::max<uint32_t>(r.lbnd[0], 2);
-      G.EM[0] = (uint64_t)((double)r.total[0] * 1.1) + 4ull * n + 4096;
+      G.EM[0] = (uint64_t)((double)r.total[0] * 1.1) + 4ull * G.capM[0] + 4096;
       continue;
     }
     if (p < np) {
@@ -2921,8 +2942,9 @@ void plan_graph(mv_graph &G, const BuildTrace &tr) {
       G.capB[p] = grow(0, 1.0, 256);
       G.LbM[p] = std::max<uint32_t>(4 * std::max<uint32_t>(last.lbnd[0], 2), 64);
       G.LbB[p] = std::max<uint32_t>(4 * last.lbnd[1], 2048);
-      G.EM[p] = (uint64_t)G.capM[p] * G.LbM[p] + 4096;
-      G.EB[p] = (uint64_t)G.capB[p] * 1024 + 65536;
+      // (lists grow at most fourfold from one pass to the next)
+      G.EM[p] = std::max<uint64_t>((uint64_t)G.capM[p] * G.LbM[p], 4 * last.total[0]) + 4096;
+      G.EB[p] = std::max<uint64_t>((uint64_t)G.capB[p] * 1024, 4 * last.total[1]) + 65536;
     }
   }
   G.st_cap = std::max<uint64_t>(tr.F + tr.F / 4, 4096);
@@ -3032,6 +3054,9 @@ mv_status graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
     }
     (void)device_cus();
     HIPCHK(hipStreamCreateWithFlags(&G->s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
     // record once against a counting arena (its addresses are only kernel
     // arguments of a graph that never runs) to size the arena, then for real
     {
@@ -3070,6 +3095,7 @@ mv_status graph_launch(mv_graph *G, mv_tess **out, uint32_t *replayed) {
     return st;
   }
   const unsigned long long *st = G->h_status;
+  G->last_violation = (uint32_t)st[0] | (st[1] != ~0ull ? (uint32_t)GV_PASS : 0u);
   if (st[0] != 0ull || st[1] != ~0ull) {
     // the replay needed more than was recorded (or met an error): the
     // ordinary build
@@ -3192,6 +3218,8 @@ mv_status mv_graph_launch(mv_graph *g, mv_tess **out, uint32_t *replayed) {
 }
 
 void mv_graph_free(mv_graph *g) { delete g; }
+
+uint32_t mv_graph_last_violation(const mv_graph *g) { return g ? g->last_violation : 0u; }
 
 mv_status mv_build_batch(const double *xyz, const uint64_t *offsets, uint32_t P,
                          const uint8_t *mask, const double *anchors, const double *widths,

@@ -324,6 +324,11 @@ class BuildGraph:
             raise VoronoiError(st)
         return [Voronoi._from_handle(C.c_void_p(hs[p])) for p in range(self.P)], bool(rep.value)
 
+    def last_violation(self) -> int:
+        """Why the last launch ran the ordinary build (0: it replayed): bits
+        of mv_graph_last_violation."""
+        return int(_native.lib().mv_graph_last_violation(self._h))
+
     def close(self):
         if self._h is not None:
             _native.lib().mv_graph_free(self._h)
