@@ -219,13 +219,16 @@ __device__ __forceinline__ void wc_set(int &cyc, int lane, int at, int val) {
   cyc = lane == at ? val : cyc;
 }
 
-// KEEP: also append every completed cell to the kept-cell store.  EXACT:
+// RESUME: item i of A.sub continues the cell A.resume[i] that the pair kernel
+// handed over (its own instantiation, so that the others keep their
+// registers).  KEEP: also append every completed cell to the kept-cell
+// store.  EXACT:
 // decide near-degenerate tests with the exact in-sphere predicate; without
 // it (its call holds ~30 VGPRs) the kernel fits 96 VGPRs, i.e. five
 // wavefronts per SIMD (as many as the 8 KiB of LDS per wavefront allow), and
 // hands such cells (rare outside lattices) to the EXACT instantiation
 // through A.exact_list.
-template <bool KEEP, bool EXACT>
+template <bool KEEP, bool EXACT, bool RESUME = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(EXACT || KEEP ? 3 : 5)))
 k_clip_wave(WaveArgs A) {
@@ -236,7 +239,7 @@ k_clip_wave(WaveArgs A) {
 
   // nsub: the item count on the device, bounded by nwork (or by the
   // hand-over capacity when resuming)
-  const uint32_t nwork = A.nsub ? min(*A.nsub, A.resume ? A.ho_cap : A.nwork) : A.nwork;
+  const uint32_t nwork = A.nsub ? min(*A.nsub, RESUME ? A.ho_cap : A.nwork) : A.nwork;
   uint32_t next = 0;
   if (lane == 0) next = atomicAdd(A.queue, 1u);
   next = (uint32_t)rl_i((int)next, 0);
@@ -263,7 +266,7 @@ k_clip_wave(WaveArgs A) {
     uint8_t state = CS_DONE;
     uint32_t err = 0, npop = 0, nexact = 0;
     // a cell handed over by the pair kernel continues from its saved state
-    const Handover *R = A.resume ? A.resume + next : nullptr;
+    const Handover *R = RESUME ? A.resume + next : nullptr;
 
     // ---- ConvexCell::init (convex_cell.rs:294-312)
     if (R) {

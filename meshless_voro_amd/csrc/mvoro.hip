@@ -1349,7 +1349,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
             R.nsub = W.ho_cnt;
             R.queue = queue + 3;
             R.resume = W.ho;
-            hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, R);
+            hipLaunchKernelGGL((k_clip_wave<false, false, true>), dim3(blocks), dim3(256), 0, s,
+                               R);
           } else {
             hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(blocks), dim3(256), 0, s, W);
           }
