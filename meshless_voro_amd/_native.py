@@ -51,7 +51,7 @@ class mv_options(C.Structure):
                 ("clip_blocks_per_cu", C.c_uint32),
                 ("coop_max_planes", C.c_uint32), ("coop_max_vertices", C.c_uint32),
                 ("cells_per_wave", C.c_uint32), ("first_list_cap", C.c_uint32),
-                ("knn_min_tile", C.c_uint32), ("reserved", C.c_uint32)]
+                ("knn_min_tile", C.c_uint32), ("pipeline", C.c_uint32)]
 
 
 MV_MODEL_MAGIC = 0x4B4C564D

@@ -79,6 +79,7 @@ struct WaveArgs {
   Problems PR;            // boxes, grids (total stream) per problem
   const uint32_t *work;
   uint32_t nwork;
+  uint32_t w0;               // first item (items w0 .. w0 + nwork - 1 without sub)
   const uint32_t *sub;       // items to build (nullptr: all nwork) ...
   const unsigned int *nsub;  // ... and their count, read on the device
   uint32_t *exact_list;      // items that met a near-degenerate test ...
@@ -245,7 +246,7 @@ k_clip_wave(WaveArgs A) {
   next = (uint32_t)rl_i((int)next, 0);
   for (;;) {
     if (next >= nwork) break;
-    const uint32_t w = A.sub ? A.sub[next] : next;
+    const uint32_t w = A.sub ? A.sub[next] : A.w0 + next;
     // claim the following item now; its latency overlaps this cell
     uint32_t nx = 0;
     if (lane == 0) nx = atomicAdd(A.queue, 1u);
@@ -430,11 +431,14 @@ k_clip_wave(WaveArgs A) {
           for (;;) {
             if (t == tbad) break;
             const bool two = t + 1 < nb && t + 1 != tbad;
-            const double4 e2 = S.u.e.nd[min(t + 2 + hi, 63)];
-            const double2 de2 = S.u.e.de[min(t + 2 + hi, 63)];
             const uint64_t ns = wave_ballot(!(vs < de.x));
             const double c = dot3(xyz(e), vl) - e.w;
             const uint64_t tm = wave_ballot(fabs(c) < de.y) | wave_ballot(c < 0.);
+            // the next pair, read into the same registers now that they are
+            // used: the loads overlap the decisions below, and no register
+            // copy is needed at the end of the pass
+            e = S.u.e.nd[min(t + 2 + hi, 63)];
+            de = S.u.e.de[min(t + 2 + hi, 63)];
             if (!(ns & act_m)) {
               stop_hit = true;
               break;
@@ -447,8 +451,6 @@ k_clip_wave(WaveArgs A) {
             }
             if ((tm >> 32) & act_m) break;
             if (++t >= nb) break;
-            e = e2;
-            de = de2;
           }
           // a pair cut short by the bad entry ends on it: t == tbad
         } else {

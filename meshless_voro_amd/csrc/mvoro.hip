@@ -691,17 +691,21 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
                 uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
                 hipStream_t s, uint32_t min_tile = 0, const uint32_t *dn = nullptr,
-                uint32_t big_blocks = 0) {
+                uint32_t big_blocks = 0, uint32_t max_blocks_knn = 0) {
+  // max_blocks_knn: a smaller grid (the kernels loop over their items)
+  auto grid = [&](uint32_t b) {
+    return dim3(max_blocks_knn ? std::min(b, max_blocks_knn) : b);
+  };
   if (cnt == 0) return;
   if (!big && min_tile > 512 && L <= 512) L = min_tile >= 2048 ? 1024u : 512u;
   if (!big && L <= 256) {
-    hipLaunchKernelGGL((k_knn<512, 4>), dim3((cnt + 3) / 4), dim3(256), 0, s, spos,
+    hipLaunchKernelGGL((k_knn<512, 4>), grid((cnt + 3) / 4), dim3(256), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else if (!big && L <= 512) {
-    hipLaunchKernelGGL((k_knn<1024, 2>), dim3((cnt + 1) / 2), dim3(128), 0, s, spos,
+    hipLaunchKernelGGL((k_knn<1024, 2>), grid((cnt + 1) / 2), dim3(128), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else if (!big && L <= 1024) {
-    hipLaunchKernelGGL((k_knn<2048, 1>), dim3(cnt), dim3(64), 0, s, spos,
+    hipLaunchKernelGGL((k_knn<2048, 1>), grid(cnt), dim3(64), 0, s, spos,
                        cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
   } else {
     uint32_t cap = BIG_KNN_CAP;
@@ -1243,6 +1247,13 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       double *nr_wide = need(TT.get<double>(std::max<uint32_t>(nwide, 1)));
       // this pass's kdist per list
       const double *kd_main = nullptr, *kd_big = nullptr, *kd_wide = nullptr;
+      // events of a pipelined first pass (destroyed once the pass is done)
+      struct PipeEvents {
+        std::vector<hipEvent_t> ev;
+        ~PipeEvents() {
+          for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+      } pev;
       // one list on one stream; untimed runs do not wait for their kernels
       // tier 0: the main list (wavefront kernel), 1: the large-cell kernel,
       // 2: the wide kernel; tiers 1 and 2 gather with the long-list kernel
@@ -1273,9 +1284,38 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           r.lbnd[tier] = lbnd;
           r.total[tier] = total;
         }
+        // pipelined first pass (main list, one cell per wavefront): the lists
+        // of chunk k + 1 are built on the second stream, with a few blocks
+        // per CU, while chunk k is clipped (the clip kernel's blocks leave
+        // room for them and the list kernel's work fills the VALU issue slots
+        // the clipper leaves)
+        const uint32_t pchunks = opt.pipeline & 0xFFu;
+        const uint32_t pblocks = std::max<uint32_t>((opt.pipeline >> 8) & 0xFFu, 1u);
+        const bool pipe = pchunks > 1 && passes == 1 && tier == 0 && !exact_order && !T->keep &&
+                          opt.cells_per_wave != 2 && s2 != s && cnt >= 64u * pchunks;
+        std::vector<uint32_t> cbnd;
         Timer tkk(s, timed);
-        launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
-                   s, opt.knn_min_tile);
+        if (!pipe) {
+          launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
+                     s, opt.knn_min_tile);
+        } else {
+          cbnd.resize(pchunks + 1);
+          for (uint32_t k = 0; k <= pchunks; k++) cbnd[k] = (uint32_t)((uint64_t)cnt * k / pchunks);
+          pev.ev.assign(pchunks, nullptr);
+          for (hipEvent_t &e : pev.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+          // the second stream starts after the offsets
+          HIPCHK(hipEventRecord(pev.ev[0], s));
+          HIPCHK(hipStreamWaitEvent(s2, pev.ev[0], 0));
+          launch_knn(spos, cell_start, PR, wl, cbnd[1], lbnd, ll, R0, kofs, kpk, kcnt, kdist, false,
+                     TT, s, opt.knn_min_tile);
+          for (uint32_t k = 1; k < pchunks; k++) {
+            const uint32_t c0 = cbnd[k];
+            launch_knn(spos, cell_start, PR, wl + c0, cbnd[k + 1] - c0, lbnd, ll + c0,
+                       R0 ? R0 + c0 : nullptr, kofs + c0, kpk, kcnt + c0, kdist + c0, false, TT,
+                       s2, opt.knn_min_tile, nullptr, 0, (uint32_t)device_cus() * pblocks);
+            HIPCHK(hipEventRecord(pev.ev[k], s2));
+          }
+        }
         ms_knn_k += tkk.stop();
         n_knn++;
         q_knn += cnt;
@@ -1338,6 +1378,20 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           Timer tcw(s, timed);
           if (T->keep) {
             hipLaunchKernelGGL((k_clip_wave<true, true>), dim3(blocks), dim3(256), 0, s, W);
+          } else if (pipe) {
+            // chunk k after its lists
+            unsigned int *qk = need(TT.get<unsigned int>(pchunks));
+            HIPCHK(hipMemsetAsync(qk, 0, pchunks * sizeof(unsigned int), s));
+            for (uint32_t k = 0; k < pchunks; k++) {
+              if (k) HIPCHK(hipStreamWaitEvent(s, pev.ev[k], 0));
+              WaveArgs Wk = W;
+              Wk.w0 = cbnd[k];
+              Wk.nwork = cbnd[k + 1] - cbnd[k];
+              Wk.queue = qk + k;
+              const uint32_t bk =
+                  std::min<uint32_t>((Wk.nwork + 3) / 4, (uint32_t)device_cus() * bpc);
+              hipLaunchKernelGGL((k_clip_wave<false, false>), dim3(bk), dim3(256), 0, s, Wk);
+            }
           } else if (pair) {
             // four blocks of four wavefronts fill a CU's LDS
             const uint32_t pblocks =

@@ -59,7 +59,8 @@ class BuildOptions:
                  lanes: int = 0, occupancy16: int = 0, exact_order: bool = False,
                  model=None, clip_blocks_per_cu: int = 0, length_order: bool = True,
                  coop_max_planes: int = 0, coop_max_vertices: int = 0,
-                 cells_per_wave: int = 1, first_list_cap: int = 0, knn_min_tile: int = 0):
+                 cells_per_wave: int = 1, first_list_cap: int = 0, knn_min_tile: int = 0,
+                 pipeline: int = 0):
         self.mode, self.k0, self.k_max, self.lanes = mode, k0, k_max, lanes
         self.occupancy16 = occupancy16
         self.exact_order = exact_order
@@ -73,6 +74,9 @@ class BuildOptions:
         self.cells_per_wave = cells_per_wave
         self.first_list_cap = first_list_cap
         self.knn_min_tile = knn_min_tile
+        # first pass in chunks, the next chunk's lists beside the clipping:
+        # chunks | (list blocks per CU << 8); 0 = off
+        self.pipeline = pipeline
         # learned mode: a meshless_voro_amd.learned.Model (None: the
         # committed checkpoint)
         self.model = model
@@ -90,6 +94,7 @@ class BuildOptions:
         o.cells_per_wave = self.cells_per_wave
         o.first_list_cap = self.first_list_cap
         o.knn_min_tile = self.knn_min_tile
+        o.pipeline = self.pipeline
         if self.mode == MODE_LEARNED:
             from . import learned
             m = self.model if self.model is not None else learned.default_model()
