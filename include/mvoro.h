@@ -90,7 +90,10 @@ typedef struct mv_options {
                                    multiple of 8 in [16, 1024]; 0 = 256 */
   uint32_t knn_min_tile;        /* neighbour gathers: smallest LDS tile
                                    (512, 1024 or 2048 entries); 0 = 512 */
-  uint32_t reserved;
+  uint32_t pipeline;            /* first pass: bits 0-7 = chunks C (0 or 1:
+                                   off); the lists of chunk k + 1 are built
+                                   on a second stream with bits 8-15 (0 = 1)
+                                   blocks per CU while chunk k is clipped */
 } mv_options;
 
 /* Run every cell on the thread-per-cell kernel, which sums each integral in
@@ -236,6 +239,11 @@ typedef struct mv_stats {
   uint64_t wide_cells;          /* cells moved to the wide kernel */
   uint64_t pair_handovers;      /* cells_per_wave = 2: cells continued on the
                                    one-cell kernel from their saved state */
+  uint64_t knn_big_items;       /* lists built by the long-list kernel */
+  uint64_t knn_big_gathers;     /* its ball gathers (more than one per list
+                                   when the first radius held too few or too
+                                   many entries) */
+  uint64_t knn_big_entries;     /* entries in its last ball of each list */
 } mv_stats;
 void mv_get_stats(const mv_tess *t, mv_stats *s);
 

@@ -574,14 +574,16 @@ __device__ __forceinline__ uint32_t knn_fill(
 
 // knn_fill with every wavefront of a KB_THREADS block gathering (all threads
 // call it; the count comes back through the LDS word *blk_cnt).
+// *iters: the gathers made.
 __device__ __forceinline__ uint32_t knn_fill_blk(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
     const GridParams &g, const double *width, const double q[3], double &R,
     uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt, int wid,
-    int nw) {
+    int nw, uint32_t *iters) {
   double r_lo = 0., r_hi = 0.;
   uint32_t cnt = 0;
   for (int it = 0; it < 96; it++) {
+    *iters = it + 1;
     if (threadIdx.x == 0) *blk_cnt = 0u;
     __syncthreads();
     knn_gather<true>(spos, cell_start, g, width, q, R, bkey, bpk, cap, blk_cnt, wid, nw);
@@ -962,6 +964,39 @@ __device__ __forceinline__ void flip_bitonic(K *key, P *pk, uint32_t m, uint32_t
   }
 }
 
+// The items of a long-list launch, longest lists first (64 length classes,
+// two per octave; any order within a class), and the launch's work counter
+// set to zero.  One block of 1024 threads.
+constexpr uint32_t LEN_CLASSES = 64;
+__device__ __forceinline__ uint32_t len_class(uint32_t L) {
+  if (L < 2u) return LEN_CLASSES - 1u;
+  const uint32_t b = 31u - (uint32_t)__clz((int)L);  // floor(log2 L) >= 1
+  const uint32_t half = (L >> (b - 1u)) & 1u;         // the next bit
+  return LEN_CLASSES - 1u - min(2u * b + half, LEN_CLASSES - 1u);
+}
+__global__ void __launch_bounds__(1024)
+    k_order_by_len(const uint32_t *__restrict__ Lper, uint32_t n, uint32_t *__restrict__ perm,
+                   uint32_t *__restrict__ queue) {
+  __shared__ uint32_t cnt[LEN_CLASSES];
+  const uint32_t tid = threadIdx.x;
+  if (tid < LEN_CLASSES) cnt[tid] = 0u;
+  if (tid == 0) *queue = 0u;
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += blockDim.x) atomicAdd(&cnt[len_class(Lper[i])], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t s = 0;
+    for (uint32_t c = 0; c < LEN_CLASSES; c++) {
+      const uint32_t v = cnt[c];
+      cnt[c] = s;
+      s += v;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += blockDim.x)
+    perm[atomicAdd(&cnt[len_class(Lper[i])], 1u)] = i;
+}
+
 __global__ void __launch_bounds__(KB_THREADS)
     k_knn_big(const double4 *__restrict__ spos,
               const uint32_t *__restrict__ cell_start_all, Problems PR,
@@ -972,13 +1007,15 @@ __global__ void __launch_bounds__(KB_THREADS)
               uint32_t *__restrict__ kcnt, double *__restrict__ kdist,
               double *__restrict__ gkey, uint64_t *__restrict__ gpk,
               double *__restrict__ gkey2, uint64_t *__restrict__ gpk2,
-              uint32_t cap, const uint32_t *__restrict__ dn) {
+              uint32_t cap, const uint32_t *__restrict__ dn,
+              unsigned long long *__restrict__ gstat, const uint32_t *__restrict__ perm,
+              uint32_t *__restrict__ queue) {
   if (dn) nwork = min(*dn, nwork);
   __shared__ uint32_t hist[KB_BINS];
   __shared__ double wkey[KB_WIN];
   __shared__ uint64_t wpk[KB_WIN];
   __shared__ uint32_t s_wsum[KB_THREADS / 64];
-  __shared__ uint32_t s_gather;
+  __shared__ uint32_t s_gather, s_next;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
   const uint32_t wid = tid >> 6;
@@ -986,7 +1023,17 @@ __global__ void __launch_bounds__(KB_THREADS)
   uint64_t *bpk = gpk + (size_t)blockIdx.x * cap;
   double *tkey = gkey2 + (size_t)blockIdx.x * cap;
   uint64_t *tpk = gpk2 + (size_t)blockIdx.x * cap;
-  for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+  // items in `perm` order, claimed one at a time from *queue (list costs
+  // differ by orders of magnitude), or blockIdx.x, + gridDim.x, ...
+  auto claim = [&]() -> uint32_t {
+    __syncthreads();
+    if (tid == 0) s_next = atomicAdd(queue, 1u);
+    __syncthreads();
+    return s_next;
+  };
+  for (uint32_t j = queue ? claim() : blockIdx.x; j < nwork;
+       j = queue ? claim() : j + gridDim.x) {
+    const uint32_t w = perm ? perm[j] : j;
     const uint32_t slot = work[w];
     const uint32_t pr = problem_of(PR, slot);
     const BoxParams &boxr = PR.boxes[pr];
@@ -999,8 +1046,15 @@ __global__ void __launch_bounds__(KB_THREADS)
     const GridParams g = PR.grids[pru];
     const uint32_t *cell_start = cell_start_all + PR.coff[pru];
     double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
+    uint32_t iters = 0;
     const uint32_t cnt = knn_fill_blk(spos, cell_start, g, width, q, R, Lw, bkey, bpk, cap,
-                                      &s_gather, (int)wid, KB_THREADS / 64);
+                                      &s_gather, (int)wid, KB_THREADS / 64, &iters);
+    if (gstat && tid == 0) {
+      // items, ball gathers, entries gathered by the last one
+      atomicAdd(gstat, 1ull);
+      atomicAdd(gstat + 1, (unsigned long long)iters);
+      atomicAdd(gstat + 2, (unsigned long long)cnt);
+    }
     if (cnt > cap) {
       if (tid == 0) kcnt[w] = 0xFFFFFFFFu;
       __syncthreads();
@@ -1056,8 +1110,11 @@ __global__ void __launch_bounds__(KB_THREADS)
       tpk[pos] = bpk[e];
     }
     __syncthreads();
-    // windows of whole bins, sorted back into the first buffer
-    for (uint32_t s0 = 0; s0 < cnt;) {
+    // windows of whole bins, sorted back into the first buffer, up to the
+    // one that holds entry outn (the first left out: its distance bounds
+    // the list); the bins past it stay unsorted
+    const uint32_t outn = min(cnt, Lw);
+    for (uint32_t s0 = 0; s0 < cnt && s0 <= outn;) {
       // the bins that end at most KB_WIN past s0 (bin ends do not decrease)
       uint32_t lo = 0, hi = KB_BINS;
       const uint32_t lim = s0 + KB_WIN;
@@ -1095,7 +1152,6 @@ __global__ void __launch_bounds__(KB_THREADS)
         s0 = e1;
       }
     }
-    const uint32_t outn = min(cnt, Lw);
     uint64_t *out = kpk + (kofs ? kofs[w] : (uint64_t)w * L);
     for (uint32_t e = tid; e < outn; e += KB_THREADS) out[e] = bpk[e];
     if (tid == 0) {
@@ -1112,12 +1168,18 @@ __global__ void __launch_bounds__(KB_THREADS)
 // (sr >= kd, else the cell would have stopped): the entries within sr are
 // about L (sr / kd)^3 for a locally uniform density; the length grows by that
 // estimate (+10 %), at least 2x and at most 4x per pass, and the next gather
-// starts at the radius expected to hold it.
+// starts at the radius expected to hold it.  With the learned proposal's
+// uncapped length kslot (by slot of work[w]; null otherwise) the growth may
+// pass 4x up to that proposal: a cell whose list the first-list cap cut
+// short (rim cells of clusters need 10^4-10^5 entries) then skips passes of
+// fourfold growth.
 __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
                             const double *__restrict__ sr, const double *__restrict__ kd,
                             float *__restrict__ Rout, uint32_t kmax,
                             unsigned int *__restrict__ cap_err,
-                            const uint32_t *__restrict__ dn) {
+                            const uint32_t *__restrict__ dn,
+                            const uint32_t *__restrict__ work,
+                            const float *__restrict__ kslot) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork || (dn && w >= *dn)) return;
   const uint32_t Lp = Lio[w];
@@ -1126,10 +1188,42 @@ __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
   const double ratio = (k > 0. && r > k && isfinite(r)) ? r / k : 1.;
   const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
   const double lo = 2. * (double)Lp, hi = 4. * (double)Lp;
-  const double Ln = fmin(fmax(est, lo), hi);
+  // learned: growth past 4x up to the model's uncapped proposal
+  const double hi2 = kslot ? fmax(hi, (double)kslot[work[w]]) : hi;
+  const double Ln = fmin(fmax(est, lo), hi2);
   const uint32_t Lnew = (uint32_t)fmin(Ln, (double)kmax);
   Lio[w] = Lnew;
   Rout[w] = (k > 0. && isfinite(k)) ? (float)(k * cbrt((double)Lnew / (double)Lp)) : 0.f;
+}
+
+// Redone main lists longer than `lmax` go to the end of the list, so that
+// the wavefront list kernels gather the others and the long-list kernel
+// only these.  flag[i] = L[i] > lmax; k_move_long sends flagged items to
+// position pos[i] of (lw, lL, lR) and the others, in order, to (sw, sL, sR).
+__global__ void k_flag_long(const uint32_t *__restrict__ L, uint32_t n, uint32_t lmax,
+                            uint32_t *__restrict__ flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = L[i] > lmax ? 1u : 0u;
+}
+__global__ void k_move_long(const uint32_t *__restrict__ w, const uint32_t *__restrict__ L,
+                            const float *__restrict__ R, const uint32_t *__restrict__ flag,
+                            const uint32_t *__restrict__ pos, uint32_t n,
+                            uint32_t *__restrict__ lw, uint32_t *__restrict__ lL,
+                            float *__restrict__ lR, uint32_t *__restrict__ sw,
+                            uint32_t *__restrict__ sL, float *__restrict__ sR) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flag[i]) {
+    const uint32_t p = pos[i];
+    lw[p] = w[i];
+    lL[p] = L[i];
+    lR[p] = R[i];
+  } else {
+    const uint32_t p = i - pos[i];
+    sw[p] = w[i];
+    sL[p] = L[i];
+    sR[p] = R[i];
+  }
 }
 
 __global__ void k_compact_redo(const uint32_t *__restrict__ in_w,

@@ -66,6 +66,9 @@ struct Arena {
 thread_local Arena *t_arena = nullptr;
 // set while a captured build (mv_graph) is recorded
 thread_local bool t_graph = false;
+// the long-list kernel's counters of the current build (items, ball
+// gathers, entries gathered), or null
+thread_local unsigned long long *t_knn_stat = nullptr;
 bool arena_active() { return t_arena != nullptr; }
 size_t arena_free() {
   return t_arena && t_arena->used < t_arena->size ? t_arena->size - t_arena->used : 0;
@@ -682,6 +685,8 @@ constexpr uint32_t WIDE_P_MAX = 1u << 20;
 constexpr uint32_t WIDE_BLOCKS = 256;
 constexpr uint32_t BIG_KNN_BLOCKS = 1024;
 constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
+// the longest list the wavefront list kernels build (2,048-entry LDS tile)
+constexpr uint32_t MAIN_LIST_MAX = 1024;
 
 // The list kernel for lists of at most L entries: the smallest LDS tile
 // (512, 1024, 2048 gathered entries) that holds twice L and at least
@@ -720,9 +725,16 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
     uint64_t *gpk = need(TT.get<uint64_t>((size_t)blocks * cap));
     double *gkey2 = need(TT.get<double>((size_t)blocks * cap));
     uint64_t *gpk2 = need(TT.get<uint64_t>((size_t)blocks * cap));
+    // lists of different lengths: longest first, claimed one at a time
+    uint32_t *perm = nullptr, *queue = nullptr;
+    if (Lper && !dn && cnt > 1) {
+      perm = need(TT.get<uint32_t>(cnt));
+      queue = need(TT.get<uint32_t>(1));
+      hipLaunchKernelGGL(k_order_by_len, dim3(1), dim3(1024), 0, s, Lper, cnt, perm, queue);
+    }
     hipLaunchKernelGGL(k_knn_big, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start, PR,
                        wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, gkey2, gpk2,
-                       cap, dn);
+                       cap, dn, t_graph ? nullptr : t_knn_stat, (const uint32_t *)perm, queue);
   }
   HIPCHK(hipGetLastError());
 }
@@ -1052,6 +1064,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : MV_K_MAX, MV_K_MAX), 2);
   const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
   float *Rfirst = nullptr;
+  // learned: each cell's proposed length before the first-list cap, by slot
+  float *kslot = nullptr;
   uint32_t lbound = k0;  // upper bound of the current main list's lengths
   double ms_model = 0;
   if (opt.mode == MV_MODE_LEARNED && nwork) {
@@ -1080,8 +1094,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     lbound = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(cap0, 16u), 1024u) & ~7u,
                                 kmax);
     const float rscale = opt.model->rscale > 0.f ? opt.model->rscale : 0.f;
+    // (a traced build plans a captured one, which grows its lists per pass)
+    if (!t_trace) kslot = need(TT.get<float>(n));
     hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
-                       std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
+                       std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0,
+                       (const uint32_t *)lists[0], kslot);
     // longest lists first (or grid order on request)
     const uint32_t NB = std::min<uint32_t>(lbound / 8 + 1, LEN_MAX_BUCKETS);
     uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
@@ -1116,8 +1133,13 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   uint32_t *link_in = need(TT.get<uint32_t>(n));
   HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
   uint64_t st_cap = std::max<uint64_t>(4096, 16ull * nwork);
-  unsigned long long *counters = need(TT.get<unsigned long long>(4));
-  // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow
+  unsigned long long *counters = need(TT.get<unsigned long long>(7));
+  // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow,
+  // [4..6] the long-list kernel's items, ball gathers and entries gathered
+  struct KnnStatScope {
+    explicit KnnStatScope(unsigned long long *p) { t_knn_stat = p; }
+    ~KnnStatScope() { t_knn_stat = nullptr; }
+  } knn_stat_scope(counters + 4);
   // kept cells: arenas sized for ~48 half-spaces and ~64 vertices per cell,
   // doubled when a pass runs out
   T->keep = (opt.flags & MV_FLAG_KEEP_CELLS) != 0;
@@ -1211,7 +1233,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     st_area = need(TT.get<double>(st_cap));
     st_cen = need(TT.get<double>(3 * st_cap));
     st_norm = need(TT.get<double>(3 * st_cap));
-    HIPCHK(hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
+    HIPCHK(hipMemsetAsync(counters, 0, 7 * sizeof(unsigned long long), s));
     HIPCHK(hipMemsetAsync(cs_cur, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long err_init = ~0ull;
     HIPCHK(hipMemcpyAsync(counters + 1, &err_init, sizeof err_init, hipMemcpyHostToDevice, s));
@@ -1225,6 +1247,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     // bounds on the list lengths of the main and big lists (all clamped to
     // kmax; a list whose lower bound is kmax cannot grow any more)
     uint32_t lb_main = std::min(lbound, kmax), lb_big = lb_main, lb_wide = lb_main;
+    uint32_t nw_tail = 0;  // long lists at the end of the main list
     const float *Rbig = nullptr, *Rwide = nullptr;
     HIPCHK(hipMemsetAsync(cap_err, 0, sizeof(unsigned int), s));
     const float *Rmain = Rfirst;
@@ -1257,21 +1280,29 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       // one list on one stream; untimed runs do not wait for their kernels
       // tier 0: the main list (wavefront kernel), 1: the large-cell kernel,
       // 2: the wide kernel; tiers 1 and 2 gather with the long-list kernel
+      // ntail: the last ntail lists of the main list are the long ones
       auto run = [&](uint32_t *wl, uint32_t *ll, uint32_t cnt, uint32_t lbnd, const float *R0,
-                     uint8_t *state, int tier, hipStream_t s, bool timed) {
+                     uint8_t *state, int tier, hipStream_t s, bool timed, uint32_t ntail = 0) {
         const bool big = tier > 0;
         double *need_r = tier == 2 ? nr_wide : big ? nr_big : nr_main;
         Timer tk(s, timed);
         uint64_t *kofs = need(TT.get<uint64_t>(cnt));
         // redone lists: the longest one picks the list kernel (the growth
         // bound is loose); read with the scan's total
-        if (passes > 1)
-          hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s, (const uint32_t *)ll, cnt,
-                             host_word() + 1);
-        const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
+        const uint32_t nhead = cnt - ntail;
         if (passes > 1) {
-          unsigned long long w2[2];
-          read_words(s, w2, 2);
+          hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s, (const uint32_t *)ll, nhead,
+                             host_word() + 1);
+          if (ntail)
+            hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s,
+                               (const uint32_t *)(ll + nhead), ntail, host_word() + 2);
+        }
+        const uint64_t total = scan_excl<uint32_t, uint64_t>(ll, kofs, cnt, s);
+        uint32_t ltail = 0;
+        if (passes > 1) {
+          unsigned long long w2[3];
+          read_words(s, w2, 3);
+          ltail = ntail ? std::min<uint32_t>(lbnd, (uint32_t)w2[2]) : 0u;
           lbnd = std::min<uint32_t>(lbnd, (uint32_t)w2[1]);
         }
         uint64_t *kpk = need(TT.get<uint64_t>(total));
@@ -1295,9 +1326,17 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
                           opt.cells_per_wave != 2 && s2 != s && cnt >= 64u * pchunks;
         std::vector<uint32_t> cbnd;
         Timer tkk(s, timed);
-        if (!pipe) {
-          launch_knn(spos, cell_start, PR, wl, cnt, lbnd, ll, R0, kofs, kpk, kcnt, kdist, big, TT,
-                     s, opt.knn_min_tile);
+        if (!pipe && ntail && nhead) {
+          // short lists on the wavefront list kernels, the long tail on the
+          // long-list kernel
+          launch_knn(spos, cell_start, PR, wl, nhead, lbnd, ll, R0, kofs, kpk, kcnt, kdist, false,
+                     TT, s, opt.knn_min_tile);
+          launch_knn(spos, cell_start, PR, wl + nhead, ntail, ltail, ll + nhead,
+                     R0 ? R0 + nhead : nullptr, kofs + nhead, kpk, kcnt + nhead, kdist + nhead,
+                     true, TT, s);
+        } else if (!pipe) {
+          launch_knn(spos, cell_start, PR, wl, cnt, std::max(lbnd, ltail), ll, R0, kofs, kpk,
+                     kcnt, kdist, big, TT, s, opt.knn_min_tile);
         } else {
           cbnd.resize(pchunks + 1);
           for (uint32_t k = 0; k <= pchunks; k++) cbnd[k] = (uint32_t)((uint64_t)cnt * k / pchunks);
@@ -1523,7 +1562,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       Timer tbig(s2, nbig + nwide > 0);
       if (nbig) run(cb, cbl, nbig, lb_big, Rbig, bstate, 1, s2, false);
       if (nwide) run(cx, cxl, nwide, lb_wide, Rwide, xstate, 2, s2, false);
-      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, 0, s, true);
+      if (nw) run(cw, cl, nw, lb_main, Rmain, wstate, 0, s, true, nw_tail);
       if (nbig + nwide) ms_clip += tbig.stop();
       HIPCHK(hipStreamSynchronize(s2));
       unsigned long long cnts[6];
@@ -1606,15 +1645,18 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       if (nw_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
                            (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr, (const uint32_t *)cw2,
+                           (const float *)kslot);
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
                            (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr, (const uint32_t *)cb2,
+                           (const float *)kslot);
       if (nx_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nx_next)), dim3(256), 0, s, nx_next, cxl2,
                            (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
+                           (const float *)nullptr);
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
@@ -1626,6 +1668,35 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       widecells += nbcap;
       redone += nw_next + nb_next + nx_next;
       if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
+      // the main lists longer than the wavefront list kernels take go to
+      // its end (gathered by the long-list kernel)
+      nw_tail = 0;
+      if (nw_next && (uint64_t)lb_main * GROW > MAIN_LIST_MAX && !exact_order) {
+        Temps T3;
+        uint32_t *fl = need(T3.get<uint32_t>(nw_next));
+        uint32_t *ps = need(T3.get<uint32_t>(nw_next));
+        hipLaunchKernelGGL(k_flag_long, dim3(nblk(nw_next)), dim3(256), 0, s,
+                           (const uint32_t *)cl2, nw_next, MAIN_LIST_MAX, fl);
+        const uint32_t nl = scan_excl<uint32_t, uint32_t>(fl, ps, nw_next, s);
+        if (nl && nl < nw_next) {
+          uint32_t *sw = need(T3.get<uint32_t>(nw_next));
+          uint32_t *sL = need(T3.get<uint32_t>(nw_next));
+          float *sR = need(T3.get<float>(nw_next));
+          const uint32_t o = nw_next - nl;
+          hipLaunchKernelGGL(k_move_long, dim3(nblk(nw_next)), dim3(256), 0, s,
+                             (const uint32_t *)cw2, (const uint32_t *)cl2, (const float *)Rm_next,
+                             (const uint32_t *)fl, (const uint32_t *)ps, nw_next, sw + o, sL + o,
+                             sR + o, sw, sL, sR);
+          HIPCHK(hipGetLastError());
+          HIPCHK(hipMemcpyAsync(cw2, sw, nw_next * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+          HIPCHK(hipMemcpyAsync(cl2, sL, nw_next * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+          HIPCHK(hipMemcpyAsync(Rm_next, sR, nw_next * sizeof(float), hipMemcpyDeviceToDevice,
+                                s));
+          // (the temporaries are freed when T3 ends: the copies finish first)
+          HIPCHK(hipStreamSynchronize(s));
+          nw_tail = nl;
+        }
+      }
       const uint32_t lb_main_old = lb_main;
       // bounds on the new lengths (k_redo_size grows them at most GROW x)
       auto grown = [&](uint32_t lb) {
@@ -1634,6 +1705,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       const uint32_t lb_big_old = lb_big;
       lb_main = grown(lb_main);
       lb_big = std::max(grown(lb_big), std::min(lb_main_old, kmax));
+      if (kslot) {
+        // learned redo lengths grow up to the proposal (the list kernels
+        // read the actual longest list)
+        lb_main = kmax;
+        lb_big = kmax;
+      }
       lb_wide = std::max(grown(lb_wide), std::min(lb_big_old, kmax));
       Rmain = Rm_next;
       Rbig = Rb_next;
@@ -1652,9 +1729,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       ping_x = 11 - ping_x;
     }
     if (!overflowed) {
-      unsigned long long cnts[4];
+      unsigned long long cnts[7];
       HIPCHK(hipMemcpy(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost));
       T->stats.exact_tests = cnts[2];
+      T->stats.knn_big_items = cnts[4];
+      T->stats.knn_big_gathers = cnts[5];
+      T->stats.knn_big_entries = cnts[6];
       if (T->keep) {
         unsigned long long used[2];
         HIPCHK(hipMemcpy(used, cs_cur, sizeof used, hipMemcpyDeviceToHost));
@@ -2573,7 +2653,7 @@ void graph_body(mv_graph &G) {
     const float rscale = G.model.rscale > 0.f ? G.model.rscale : 0.f;
     hipLaunchKernelGGL(k_proposal, dim3(nb0), dim3(256), 0, s, (const float *)Y,
                        (const float *)hl, cap0, std::min<uint32_t>(16, kmax), G.lbound0,
-                       kscale, rscale, lens[0], R0);
+                       kscale, rscale, lens[0], R0, (const uint32_t *)nullptr, (float *)nullptr);
     hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb0), dim3(256), 0, s, lens[0],
                        (const uint32_t *)cm, cap0, 0u);
     if (opt.flags & MV_FLAG_NO_LENGTH_ORDER) {
@@ -2815,7 +2895,8 @@ void graph_body(mv_graph &G) {
       if (capM2) {
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capM2)), dim3(256), 0, s, capM2, lens[im2],
                            (const double *)rr_m, (const double *)kk_m, Rm2, kmax, cap_err,
-                           (const uint32_t *)(cm + p + 1));
+                           (const uint32_t *)(cm + p + 1), (const uint32_t *)nullptr,
+                           (const float *)nullptr);
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lens[im2],
                            (const uint32_t *)(cm + p + 1), capM2, 0u);
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lists[im2],
@@ -2824,7 +2905,8 @@ void graph_body(mv_graph &G) {
       if (capB2) {
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capB2)), dim3(256), 0, s, capB2, lens[ib2],
                            (const double *)rr_b, (const double *)kk_b, Rb2, kmax, cap_err,
-                           (const uint32_t *)(cbn + p + 1));
+                           (const uint32_t *)(cbn + p + 1), (const uint32_t *)nullptr,
+                           (const float *)nullptr);
         // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_range_fill_f32, dim3(nblk(capB2)), dim3(256), 0, s, Rb2,
                            (const uint32_t *)(cbn + p + 1), (const uint32_t *)(cb + p + 1),
