@@ -244,6 +244,13 @@ typedef struct mv_stats {
                                    when the first radius held too few or too
                                    many entries) */
   uint64_t knn_big_entries;     /* entries in its last ball of each list */
+  uint64_t knn_gathers;         /* ball gathers of the other list kernels
+                                   (knn_queries when every first radius
+                                   held the list) */
+  uint64_t knn_reads;           /* seeds those gathers read (tested against
+                                   the ball) */
+  uint64_t dense_cells;         /* grid cells of more than 256 seeds split
+                                   into sub-columns for the gathers */
 } mv_stats;
 void mv_get_stats(const mv_tess *t, mv_stats *s);
 

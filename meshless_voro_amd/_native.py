@@ -95,7 +95,9 @@ class mv_stats(C.Structure):
                 ("ms_clip_wave", C.c_double), ("ms_knn_kernels", C.c_double),
                 ("ms_model", C.c_double), ("wide_cells", C.c_uint64),
                 ("pair_handovers", C.c_uint64), ("knn_big_items", C.c_uint64),
-                ("knn_big_gathers", C.c_uint64), ("knn_big_entries", C.c_uint64)]
+                ("knn_big_gathers", C.c_uint64), ("knn_big_entries", C.c_uint64),
+                ("knn_gathers", C.c_uint64), ("knn_reads", C.c_uint64),
+                ("dense_cells", C.c_uint64)]
 
 
 class mv_device_view(C.Structure):

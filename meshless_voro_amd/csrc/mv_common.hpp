@@ -118,9 +118,41 @@ struct Problems {
   const GridParams *grids;
   const uint32_t *coff;          // first grid cell of each problem
   const uint32_t *prob_of_slot;  // problem of each sorted slot (or nullptr)
+  // dense cells' sub-columns (SubCols; all null when there are none)
+  const uint32_t *sub_base, *sub_start, *sub_slot, *sub_key;
+  const double *sub_xy;
 };
 __device__ __forceinline__ uint32_t problem_of(const Problems &P, uint32_t slot) {
   return P.prob_of_slot ? P.prob_of_slot[slot] : 0u;
+}
+
+// Dense grid cells (more than SUB_MIN seeds, 3D grids) split into S x S
+// sub-columns in x and y, S = sub_side(seeds), at boundaries that are
+// quantiles of the cell's x and y: base[cell] is the first word of the
+// cell's table in `start` (~0u for the other cells): S, the first of its
+// boundaries in `xy` (S + 1 in x from -inf to +inf, then S + 1 in y), an
+// unused word, then S*S + 1 positions into `slot` and `key`: the cell's
+// slots by sub-column (x-major), each sub-column in the cell's own (zkey)
+// order, and their zkeys.  A gather reads only the sub-columns that reach
+// its ball, each cut to the ball's chord in z.
+constexpr uint32_t SUB_MIN = 256;
+constexpr uint32_t SUB_SMAX = 63;   // S + 1 boundaries fit one wavefront
+constexpr uint32_t SUB_BINS = 4096; // histogram bins of the quantiles
+constexpr uint32_t SUB_HEAD = 3;    // words before the positions
+struct SubCols {
+  const uint32_t *base, *start, *slot, *key;
+  const double *xy;
+};
+__device__ __forceinline__ SubCols sub_cols(const Problems &P, uint32_t pr) {
+  return SubCols{P.sub_base ? P.sub_base + P.coff[pr] : nullptr, P.sub_start, P.sub_slot,
+                 P.sub_key, P.sub_xy};
+}
+// S for a cell of m seeds: the smallest S >= 2 with 16 S^2 >= m (about 16
+// seeds per sub-column), at most SUB_SMAX
+__device__ __forceinline__ uint32_t sub_side(uint32_t m) {
+  uint32_t S = 2;
+  while (S < SUB_SMAX && 16u * S * S < m) S++;
+  return S;
 }
 
 // Kept cells (VoronoiIntegrator): every completed cell's half-spaces and

@@ -235,6 +235,170 @@ __global__ void __launch_bounds__(GS_THREADS)
   }
 }
 
+// The sub-columns of the dense cells of 3D grids (SubCols), one wavefront
+// per cell of the large-cell list.  The S - 1 inner boundaries in x are
+// quantiles of the cell's x (k m / S seeds below the k-th, from a histogram
+// of SUB_BINS bins over the cell), likewise in y; the outer ones are -inf and
+// +inf.  A seed's sub-column is (slab of x, slab of y), the slab of v being
+// the number of inner boundaries <= v, so every seed lies in its
+// sub-column's box exactly.  For a seed cluster whose density is a product
+// of its x and y marginals (a Gaussian) the sub-columns hold m / S^2 seeds
+// each wherever the cluster is.  The slots are then placed 64 at a time in
+// the cell's order (a lane's place is its sub-column's cursor plus the
+// lanes before it with the same sub-column), so that every sub-column keeps
+// the cell's zkey order.  Tables come from two pools; a cell whose table
+// does not fit keeps base = ~0u (its gathers read the whole cell).
+__global__ void __launch_bounds__(64)
+    k_grid_subcols(const uint32_t *__restrict__ cell_start, const uint32_t *__restrict__ big_list,
+                   const unsigned int *__restrict__ big_cnt, const double4 *__restrict__ spos,
+                   const GridParams *__restrict__ grids, const uint32_t *__restrict__ coff,
+                   const uint32_t *__restrict__ prob_of_slot, uint32_t *__restrict__ sub_base,
+                   uint32_t *__restrict__ pool, uint32_t pool_cap,
+                   unsigned int *__restrict__ pool_cnt, double *__restrict__ xy,
+                   uint32_t xy_cap, unsigned int *__restrict__ xy_cnt,
+                   uint32_t *__restrict__ sub_slot, uint32_t *__restrict__ sub_key) {
+  __shared__ uint32_t hist[SUB_BINS];
+  __shared__ uint32_t cur[SUB_SMAX * SUB_SMAX];
+  __shared__ double bd[2][SUB_SMAX + 1];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nbig = *big_cnt;
+  for (uint32_t k = blockIdx.x; k < nbig; k += gridDim.x) {
+    const uint32_t c = big_list[k];
+    const uint32_t b = cell_start[c], m = cell_start[c + 1] - b;
+    if (m <= SUB_MIN) continue;
+    const uint32_t pr = prob_of_slot ? prob_of_slot[b] : 0u;
+    const GridParams &g = grids[pr];
+    if (g.inv_h[0] == 0. || g.inv_h[1] == 0. || g.inv_h[2] == 0.) continue;
+    const uint32_t local = c - coff[pr];
+    const uint32_t ca = local / ((uint32_t)g.G[1] * (uint32_t)g.G[2]);
+    const uint32_t cb = (local / (uint32_t)g.G[2]) % (uint32_t)g.G[1];
+    const uint32_t S = sub_side(m), ns = S * S;
+    uint32_t base = 0, xo = 0;
+    if (lane == 0) {
+      base = atomicAdd(pool_cnt, ns + SUB_HEAD + 1u);
+      xo = atomicAdd(xy_cnt, 2u * (S + 1u));
+    }
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    xo = (uint32_t)__shfl((int)xo, 0, 64);
+    if ((uint64_t)base + ns + SUB_HEAD + 1u > pool_cap || (uint64_t)xo + 2u * (S + 1u) > xy_cap)
+      continue;
+    // the quantile boundaries in x (d = 0) and y (d = 1)
+    for (int d = 0; d < 2; d++) {
+      const double c0 = g.lo[d] + (double)(d == 0 ? ca : cb) * g.h[d];
+      const double fb = (double)SUB_BINS * g.inv_h[d];
+      for (uint32_t i = lane; i < SUB_BINS; i += 64) hist[i] = 0u;
+      __syncthreads();
+      for (uint32_t i = lane; i < m; i += 64) {
+        const double4 p = spos[b + i];
+        const double u = floor(((d == 0 ? p.x : p.y) - c0) * fb);
+        const uint32_t e = u < 0. ? 0u : (uint32_t)fmin(u, (double)(SUB_BINS - 1));
+        atomicAdd(&hist[e], 1u);
+      }
+      __syncthreads();
+      // inclusive scan, 64 bins at a time
+      uint32_t carry = 0;
+      for (uint32_t s0 = 0; s0 < SUB_BINS; s0 += 64) {
+        const uint32_t v = hist[s0 + lane];
+        uint32_t tot;
+        const uint32_t ex = wave_excl_scan(v, &tot);
+        hist[s0 + lane] = carry + ex + v;
+        carry += tot;
+      }
+      __syncthreads();
+      if (lane >= 1 && lane < S) {
+        // the first bin whose cumulative count reaches ceil(lane m / S);
+        // the boundary is its upper edge
+        const uint32_t target = (uint32_t)(((uint64_t)lane * m + S - 1u) / S);
+        uint32_t lo = 0, hi = SUB_BINS - 1u;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (hist[mid] >= target)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        bd[d][lane] = c0 + (double)(lo + 1u) / fb;
+      }
+      if (lane == 0) {
+        bd[d][0] = -INFINITY;
+        bd[d][S] = INFINITY;
+      }
+      __syncthreads();
+    }
+    // slab of v: inner boundaries <= v
+    auto slab = [&](double v, const double *bv) -> uint32_t {
+      uint32_t lo = 1, hi = S;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bv[mid] <= v)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      return lo - 1u;
+    };
+    auto sub_of = [&](const double4 &p) -> uint32_t {
+      return slab(p.x, bd[0]) * S + slab(p.y, bd[1]);
+    };
+    for (uint32_t i = lane; i < ns; i += 64) cur[i] = 0u;
+    __syncthreads();
+    for (uint32_t i = lane; i < m; i += 64) atomicAdd(&cur[sub_of(spos[b + i])], 1u);
+    __syncthreads();
+    // exclusive scan of the sub-column counts: their first positions
+    uint32_t carry = 0;
+    for (uint32_t s0 = 0; s0 < ns; s0 += 64) {
+      const uint32_t i = s0 + lane;
+      const uint32_t v = i < ns ? cur[i] : 0u;
+      uint32_t tot;
+      const uint32_t ex = wave_excl_scan(v, &tot);
+      if (i < ns) {
+        cur[i] = b + carry + ex;
+        pool[base + SUB_HEAD + i] = b + carry + ex;
+      }
+      carry += tot;
+    }
+    if (lane == 0) {
+      pool[base] = S;
+      pool[base + 1] = xo;
+      pool[base + SUB_HEAD + ns] = b + m;
+    }
+    if (lane <= S) {
+      xy[xo + lane] = bd[0][lane];
+      xy[xo + S + 1u + lane] = bd[1][lane];
+    }
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < m; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      const bool valid = i < m;
+      double4 p = make_double4(0., 0., 0., 0.);
+      if (valid) p = spos[b + i];
+      const uint32_t sid = valid ? sub_of(p) : ~0u;
+      uint32_t rank = 0;
+      bool last = true;
+      for (int j = 0; j < 64; j++) {
+        const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)sid, j);
+        if (sj == sid) {
+          if (j < (int)lane) rank++;
+          if (j > (int)lane) last = false;
+        }
+      }
+      if (valid) {
+        const uint32_t pos = cur[sid] + rank;
+        sub_slot[pos] = b + i;
+        sub_key[pos] = zkey(p.z);
+      }
+      __syncthreads();
+      if (valid && last) cur[sid] += rank + 1u;
+      __syncthreads();
+    }
+    if (lane == 0) {
+      sub_base[c] = base;
+      atomicAdd(xy_cnt + 1, 1u);  // cells with sub-columns
+    }
+    __syncthreads();
+  }
+}
+
 // sorted positions; w carries the generator index bits
 __global__ void k_grid_gather(const double4 *__restrict__ gpos,
                               const uint32_t *__restrict__ order, size_t n,
@@ -307,6 +471,12 @@ template <typename TIn, typename TOut>
 __global__ void k_scan_total(const TIn *__restrict__ last_in, const TOut *__restrict__ last_out,
                              unsigned long long *__restrict__ total) {
   *total = (unsigned long long)(*last_out + (TOut)*last_in);
+}
+
+// two device words, written where the host reads them
+__global__ void k_word_u32(const uint32_t *__restrict__ v, unsigned long long *__restrict__ out) {
+  out[0] = v[0];
+  out[1] = v[1];
 }
 
 // largest of n values (one block), written where the host reads it
@@ -394,12 +564,14 @@ __device__ __forceinline__ double entry_dist(const double4 *__restrict__ spos,
 // BLK: the wavefronts of a block share the rows (wavefront wid of nw) and
 // reserve their output slots through the LDS counter *blk_cnt (zeroed by
 // the caller), which ends as the count; the return value is then unused.
-template <bool BLK = false>
+// SUB: the grid has dense cells with sub-columns (sc.base set); without
+// them the gather is compiled without that path.
+template <bool BLK = false, bool SUB = false>
 __device__ __forceinline__ uint32_t knn_gather(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
-    const GridParams &g, const double *width, const double q[3], double R,
+    const SubCols &sc, const GridParams &g, const double *width, const double q[3], double R,
     double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt = nullptr, int wid = 0,
-    int nw = 1) {
+    int nw = 1, uint32_t *reads = nullptr) {
   const uint32_t lane = __lane_id();
   const double R2 = R * R;
   const double rr = R + g.margin;
@@ -421,6 +593,78 @@ __device__ __forceinline__ uint32_t knn_gather(
     hi = b > (double)(g.G[c] - 1) ? g.G[c] - 1 : (int)b;
     return true;
   };
+  // [t0, t0 + len) cut to the seeds whose zkey lies in [klo, khi] (the
+  // range is in zkey order; keys[t] is the key of position t, or null for
+  // the key of spos[t])
+  auto zcut = [&](uint32_t &t0, uint32_t &len, uint32_t klo, uint32_t khi,
+                  const uint32_t *keys) {
+    auto key = [&](uint32_t t) { return keys ? keys[t] : zkey(spos[t].z); };
+    uint32_t lo = t0, hi = t0 + len;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (key(mid) < klo)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    uint32_t lo2 = lo, hi2 = t0 + len;
+    while (lo2 < hi2) {
+      const uint32_t mid = (lo2 + hi2) >> 1;
+      if (key(mid) <= khi)
+        lo2 = mid + 1;
+      else
+        hi2 = mid;
+    }
+    t0 = lo;
+    len = lo2 - lo;
+  };
+  // the candidates of the lanes' ranges (positions t0 + j, or ind[t0 + j]
+  // with an index), numbered consecutively and read 64 at a time
+  auto read = [&](uint32_t t0, uint32_t len, const uint32_t *ind, const double *qs,
+                  uint32_t img) {
+    uint32_t total;
+    const uint32_t off = wave_excl_scan(len, &total);
+    if (reads) *reads += total;
+    for (uint32_t cb = 0; cb < total; cb += 64) {
+      const uint32_t gi = cb + lane;
+      // largest row r with off[r] <= gi (rows past the end hold
+      // off = total, empty rows share the next row's offset)
+      uint32_t rsel = 0;
+#pragma unroll
+      for (uint32_t st = 32; st > 0; st >>= 1) {
+        const uint32_t o = (uint32_t)__shfl((int)off, (int)(rsel + st), 64);
+        if (rsel + st < 64 && o <= gi) rsel += st;
+      }
+      const uint32_t orow = (uint32_t)__shfl((int)off, (int)rsel, 64);
+      const uint32_t trow = (uint32_t)__shfl((int)t0, (int)rsel, 64);
+      bool keep = false;
+      double key = 0.;
+      uint64_t pk = 0;
+      if (gi < total) {
+        const uint32_t tp = trow + (gi - orow);
+        const uint32_t t = ind ? ind[tp] : tp;
+        const double4 gp = spos[t];
+        const double d0 = qs[0] - gp.x, d1 = qs[1] - gp.y, dz = qs[2] - gp.z;
+        key = (d0 * d0 + d1 * d1) + dz * dz;
+        keep = key <= R2;
+        const uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
+        pk = (gidx << 32) | ((uint64_t)t << 5) | img;
+      }
+      const uint64_t m = wave_ballot(keep);
+      uint32_t base = cnt;
+      if (BLK && m) {
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(blk_cnt, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)b0, 0, 64);
+      }
+      const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (keep && pos < cap) {
+        bkey[pos] = key;
+        bpk[pos] = pk;
+      }
+      cnt += (uint32_t)__popcll(m);
+    }
+  };
   for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++) {
     double qs[3];
     int lo[3], hi[3];
@@ -435,6 +679,9 @@ __device__ __forceinline__ uint32_t knn_gather(
         for (int r0 = BLK ? 64 * wid : 0; r0 < nrows; r0 += BLK ? 64 * nw : 64) {
           const int r = r0 + (int)lane;
           uint32_t t0 = 0, len = 0;
+          // dense cells with sub-columns at the row's ends: read after the
+          // rows, sub-column by sub-column
+          uint32_t dfirst = ~0u, dlast = ~0u;
           if (r < nrows) {
             const int a = lo[0] + r / nb_, b = lo[1] + r % nb_;
             // squared distance from qs to the column of cells (a, b)
@@ -462,72 +709,81 @@ __device__ __forceinline__ uint32_t knn_gather(
                 const size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
                 t0 = cell_start[row + c0];
                 len = cell_start[row + c1 + 1] - t0;
+                if (SUB && len > SUB_MIN) {
+                  // the row's first and last cells, when dense, leave the
+                  // range (the rest stays in zkey order)
+                  const uint32_t te = t0 + len;
+                  uint32_t s1 = t0, s2 = te;
+                  const uint32_t e0 = c0 == c1 ? te : cell_start[row + c0 + 1];
+                  if (e0 - t0 > SUB_MIN) {
+                    dfirst = sc.base[row + c0];
+                    if (dfirst != ~0u) s1 = e0;
+                  }
+                  if (c1 > c0) {
+                    const uint32_t b1 = cell_start[row + c1];
+                    if (te - b1 > SUB_MIN) {
+                      dlast = sc.base[row + c1];
+                      if (dlast != ~0u) s2 = b1;
+                    }
+                  }
+                  t0 = s1;
+                  len = s2 > s1 ? s2 - s1 : 0u;
+                }
                 if (len > 32u && g.inv_h[2] != 0.) {
                   // a long row (dense cells): its seeds are in zkey order;
                   // keep those whose z can lie in the chord
                   const double hc = sqrt(rr2 - d2);
-                  const uint32_t klo = zkey(qs[2] - hc), khi = zkey(qs[2] + hc);
-                  uint32_t lo = t0, hi = t0 + len;
-                  while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (zkey(spos[mid].z) < klo)
-                      lo = mid + 1;
-                    else
-                      hi = mid;
-                  }
-                  uint32_t lo2 = lo, hi2 = t0 + len;
-                  while (lo2 < hi2) {
-                    const uint32_t mid = (lo2 + hi2) >> 1;
-                    if (zkey(spos[mid].z) <= khi)
-                      lo2 = mid + 1;
-                    else
-                      hi2 = mid;
-                  }
-                  t0 = lo;
-                  len = lo2 - lo;
+                  zcut(t0, len, zkey(qs[2] - hc), zkey(qs[2] + hc), nullptr);
                 }
               }
             }
           }
-          uint32_t total;
-          const uint32_t off = wave_excl_scan(len, &total);
-          for (uint32_t cb = 0; cb < total; cb += 64) {
-            const uint32_t gi = cb + lane;
-            // largest row r with off[r] <= gi (rows past the end hold
-            // off = total, empty rows share the next row's offset)
-            uint32_t rsel = 0;
-#pragma unroll
-            for (uint32_t st = 32; st > 0; st >>= 1) {
-              const uint32_t o = (uint32_t)__shfl((int)off, (int)(rsel + st), 64);
-              if (rsel + st < 64 && o <= gi) rsel += st;
+          read(t0, len, nullptr, qs, img);
+          // the dense cells: the sub-columns that reach the ball, each cut
+          // to its chord, the lanes over the sub-columns
+          for (int side = 0; side < (SUB ? 2 : 0); side++) {
+            uint64_t dmask = wave_ballot((side ? dlast : dfirst) != ~0u);
+            while (dmask) {
+              const int l = (int)__builtin_ctzll(dmask);
+              dmask &= dmask - 1ull;
+              const uint32_t sb =
+                  (uint32_t)__builtin_amdgcn_readlane((int)(side ? dlast : dfirst), l);
+              const uint32_t S = sc.start[sb], xo = sc.start[sb + 1];
+              const uint32_t *st = sc.start + sb + SUB_HEAD;
+              // boundary `lane` of x and of y
+              const double Xl = lane <= S ? sc.xy[xo + lane] : INFINITY;
+              const double Yl = lane <= S ? sc.xy[xo + S + 1u + lane] : INFINITY;
+              const bool inner = lane >= 1u && lane < S;
+              // the slabs that meet [qs - rr, qs + rr] in x and in y
+              const uint32_t i0 = (uint32_t)__popcll(wave_ballot(inner && Xl <= qs[0] - rr));
+              const uint32_t i1 = (uint32_t)__popcll(wave_ballot(inner && Xl <= qs[0] + rr));
+              const uint32_t j0 = (uint32_t)__popcll(wave_ballot(inner && Yl <= qs[1] - rr));
+              const uint32_t j1 = (uint32_t)__popcll(wave_ballot(inner && Yl <= qs[1] + rr));
+              const uint32_t ny = j1 - j0 + 1u, nxy = (i1 - i0 + 1u) * ny;
+              for (uint32_t s0 = 0; s0 < nxy; s0 += 64) {
+                const uint32_t e = min(s0 + lane, nxy - 1u);
+                const uint32_t ex = e / ny;
+                const uint32_t si = i0 + ex, sj = j0 + (e - ex * ny);
+                const double xa = __shfl(Xl, (int)si, 64), xb = __shfl(Xl, (int)si + 1, 64);
+                const double ya = __shfl(Yl, (int)sj, 64), yb = __shfl(Yl, (int)sj + 1, 64);
+                uint32_t u0 = 0, ulen = 0;
+                if (s0 + lane < nxy) {
+                  const double dx = fmax(0., fmax(xa - qs[0], qs[0] - xb));
+                  const double dy = fmax(0., fmax(ya - qs[1], qs[1] - yb));
+                  const double d2 = dx * dx + dy * dy;
+                  if (d2 <= rr2) {
+                    const uint32_t sidx = si * S + sj;
+                    u0 = st[sidx];
+                    ulen = st[sidx + 1] - u0;
+                    if (ulen > 16u) {
+                      const double hc = sqrt(rr2 - d2);
+                      zcut(u0, ulen, zkey(qs[2] - hc), zkey(qs[2] + hc), sc.key);
+                    }
+                  }
+                }
+                read(u0, ulen, sc.slot, qs, img);
+              }
             }
-            const uint32_t orow = (uint32_t)__shfl((int)off, (int)rsel, 64);
-            const uint32_t trow = (uint32_t)__shfl((int)t0, (int)rsel, 64);
-            bool keep = false;
-            double key = 0.;
-            uint64_t pk = 0;
-            if (gi < total) {
-              const uint32_t t = trow + (gi - orow);
-              const double4 gp = spos[t];
-              const double d0 = qs[0] - gp.x, d1 = qs[1] - gp.y, dz = qs[2] - gp.z;
-              key = (d0 * d0 + d1 * d1) + dz * dz;
-              keep = key <= R2;
-              const uint64_t gidx = (uint64_t)__double_as_longlong(gp.w);
-              pk = (gidx << 32) | ((uint64_t)t << 5) | img;
-            }
-            const uint64_t m = wave_ballot(keep);
-            uint32_t base = cnt;
-            if (BLK && m) {
-              uint32_t b0 = 0;
-              if (lane == 0) b0 = atomicAdd(blk_cnt, (uint32_t)__popcll(m));
-              base = (uint32_t)__shfl((int)b0, 0, 64);
-            }
-            const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (keep && pos < cap) {
-              bkey[pos] = key;
-              bpk[pos] = pk;
-            }
-            cnt += (uint32_t)__popcll(m);
           }
         }
       }
@@ -540,14 +796,17 @@ __device__ __forceinline__ uint32_t knn_gather(
 // stream) and at most `cap`.  Growth/shrink steps from the density, then
 // bisection between a too-small and a too-large radius.  Returns the count
 // (> cap when no radius fits).
+template <bool SUB>
 __device__ __forceinline__ uint32_t knn_fill(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
-    const GridParams &g, const double *width, const double q[3], double &R,
-    uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap) {
+    const SubCols &sc, const GridParams &g, const double *width, const double q[3], double &R,
+    uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *iters, uint32_t *reads) {
   double r_lo = 0., r_hi = 0.;
   uint32_t cnt = 0;
   for (int it = 0; it < 96; it++) {
-    cnt = knn_gather(spos, cell_start, g, width, q, R, bkey, bpk, cap);
+    *iters = it + 1;
+    cnt = knn_gather<false, SUB>(spos, cell_start, sc, g, width, q, R, bkey, bpk, cap, nullptr, 0,
+                                 1, reads);
     bool all = (uint64_t)cnt >= g.total_stream;
     if (cnt <= cap && (cnt >= Lw || all)) return cnt;
     if (cnt > cap) {
@@ -575,9 +834,10 @@ __device__ __forceinline__ uint32_t knn_fill(
 // knn_fill with every wavefront of a KB_THREADS block gathering (all threads
 // call it; the count comes back through the LDS word *blk_cnt).
 // *iters: the gathers made.
+template <bool SUB>
 __device__ __forceinline__ uint32_t knn_fill_blk(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
-    const GridParams &g, const double *width, const double q[3], double &R,
+    const SubCols &sc, const GridParams &g, const double *width, const double q[3], double &R,
     uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt, int wid,
     int nw, uint32_t *iters) {
   double r_lo = 0., r_hi = 0.;
@@ -586,7 +846,7 @@ __device__ __forceinline__ uint32_t knn_fill_blk(
     *iters = it + 1;
     if (threadIdx.x == 0) *blk_cnt = 0u;
     __syncthreads();
-    knn_gather<true>(spos, cell_start, g, width, q, R, bkey, bpk, cap, blk_cnt, wid, nw);
+    knn_gather<true, SUB>(spos, cell_start, sc, g, width, q, R, bkey, bpk, cap, blk_cnt, wid, nw);
     __syncthreads();
     cnt = *blk_cnt;
     __syncthreads();
@@ -623,6 +883,62 @@ __device__ __forceinline__ double ball_certificate(const double q[3], double R,
   const double M = fmax(fabs(q[0]), fmax(fabs(q[1]), fabs(q[2]))) +
                    fmax(width[0], fmax(width[1], width[2])) + R;
   return R - 1e-13 * M;
+}
+
+// Start radius of a list of Lw entries for a seed q of a dense cell with
+// sub-columns: the density around q from the z-span of the (up to) 17
+// seeds of q's sub-column nearest to q in z order, over the sub-column's
+// box; 0 when q's cell has no sub-columns, q's sub-column is unbounded (an
+// outer one) or holds fewer than 8 seeds.  Inside a seed cluster it is far
+// closer to the radius that holds Lw seeds than one from the density of a
+// block of grid cells.
+__device__ __forceinline__ double dense_start_radius(
+    const double4 *__restrict__ spos, const SubCols &sc, const GridParams &g,
+    const double q[3], uint32_t Lw) {
+  if (!sc.base || g.inv_h[0] == 0. || g.inv_h[1] == 0. || g.inv_h[2] == 0.) return 0.;
+  int C[3];
+  for (int c = 0; c < 3; c++) {
+    const double f = floor((q[c] - g.lo[c]) * g.inv_h[c]);
+    C[c] = (int)fmax(0., fmin(f, (double)(g.G[c] - 1)));
+  }
+  const uint32_t sb = sc.base[((size_t)C[0] * g.G[1] + C[1]) * g.G[2] + C[2]];
+  if (sb == ~0u) return 0.;
+  const uint32_t S = sc.start[sb], xo = sc.start[sb + 1];
+  auto slab = [&](double v, const double *bv) -> uint32_t {
+    uint32_t lo = 1, hi = S;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (bv[mid] <= v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo - 1u;
+  };
+  const double *X = sc.xy + xo, *Y = sc.xy + xo + S + 1u;
+  const uint32_t i = slab(q[0], X), j = slab(q[1], Y);
+  const double area = (X[i + 1] - X[i]) * (Y[j + 1] - Y[j]);
+  if (!(area > 0.) || !isfinite(area)) return 0.;
+  const uint32_t *st = sc.start + sb + SUB_HEAD;
+  const uint32_t u0 = st[i * S + j], u1 = st[i * S + j + 1];
+  if (u1 - u0 < 8u) return 0.;
+  const uint32_t kz = zkey(q[2]);
+  uint32_t lo = u0, hi = u1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sc.key[mid] < kz)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  const uint32_t K = min(8u, (u1 - u0) / 2u);
+  uint32_t a = lo >= u0 + K ? lo - K : u0;
+  const uint32_t b = min(u1 - 1u, a + 2u * K);
+  a = b >= u0 + 2u * K ? b - 2u * K : u0;
+  const double dz = spos[sc.slot[b]].z - spos[sc.slot[a]].z;
+  if (!(dz > 0.)) return 0.;
+  const double rho = (double)(b - a) / (area * dz);
+  return cbrt((1.3 * (double)Lw + 8.) / rho / 4.18879020478639);
 }
 
 __device__ __forceinline__ double knn_start_radius(
@@ -789,7 +1105,9 @@ __device__ __forceinline__ uint64_t packed_entry(const double *key, uint32_t s) 
   return ((uint64_t)__double_as_longlong(key[s]) & ~0x7Full) | (uint64_t)s;
 }
 
-template <int CAP, int WPB>
+// SUB: the grid has dense cells with sub-columns (a separate instantiation:
+// that path's registers would cost the other one a wavefront per SIMD)
+template <int CAP, int WPB, bool SUB>
 __global__ void __launch_bounds__(256)
     k_knn(const double4 *__restrict__ spos,
           const uint32_t *__restrict__ cell_start_all, Problems PR,
@@ -797,7 +1115,8 @@ __global__ void __launch_bounds__(256)
           uint32_t L, const uint32_t *__restrict__ Lper,
           const float *__restrict__ Rper, const uint64_t *__restrict__ kofs,
           uint64_t *__restrict__ kpk, uint32_t *__restrict__ kcnt,
-          double *__restrict__ kdist, const uint32_t *__restrict__ dn) {
+          double *__restrict__ kdist, const uint32_t *__restrict__ dn,
+          unsigned long long *__restrict__ kstat, uint32_t proposal) {
   __shared__ KnnLds<CAP> lds[WPB];
   // dn: the item count on the device (nwork bounds it)
   if (dn) nwork = min(*dn, nwork);
@@ -819,14 +1138,29 @@ __global__ void __launch_bounds__(256)
     const double4 qq = spos[slot];
     const double q[3] = {qq.x, qq.y, qq.z};
     const uint32_t Lw = Lper ? Lper[w] : L;
-    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w]
-                                       : knn_start_radius(cell_start, g, q, Lw);
+    const SubCols sc = sub_cols(PR, pr);
+    // radii of redone lists come from the previous ball; first radii
+    // (proposed, or from the density of a block of grid cells) are bounded
+    // by the density inside dense cells
+    const bool given = Rper && Rper[w] > 0.f;
+    double R = given ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
+    if (SUB && (!given || proposal)) {
+      const double Rd = dense_start_radius(spos, sc, g, q, Lw);
+      if (Rd > 0.) R = given ? fmin(R, 1.5 * Rd) : Rd;
+    }
     // a ball with three quarters of the requested entries is kept (its list
     // ends at the ball, certified by its radius) rather than gathered again:
     // a cell that needs more is redone with a longer list anyway
     const uint32_t Lacc = max(1u, (Lw * 3u) / 4u);
-    uint32_t cnt = knn_fill(spos, cell_start, g, box.width, q, R, Lacc, buf.key,
-                            buf.pk, CAP);
+    uint32_t iters = 0, reads = 0;
+    uint32_t cnt = knn_fill<SUB>(spos, cell_start, sc, g, box.width, q, R, Lacc, buf.key,
+                                 buf.pk, CAP, &iters, &reads);
+    // ball gathers and candidates read, on 64 counters each (by block) the
+    // host adds up
+    if (kstat && lane == 0) {
+      atomicAdd(kstat + (blockIdx.x & 63u), (unsigned long long)iters);
+      atomicAdd(kstat + 64 + (blockIdx.x & 63u), (unsigned long long)reads);
+    }
     __builtin_amdgcn_wave_barrier();
     if (cnt > (uint32_t)CAP) {
       if (lane == 0) kcnt[w] = 0xFFFFFFFFu;  // does not fit this variant
@@ -997,6 +1331,7 @@ __global__ void __launch_bounds__(1024)
     perm[atomicAdd(&cnt[len_class(Lper[i])], 1u)] = i;
 }
 
+template <bool SUB>
 __global__ void __launch_bounds__(KB_THREADS)
     k_knn_big(const double4 *__restrict__ spos,
               const uint32_t *__restrict__ cell_start_all, Problems PR,
@@ -1047,8 +1382,9 @@ __global__ void __launch_bounds__(KB_THREADS)
     const uint32_t *cell_start = cell_start_all + PR.coff[pru];
     double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
     uint32_t iters = 0;
-    const uint32_t cnt = knn_fill_blk(spos, cell_start, g, width, q, R, Lw, bkey, bpk, cap,
-                                      &s_gather, (int)wid, KB_THREADS / 64, &iters);
+    const SubCols sc = sub_cols(PR, pru);
+    const uint32_t cnt = knn_fill_blk<SUB>(spos, cell_start, sc, g, width, q, R, Lw, bkey, bpk,
+                                           cap, &s_gather, (int)wid, KB_THREADS / 64, &iters);
     if (gstat && tid == 0) {
       // items, ball gathers, entries gathered by the last one
       atomicAdd(gstat, 1ull);
@@ -1167,19 +1503,14 @@ __global__ void __launch_bounds__(KB_THREADS)
 // when its list ran out and the distance kd of the first entry left out
 // (sr >= kd, else the cell would have stopped): the entries within sr are
 // about L (sr / kd)^3 for a locally uniform density; the length grows by that
-// estimate (+10 %), at least 2x and at most 4x per pass, and the next gather
-// starts at the radius expected to hold it.  With the learned proposal's
-// uncapped length kslot (by slot of work[w]; null otherwise) the growth may
-// pass 4x up to that proposal: a cell whose list the first-list cap cut
-// short (rim cells of clusters need 10^4-10^5 entries) then skips passes of
-// fourfold growth.
+// estimate (+10 %), at least 2x and at most 4x per pass (16x from 4,096
+// entries on: the few cells with such lists hold up each pass), and the next gather
+// starts at the radius expected to hold it.
 __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
                             const double *__restrict__ sr, const double *__restrict__ kd,
                             float *__restrict__ Rout, uint32_t kmax,
                             unsigned int *__restrict__ cap_err,
-                            const uint32_t *__restrict__ dn,
-                            const uint32_t *__restrict__ work,
-                            const float *__restrict__ kslot) {
+                            const uint32_t *__restrict__ dn) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork || (dn && w >= *dn)) return;
   const uint32_t Lp = Lio[w];
@@ -1187,10 +1518,9 @@ __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
   const double k = kd[w], r = sr[w];
   const double ratio = (k > 0. && r > k && isfinite(r)) ? r / k : 1.;
   const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
-  const double lo = 2. * (double)Lp, hi = 4. * (double)Lp;
-  // learned: growth past 4x up to the model's uncapped proposal
-  const double hi2 = kslot ? fmax(hi, (double)kslot[work[w]]) : hi;
-  const double Ln = fmin(fmax(est, lo), hi2);
+  // long lists (few cells, each pass of them mostly latency) grow up to 16x
+  const double lo = 2. * (double)Lp, hi = (Lp >= 4096u ? 16. : 4.) * (double)Lp;
+  const double Ln = fmin(fmax(est, lo), hi);
   const uint32_t Lnew = (uint32_t)fmin(Ln, (double)kmax);
   Lio[w] = Lnew;
   Rout[w] = (k > 0. && isfinite(k)) ? (float)(k * cbrt((double)Lnew / (double)Lp)) : 0.f;

@@ -623,19 +623,14 @@ __global__ void k_gather_rows(const float *__restrict__ F,
 // ---------------------------------------------------------------------
 // proposal -> per-cell list length and start radius
 // ---------------------------------------------------------------------
-// kslot (may be null): the proposed length before the first-list cap, by
-// slot (work[i]): a cell the first list did not certify is redone with a list
-// sized from it.
 __global__ void k_proposal(const float *__restrict__ Y,
                            const float *__restrict__ hloc, uint32_t n,
                            uint32_t lmin, uint32_t lmax, float kscale,
                            float rscale, uint32_t *__restrict__ Lper,
-                           float *__restrict__ Rper, const uint32_t *__restrict__ work,
-                           float *__restrict__ kslot) {
+                           float *__restrict__ Rper) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float k = expf(fminf(Y[(size_t)i * NO], 12.f)) * kscale + 2.f;
-  if (kslot) kslot[work[i]] = k;
   uint32_t L = (uint32_t)ceilf(k);
   L = (L + 7u) & ~7u;  // 8-wide buckets
   L = L < lmin ? lmin : (L > lmax ? lmax : L);

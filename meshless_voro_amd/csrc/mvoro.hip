@@ -66,9 +66,11 @@ struct Arena {
 thread_local Arena *t_arena = nullptr;
 // set while a captured build (mv_graph) is recorded
 thread_local bool t_graph = false;
-// the long-list kernel's counters of the current build (items, ball
-// gathers, entries gathered), or null
+// the list kernels' counters of the current build, or null: the long-list
+// kernel's items, ball gathers and entries gathered, then 64 counters of
+// the wavefront list kernels' ball gathers and 64 of their candidates read
 thread_local unsigned long long *t_knn_stat = nullptr;
+unsigned long long *knn_stripes() { return t_knn_stat && !t_graph ? t_knn_stat + 3 : nullptr; }
 bool arena_active() { return t_arena != nullptr; }
 size_t arena_free() {
   return t_arena && t_arena->used < t_arena->size ? t_arena->size - t_arena->used : 0;
@@ -678,6 +680,8 @@ GridParams make_grid(const double bb[6], size_t n, const double width[3],
 // power of two, at least 65,536); at most 8 GiB of such buffers are in
 // flight
 constexpr uint32_t BIG_KNN_CAP = 65536;
+// words of a build's pass counters
+constexpr int NCOUNTERS = 7 + 128;
 // wide kernel: starting capacities per cell (planes, vertices), their
 // limit (plane indices have 21 bits) and the blocks of one launch
 constexpr uint32_t WIDE_P0 = 1024, WIDE_V0 = 4096;
@@ -696,24 +700,37 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
                 const uint32_t *Lper, const float *Rper, const uint64_t *kofs,
                 uint64_t *kpk, uint32_t *kcnt, double *kdist, bool big, Temps &TT,
                 hipStream_t s, uint32_t min_tile = 0, const uint32_t *dn = nullptr,
-                uint32_t big_blocks = 0, uint32_t max_blocks_knn = 0) {
+                uint32_t big_blocks = 0, uint32_t max_blocks_knn = 0, bool proposal = false) {
+  // proposal: Rper are first radii (proposed), not radii of redone lists
   // max_blocks_knn: a smaller grid (the kernels loop over their items)
   auto grid = [&](uint32_t b) {
     return dim3(max_blocks_knn ? std::min(b, max_blocks_knn) : b);
   };
   if (cnt == 0) return;
   if (!big && min_tile > 512 && L <= 512) L = min_tile >= 2048 ? 1024u : 512u;
+  // the instantiation with the dense-cell path only when the grid has some
+  const bool sub = PR.sub_base != nullptr;
+  auto wave_knn = [&](auto kern, uint32_t wpb) {
+    hipLaunchKernelGGL(kern, grid((cnt + wpb - 1) / wpb), dim3(64 * wpb), 0, s, spos, cell_start,
+                       PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn, knn_stripes(),
+                       proposal ? 1u : 0u);
+  };
   if (!big && L <= 256) {
-    hipLaunchKernelGGL((k_knn<512, 4>), grid((cnt + 3) / 4), dim3(256), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
+    if (sub)
+      wave_knn(k_knn<512, 4, true>, 4);
+    else
+      wave_knn(k_knn<512, 4, false>, 4);
   } else if (!big && L <= 512) {
-    hipLaunchKernelGGL((k_knn<1024, 2>), grid((cnt + 1) / 2), dim3(128), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
+    if (sub)
+      wave_knn(k_knn<1024, 2, true>, 2);
+    else
+      wave_knn(k_knn<1024, 2, false>, 2);
   } else if (!big && L <= 1024) {
-    hipLaunchKernelGGL((k_knn<2048, 1>), grid(cnt), dim3(64), 0, s, spos,
-                       cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, dn);
-  } else {
-    uint32_t cap = BIG_KNN_CAP;
+    if (sub)
+      wave_knn(k_knn<2048, 1, true>, 1);
+    else
+      wave_knn(k_knn<2048, 1, false>, 1);
+  } else {    uint32_t cap = BIG_KNN_CAP;
     while (cap < 2 * (uint64_t)L && cap < (1u << 24)) cap <<= 1;
     const uint64_t per_block = 2ull * cap * (sizeof(double) + sizeof(uint64_t));
     // in the allocation-free build, half of what is left of its scratch
@@ -732,9 +749,16 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
       queue = need(TT.get<uint32_t>(1));
       hipLaunchKernelGGL(k_order_by_len, dim3(1), dim3(1024), 0, s, Lper, cnt, perm, queue);
     }
-    hipLaunchKernelGGL(k_knn_big, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start, PR,
-                       wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, gkey2, gpk2,
-                       cap, dn, t_graph ? nullptr : t_knn_stat, (const uint32_t *)perm, queue);
+    if (sub)
+      hipLaunchKernelGGL(k_knn_big<true>, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start,
+                         PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk, gkey2,
+                         gpk2, cap, dn, t_graph ? nullptr : t_knn_stat, (const uint32_t *)perm,
+                         queue);
+    else
+      hipLaunchKernelGGL(k_knn_big<false>, dim3(blocks), dim3(KB_THREADS), 0, s, spos,
+                         cell_start, PR, wl, cnt, L, Lper, Rper, kofs, kpk, kcnt, kdist, gkey, gpk,
+                         gkey2, gpk2, cap, dn, t_graph ? nullptr : t_knn_stat,
+                         (const uint32_t *)perm, queue);
   }
   HIPCHK(hipGetLastError());
 }
@@ -779,7 +803,14 @@ struct GridData {
   BoxParams *d_boxes = nullptr;
   GridParams *d_grids = nullptr;
   uint32_t *d_coff = nullptr;
-  Problems PR() const { return Problems{d_boxes, d_grids, d_coff, prob_of_slot}; }
+  // the dense cells' sub-columns (SubCols; null for 1D and 2D builds)
+  uint32_t *sub_base = nullptr, *sub_start = nullptr, *sub_slot = nullptr, *sub_key = nullptr;
+  double *sub_xy = nullptr;
+  uint32_t dense_cells = 0;
+  Problems PR() const {
+    return Problems{d_boxes, d_grids, d_coff, prob_of_slot, sub_base, sub_start, sub_slot,
+                    sub_key, sub_xy};
+  }
 };
 
 void build_grid(GridData &D, const double *d_xyz, size_t n,
@@ -889,6 +920,39 @@ void build_grid(GridData &D, const double *d_xyz, size_t n,
   hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, (const double4 *)D.gpos,
                      (const uint32_t *)D.order, n, (const uint32_t *)D.prob, D.spos, D.slot_of,
                      D.prob_of_slot);
+  if (dim == 3) {
+    // sub-columns of the cells over SUB_MIN seeds (of the large-cell list
+    // the sort made).  A cell of m > 256 seeds has S^2 <= 3 m / 32 + 1
+    // sub-columns (S = ceil(sqrt(m / 16)) <= m / 64 + 1): tables of S^2 + 4
+    // words, at most n / 8 + n / 64 in all, and 2 (S + 1) <= 128 boundaries
+    const uint64_t pool_cap = (uint64_t)n / 8 + (uint64_t)n / 64 + 64;
+    const uint64_t xy_cap = (uint64_t)n / 2 + 256;
+    D.sub_base = need(TT.get<uint32_t>(ncell));
+    D.sub_start = need(TT.get<uint32_t>(pool_cap));
+    D.sub_xy = need(TT.get<double>(xy_cap));
+    D.sub_slot = need(TT.get<uint32_t>(n));
+    D.sub_key = need(TT.get<uint32_t>(n));
+    // words used in the two pools, then the cells with sub-columns
+    unsigned int *pool_cnt = need(TT.get<unsigned int>(3));
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncell)), dim3(256), 0, s, D.sub_base,
+                       (uint32_t)ncell, ~0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)pool_cnt, 3u, 0u);
+    hipLaunchKernelGGL(k_grid_subcols, dim3(2048), dim3(64), 0, s,
+                       (const uint32_t *)D.cell_start, (const uint32_t *)fill,
+                       (const unsigned int *)counts, (const double4 *)D.spos,
+                       (const GridParams *)D.d_grids, (const uint32_t *)D.d_coff,
+                       (const uint32_t *)D.prob_of_slot, D.sub_base, D.sub_start,
+                       (uint32_t)pool_cap, pool_cnt, D.sub_xy, (uint32_t)xy_cap, pool_cnt + 1,
+                       D.sub_slot, D.sub_key);
+    // no dense cell: the gathers run without the sub-column path
+    hipLaunchKernelGGL(k_word_u32, dim3(1), dim3(1), 0, s, (const uint32_t *)(pool_cnt + 1),
+                       host_word());
+    unsigned long long w2[2];
+    read_words(s, w2, 2);
+    D.dense_cells = (uint32_t)w2[1];
+    if (D.dense_cells == 0)
+      D.sub_base = D.sub_start = D.sub_slot = D.sub_key = nullptr, D.sub_xy = nullptr;
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -1064,8 +1128,6 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       std::max<uint32_t>(std::min<uint32_t>(opt.k_max ? opt.k_max : MV_K_MAX, MV_K_MAX), 2);
   const uint32_t k0 = std::min(std::max<uint32_t>(opt.k0 ? opt.k0 : 27, 2), kmax);
   float *Rfirst = nullptr;
-  // learned: each cell's proposed length before the first-list cap, by slot
-  float *kslot = nullptr;
   uint32_t lbound = k0;  // upper bound of the current main list's lengths
   double ms_model = 0;
   if (opt.mode == MV_MODE_LEARNED && nwork) {
@@ -1094,11 +1156,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     lbound = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(cap0, 16u), 1024u) & ~7u,
                                 kmax);
     const float rscale = opt.model->rscale > 0.f ? opt.model->rscale : 0.f;
-    // (a traced build plans a captured one, which grows its lists per pass)
-    if (!t_trace) kslot = need(TT.get<float>(n));
     hipLaunchKernelGGL(k_proposal, dim3(nblk(nwork)), dim3(256), 0, s, Y, hl, nwork,
-                       std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0,
-                       (const uint32_t *)lists[0], kslot);
+                       std::min<uint32_t>(16, kmax), lbound, kscale, rscale, lens[0], R0);
     // longest lists first (or grid order on request)
     const uint32_t NB = std::min<uint32_t>(lbound / 8 + 1, LEN_MAX_BUCKETS);
     uint32_t *hist = need(TT.get<uint32_t>(NB + 1));
@@ -1133,9 +1192,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   uint32_t *link_in = need(TT.get<uint32_t>(n));
   HIPCHK(hipMemsetAsync(link_in, 0, n * sizeof(uint32_t), s));
   uint64_t st_cap = std::max<uint64_t>(4096, 16ull * nwork);
-  unsigned long long *counters = need(TT.get<unsigned long long>(7));
+  unsigned long long *counters = need(TT.get<unsigned long long>(NCOUNTERS));
   // counters: [0] staging cursor, [1] error word, [2] exact tests, [3] overflow,
-  // [4..6] the long-list kernel's items, ball gathers and entries gathered
+  // [4..6] the long-list kernel's items, ball gathers and entries gathered,
+  // [7..71) the wavefront list kernels' ball gathers, [71..135) their
+  // candidates read (64 counters each)
   struct KnnStatScope {
     explicit KnnStatScope(unsigned long long *p) { t_knn_stat = p; }
     ~KnnStatScope() { t_knn_stat = nullptr; }
@@ -1233,7 +1294,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     st_area = need(TT.get<double>(st_cap));
     st_cen = need(TT.get<double>(3 * st_cap));
     st_norm = need(TT.get<double>(3 * st_cap));
-    HIPCHK(hipMemsetAsync(counters, 0, 7 * sizeof(unsigned long long), s));
+    HIPCHK(hipMemsetAsync(counters, 0, NCOUNTERS * sizeof(unsigned long long), s));
     HIPCHK(hipMemsetAsync(cs_cur, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long err_init = ~0ull;
     HIPCHK(hipMemcpyAsync(counters + 1, &err_init, sizeof err_init, hipMemcpyHostToDevice, s));
@@ -1290,6 +1351,8 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         // redone lists: the longest one picks the list kernel (the growth
         // bound is loose); read with the scan's total
         const uint32_t nhead = cnt - ntail;
+        // the first pass's radii (when given) are proposals
+        const bool first_prop = passes == 1 && R0 != nullptr;
         if (passes > 1) {
           hipLaunchKernelGGL(k_max_u32, dim3(1), dim3(1024), 0, s, (const uint32_t *)ll, nhead,
                              host_word() + 1);
@@ -1336,7 +1399,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
                      true, TT, s);
         } else if (!pipe) {
           launch_knn(spos, cell_start, PR, wl, cnt, std::max(lbnd, ltail), ll, R0, kofs, kpk,
-                     kcnt, kdist, big, TT, s, opt.knn_min_tile);
+                     kcnt, kdist, big, TT, s, opt.knn_min_tile, nullptr, 0, 0, first_prop);
         } else {
           cbnd.resize(pchunks + 1);
           for (uint32_t k = 0; k <= pchunks; k++) cbnd[k] = (uint32_t)((uint64_t)cnt * k / pchunks);
@@ -1346,12 +1409,13 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
           HIPCHK(hipEventRecord(pev.ev[0], s));
           HIPCHK(hipStreamWaitEvent(s2, pev.ev[0], 0));
           launch_knn(spos, cell_start, PR, wl, cbnd[1], lbnd, ll, R0, kofs, kpk, kcnt, kdist, false,
-                     TT, s, opt.knn_min_tile);
+                     TT, s, opt.knn_min_tile, nullptr, 0, 0, first_prop);
           for (uint32_t k = 1; k < pchunks; k++) {
             const uint32_t c0 = cbnd[k];
             launch_knn(spos, cell_start, PR, wl + c0, cbnd[k + 1] - c0, lbnd, ll + c0,
                        R0 ? R0 + c0 : nullptr, kofs + c0, kpk, kcnt + c0, kdist + c0, false, TT,
-                       s2, opt.knn_min_tile, nullptr, 0, (uint32_t)device_cus() * pblocks);
+                       s2, opt.knn_min_tile, nullptr, 0, (uint32_t)device_cus() * pblocks,
+                       first_prop);
             HIPCHK(hipEventRecord(pev.ev[k], s2));
           }
         }
@@ -1640,23 +1704,21 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       float *Rm_next = need(TT.get<float>(std::max<uint32_t>(nw_next, 1)));
       float *Rb_next = need(TT.get<float>(std::max<uint32_t>(nb_next + ncap, 1)));
       float *Rx_next = need(TT.get<float>(std::max<uint32_t>(nx_next + nbcap + nxcap, 1)));
-      // 2x to 4x per pass, from the safety radius at exhaustion
-      constexpr uint32_t GROW = 4;
+      // 2x to 4x per pass (to 16x past 4,096 entries), from the safety
+      // radius at exhaustion
+      constexpr uint32_t GROW = 16;
       if (nw_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
                            (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err,
-                           (const uint32_t *)nullptr, (const uint32_t *)cw2,
-                           (const float *)kslot);
+                           (const uint32_t *)nullptr);
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
                            (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err,
-                           (const uint32_t *)nullptr, (const uint32_t *)cb2,
-                           (const float *)kslot);
+                           (const uint32_t *)nullptr);
       if (nx_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nx_next)), dim3(256), 0, s, nx_next, cxl2,
                            (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err,
-                           (const uint32_t *)nullptr, (const uint32_t *)nullptr,
-                           (const float *)nullptr);
+                           (const uint32_t *)nullptr);
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
@@ -1698,19 +1760,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         }
       }
       const uint32_t lb_main_old = lb_main;
-      // bounds on the new lengths (k_redo_size grows them at most GROW x)
+      // bounds on the new lengths (k_redo_size grows them at most 4x, 16x
+      // from 4,096 entries)
       auto grown = [&](uint32_t lb) {
         return (uint32_t)std::min<uint64_t>((uint64_t)GROW * lb, kmax);
       };
       const uint32_t lb_big_old = lb_big;
       lb_main = grown(lb_main);
       lb_big = std::max(grown(lb_big), std::min(lb_main_old, kmax));
-      if (kslot) {
-        // learned redo lengths grow up to the proposal (the list kernels
-        // read the actual longest list)
-        lb_main = kmax;
-        lb_big = kmax;
-      }
       lb_wide = std::max(grown(lb_wide), std::min(lb_big_old, kmax));
       Rmain = Rm_next;
       Rbig = Rb_next;
@@ -1729,8 +1786,11 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       ping_x = 11 - ping_x;
     }
     if (!overflowed) {
-      unsigned long long cnts[7];
+      unsigned long long cnts[NCOUNTERS];
       HIPCHK(hipMemcpy(cnts, counters, sizeof cnts, hipMemcpyDeviceToHost));
+      T->stats.knn_gathers = T->stats.knn_reads = 0;
+      for (int k = 7; k < 71; k++) T->stats.knn_gathers += cnts[k];
+      for (int k = 71; k < NCOUNTERS; k++) T->stats.knn_reads += cnts[k];
       T->stats.exact_tests = cnts[2];
       T->stats.knn_big_items = cnts[4];
       T->stats.knn_big_gathers = cnts[5];
@@ -1764,6 +1824,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
   T->stats.big_cells = bigcells;
   T->stats.wide_cells = widecells;
   T->stats.pair_handovers = handovers;
+  T->stats.dense_cells = D.dense_cells;
   T->stats.ms_knn = ms_knn + ms_model;
   T->stats.ms_clip = ms_clip;
   T->stats.clip_wave_launches = n_cw;
@@ -2653,7 +2714,7 @@ void graph_body(mv_graph &G) {
     const float rscale = G.model.rscale > 0.f ? G.model.rscale : 0.f;
     hipLaunchKernelGGL(k_proposal, dim3(nb0), dim3(256), 0, s, (const float *)Y,
                        (const float *)hl, cap0, std::min<uint32_t>(16, kmax), G.lbound0,
-                       kscale, rscale, lens[0], R0, (const uint32_t *)nullptr, (float *)nullptr);
+                       kscale, rscale, lens[0], R0);
     hipLaunchKernelGGL(k_tail_fill_u32, dim3(nb0), dim3(256), 0, s, lens[0],
                        (const uint32_t *)cm, cap0, 0u);
     if (opt.flags & MV_FLAG_NO_LENGTH_ORDER) {
@@ -2895,8 +2956,7 @@ void graph_body(mv_graph &G) {
       if (capM2) {
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capM2)), dim3(256), 0, s, capM2, lens[im2],
                            (const double *)rr_m, (const double *)kk_m, Rm2, kmax, cap_err,
-                           (const uint32_t *)(cm + p + 1), (const uint32_t *)nullptr,
-                           (const float *)nullptr);
+                           (const uint32_t *)(cm + p + 1));
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lens[im2],
                            (const uint32_t *)(cm + p + 1), capM2, 0u);
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lists[im2],
@@ -2905,8 +2965,7 @@ void graph_body(mv_graph &G) {
       if (capB2) {
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capB2)), dim3(256), 0, s, capB2, lens[ib2],
                            (const double *)rr_b, (const double *)kk_b, Rb2, kmax, cap_err,
-                           (const uint32_t *)(cbn + p + 1), (const uint32_t *)nullptr,
-                           (const float *)nullptr);
+                           (const uint32_t *)(cbn + p + 1));
         // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_range_fill_f32, dim3(nblk(capB2)), dim3(256), 0, s, Rb2,
                            (const uint32_t *)(cbn + p + 1), (const uint32_t *)(cb + p + 1),
