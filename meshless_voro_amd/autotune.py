@@ -27,7 +27,7 @@ synchronised build).  The table is written to autotune.json beside this file
 and read by `tuned_options`.  Results do not depend on these parameters
 (every build is exact); only the time does.
 
-    python -m meshless_voro_amd.autotune [--quick]
+    python -m meshless_voro_amd.autotune [--quick] [--kinds uniform,clustered,batched]
 
 Needs a GPU.
 """
@@ -200,7 +200,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=TABLE)
     ap.add_argument("--quick", action="store_true", help="two sizes, two repetitions")
+    ap.add_argument("--kinds", default=",".join(KINDS),
+                    help="workload kinds to tune (the table keeps the others' entries)")
     args = ap.parse_args()
+    kinds = set(args.kinds.split(","))
+    if not kinds <= set(KINDS):
+        ap.error(f"--kinds: one or more of {', '.join(KINDS)}")
     import torch
     torch.cuda.init()  # torch's HIP runtime comes up before libmvoro's
     reps = 2 if args.quick else 3
@@ -215,7 +220,7 @@ def main():
     modes = [MODE_PLAIN, MODE_LEARNED] if have_model else [MODE_PLAIN]
     log = lambda s: print(s, flush=True)  # noqa: E731
     # held-out seeds (streams the benchmark sets do not use)
-    for n in sizes:
+    for n in sizes if "uniform" in kinds else []:
         for periodic in (True, False):
             x = W.uniform(n, 7000 + n.bit_length() + int(periodic))
             for mode in modes:
@@ -224,7 +229,7 @@ def main():
                 entries[k] = tune_one(x, periodic, mode, reps, log=log)
                 print(f"best {k}: {entries[k]}", flush=True)
     cl_sizes = [1 << 20] if args.quick else [1 << 20, 1 << 22]
-    for n in cl_sizes:
+    for n in cl_sizes if "clustered" in kinds else []:
         x = W.clustered(n, 7100 + n.bit_length())
         for mode in modes:
             k = _key(mode, True, n, "clustered")
@@ -232,7 +237,7 @@ def main():
             entries[k] = tune_one(x, True, mode, 2, log=log)
             print(f"best {k}: {entries[k]}", flush=True)
     probs = W.batched_problems(128 if args.quick else 512, 7200)
-    for mode in modes:
+    for mode in modes if "batched" in kinds else []:
         k = _key(mode, True, sum(len(x) for x, _ in probs), "batched")
         print(f"tune {k}", flush=True)
         entries[k] = tune_batch(probs, mode, 2, log=log)
