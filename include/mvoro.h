@@ -502,7 +502,10 @@ mv_status mv_build_device_noalloc(const double *d_xyz, size_t n,
  * are fixed at creation; d_xyz and d_mask must stay valid until
  * mv_graph_free.  Refused (MV_E_ARG): MV_FLAG_KEEP_CELLS,
  * MV_FLAG_EXACT_ORDER, cells_per_wave other than 0 or 1, no seeds.
- * Launches of one graph must not overlap. */
+ * A launch starts after the work queued before it on the null stream (as
+ * the ordinary build, which runs on it: writes to d_xyz queued there, or
+ * on a stream that synchronises with it, are seen) and returns when the
+ * results are complete.  Launches of one graph must not overlap. */
 typedef struct mv_graph mv_graph;
 mv_status mv_graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
                           const uint8_t *d_mask, const double *anchors, const double *widths,

@@ -1300,7 +1300,8 @@ __device__ __forceinline__ void flip_bitonic(K *key, P *pk, uint32_t m, uint32_t
 
 // The items of a long-list launch, longest lists first (64 length classes,
 // two per octave; any order within a class), and the launch's work counter
-// set to zero.  One block of 1024 threads.
+// set to zero.  One block of 1024 threads; dn: the item count on the device
+// (n bounds it).
 constexpr uint32_t LEN_CLASSES = 64;
 __device__ __forceinline__ uint32_t len_class(uint32_t L) {
   if (L < 2u) return LEN_CLASSES - 1u;
@@ -1310,8 +1311,9 @@ __device__ __forceinline__ uint32_t len_class(uint32_t L) {
 }
 __global__ void __launch_bounds__(1024)
     k_order_by_len(const uint32_t *__restrict__ Lper, uint32_t n, uint32_t *__restrict__ perm,
-                   uint32_t *__restrict__ queue) {
+                   uint32_t *__restrict__ queue, const uint32_t *__restrict__ dn) {
   __shared__ uint32_t cnt[LEN_CLASSES];
+  if (dn) n = min(*dn, n);
   const uint32_t tid = threadIdx.x;
   if (tid < LEN_CLASSES) cnt[tid] = 0u;
   if (tid == 0) *queue = 0u;

@@ -744,10 +744,10 @@ void launch_knn(const double4 *spos, const uint32_t *cell_start, const Problems 
     uint64_t *gpk2 = need(TT.get<uint64_t>((size_t)blocks * cap));
     // lists of different lengths: longest first, claimed one at a time
     uint32_t *perm = nullptr, *queue = nullptr;
-    if (Lper && !dn && cnt > 1) {
+    if (Lper && cnt > 1) {
       perm = need(TT.get<uint32_t>(cnt));
       queue = need(TT.get<uint32_t>(1));
-      hipLaunchKernelGGL(k_order_by_len, dim3(1), dim3(1024), 0, s, Lper, cnt, perm, queue);
+      hipLaunchKernelGGL(k_order_by_len, dim3(1), dim3(1024), 0, s, Lper, cnt, perm, queue, dn);
     }
     if (sub)
       hipLaunchKernelGGL(k_knn_big<true>, dim3(blocks), dim3(KB_THREADS), 0, s, spos, cell_start,
@@ -812,6 +812,40 @@ struct GridData {
                     sub_key, sub_xy};
   }
 };
+
+// The sub-column tables of the dense cells (of the large-cell list the
+// grid sort made), allocated from TT and built on s; cnt: words used in the
+// two pools, then the cells with sub-columns.  A cell of m > 256 seeds has
+// S^2 <= 3 m / 32 + 1 sub-columns (S = ceil(sqrt(m / 16)) <= m / 64 + 1):
+// tables of S^2 + 4 words, at most n / 8 + n / 64 in all, and 2 (S + 1) <=
+// 128 boundaries.
+struct SubTables {
+  uint32_t *base, *start, *slot, *key;
+  double *xy;
+  unsigned int *cnt;
+};
+SubTables launch_subcols(size_t n, size_t ncell, const uint32_t *cell_start,
+                         const uint32_t *big_list, const unsigned int *big_cnt,
+                         const double4 *spos, const GridParams *d_grids, const uint32_t *d_coff,
+                         const uint32_t *prob_of_slot, Temps &TT, hipStream_t s) {
+  const uint64_t pool_cap = (uint64_t)n / 8 + (uint64_t)n / 64 + 64;
+  const uint64_t xy_cap = (uint64_t)n / 2 + 256;
+  SubTables t;
+  t.base = need(TT.get<uint32_t>(ncell));
+  t.start = need(TT.get<uint32_t>(pool_cap));
+  t.xy = need(TT.get<double>(xy_cap));
+  t.slot = need(TT.get<uint32_t>(n));
+  t.key = need(TT.get<uint32_t>(n));
+  t.cnt = need(TT.get<unsigned int>(3));
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncell)), dim3(256), 0, s, t.base, (uint32_t)ncell,
+                     ~0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)t.cnt, 3u, 0u);
+  hipLaunchKernelGGL(k_grid_subcols, dim3(2048), dim3(64), 0, s, cell_start, big_list, big_cnt,
+                     spos, d_grids, d_coff, prob_of_slot, t.base, t.start, (uint32_t)pool_cap,
+                     t.cnt, t.xy, (uint32_t)xy_cap, t.cnt + 1, t.slot, t.key);
+  HIPCHK(hipGetLastError());
+  return t;
+}
 
 void build_grid(GridData &D, const double *d_xyz, size_t n,
                 const std::vector<ProblemSpec> &probs, uint32_t dim, double occ, Temps &TT,
@@ -921,37 +955,21 @@ void build_grid(GridData &D, const double *d_xyz, size_t n,
                      (const uint32_t *)D.order, n, (const uint32_t *)D.prob, D.spos, D.slot_of,
                      D.prob_of_slot);
   if (dim == 3) {
-    // sub-columns of the cells over SUB_MIN seeds (of the large-cell list
-    // the sort made).  A cell of m > 256 seeds has S^2 <= 3 m / 32 + 1
-    // sub-columns (S = ceil(sqrt(m / 16)) <= m / 64 + 1): tables of S^2 + 4
-    // words, at most n / 8 + n / 64 in all, and 2 (S + 1) <= 128 boundaries
-    const uint64_t pool_cap = (uint64_t)n / 8 + (uint64_t)n / 64 + 64;
-    const uint64_t xy_cap = (uint64_t)n / 2 + 256;
-    D.sub_base = need(TT.get<uint32_t>(ncell));
-    D.sub_start = need(TT.get<uint32_t>(pool_cap));
-    D.sub_xy = need(TT.get<double>(xy_cap));
-    D.sub_slot = need(TT.get<uint32_t>(n));
-    D.sub_key = need(TT.get<uint32_t>(n));
-    // words used in the two pools, then the cells with sub-columns
-    unsigned int *pool_cnt = need(TT.get<unsigned int>(3));
-    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncell)), dim3(256), 0, s, D.sub_base,
-                       (uint32_t)ncell, ~0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)pool_cnt, 3u, 0u);
-    hipLaunchKernelGGL(k_grid_subcols, dim3(2048), dim3(64), 0, s,
-                       (const uint32_t *)D.cell_start, (const uint32_t *)fill,
-                       (const unsigned int *)counts, (const double4 *)D.spos,
-                       (const GridParams *)D.d_grids, (const uint32_t *)D.d_coff,
-                       (const uint32_t *)D.prob_of_slot, D.sub_base, D.sub_start,
-                       (uint32_t)pool_cap, pool_cnt, D.sub_xy, (uint32_t)xy_cap, pool_cnt + 1,
-                       D.sub_slot, D.sub_key);
+    const SubTables t = launch_subcols(n, ncell, D.cell_start, fill, counts, D.spos, D.d_grids,
+                                       D.d_coff, D.prob_of_slot, TT, s);
     // no dense cell: the gathers run without the sub-column path
-    hipLaunchKernelGGL(k_word_u32, dim3(1), dim3(1), 0, s, (const uint32_t *)(pool_cnt + 1),
+    hipLaunchKernelGGL(k_word_u32, dim3(1), dim3(1), 0, s, (const uint32_t *)(t.cnt + 1),
                        host_word());
     unsigned long long w2[2];
     read_words(s, w2, 2);
     D.dense_cells = (uint32_t)w2[1];
-    if (D.dense_cells == 0)
-      D.sub_base = D.sub_start = D.sub_slot = D.sub_key = nullptr, D.sub_xy = nullptr;
+    if (D.dense_cells) {
+      D.sub_base = t.base;
+      D.sub_start = t.start;
+      D.sub_slot = t.slot;
+      D.sub_key = t.key;
+      D.sub_xy = t.xy;
+    }
   }
   HIPCHK(hipGetLastError());
 }
@@ -1011,6 +1029,7 @@ struct BuildTrace {
   size_t ncell = 0;
   uint32_t lbound0 = 0;
   uint64_t st_cap = 0, F = 0, NL = 0;
+  uint32_t dense_cells = 0;
 };
 thread_local BuildTrace *t_trace = nullptr;
 
@@ -1098,6 +1117,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
     t_trace->grids = D.grids;
     t_trace->coff = D.coff;
     t_trace->ncell = D.ncell;
+    t_trace->dense_cells = D.dense_cells;
   }
   const Problems PR = D.PR();
   const double4 *spos = D.spos;
@@ -1731,9 +1751,12 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       redone += nw_next + nb_next + nx_next;
       if (nw && !exact_order) cells_cw += nw - nw_next - ncap;
       // the main lists longer than the wavefront list kernels take go to
-      // its end (gathered by the long-list kernel)
+      // its end (gathered by the long-list kernel); not in a build that
+      // plans a captured one, whose passes gather a main list with one
+      // kernel (the two kernels accept different balls, and so leave
+      // different cells for the next pass)
       nw_tail = 0;
-      if (nw_next && (uint64_t)lb_main * GROW > MAIN_LIST_MAX && !exact_order) {
+      if (nw_next && (uint64_t)lb_main * GROW > MAIN_LIST_MAX && !exact_order && !t_trace) {
         Temps T3;
         uint32_t *fl = need(T3.get<uint32_t>(nw_next));
         uint32_t *ps = need(T3.get<uint32_t>(nw_next));
@@ -2537,6 +2560,9 @@ struct mv_graph {
   uint64_t st_cap = 0;
   size_t ncell = 0;
   uint32_t lbound0 = 0;
+  // the planning build had dense cells: the recorded build splits them into
+  // sub-columns too (so that its gathers start from the same radii)
+  bool sub = false;
   // device memory: constant tables, the status words and the arena the
   // recorded build carves every other buffer from
   std::shared_ptr<DevStore> store = std::make_shared<DevStore>();
@@ -2561,6 +2587,10 @@ struct mv_graph {
   // the recorded stream, and a second one the large-cell tier forks onto
   hipStream_t s = nullptr, s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // a launch starts after the work queued before it on the null stream
+  // (the caller's writes to the seeds), as the ordinary build, which runs
+  // on that stream
+  hipEvent_t ev_in = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   mv_stats plan_stats{};
@@ -2570,6 +2600,7 @@ struct mv_graph {
     if (graph) (void)hipGraphDestroy(graph);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_in) (void)hipEventDestroy(ev_in);
     if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
     if (h_status) (void)hipHostFree(h_status);
@@ -2665,7 +2696,16 @@ void graph_body(mv_graph &G) {
   hipLaunchKernelGGL(k_grid_gather, dim3(nb), dim3(256), 0, s, (const double4 *)gpos,
                      (const uint32_t *)order, n, (const uint32_t *)prob, spos, slot_of,
                      prob_of_slot);
-  const Problems PR{G.d_boxes, G.d_grids, G.d_coff, prob_of_slot};
+  Problems PR{G.d_boxes, G.d_grids, G.d_coff, prob_of_slot};
+  if (G.sub && G.dim == 3) {
+    const SubTables t = launch_subcols(n, ncell, cell_start, gfill, counts, spos, G.d_grids,
+                                       G.d_coff, prob_of_slot, TT, s);
+    PR.sub_base = t.base;
+    PR.sub_start = t.start;
+    PR.sub_slot = t.slot;
+    PR.sub_key = t.key;
+    PR.sub_xy = t.xy;
+  }
 
   // ---- the first pass's cells (active slots in grid order) and lengths
   uint32_t *cm = need(TT.get<uint32_t>(NPG + 1));   // main-tier cells per pass
@@ -2862,8 +2902,9 @@ void graph_body(mv_graph &G) {
       uint32_t *kcnt = need(TT.get<uint32_t>(capM));
       double *kdist = need(TT.get<double>(capM));
       kd_m = kdist;
+      // (the first pass's radii, when given, are proposals)
       launch_knn(spos, cell_start, PR, lists[im], capM, G.LbM[p], lens[im], Rm, kofs, kpk, kcnt,
-                 kdist, false, TT, s, opt.knn_min_tile, cm + p, 0);
+                 kdist, false, TT, s, opt.knn_min_tile, cm + p, 0, 0, p == 0 && Rm != nullptr);
       WaveArgs W;
       memset(&W, 0, sizeof W);
       W.spos = spos;
@@ -3132,20 +3173,27 @@ void plan_graph(mv_graph &G, const BuildTrace &tr) {
                 (uint64_t)G.capM[p] * std::min<uint32_t>(G.LbM[p], 64) + 4096;
       G.EB[p] = (uint64_t)((double)r.total[1] * 1.5) + (uint64_t)G.capB[p] * 1024 + 65536;
     } else {
-      // the spare pass: a few cells with lists up to four times the last
-      // recorded pass's
+      // the spare pass: a few cells with lists up to the growth of the last
+      // recorded pass's (at most fourfold from one pass to the next, or
+      // sixteenfold from 4,096 entries)
+      auto gf = [](uint32_t lb) -> uint32_t { return lb >= 4096u ? 16u : 4u; };
       G.capM[p] = grow(0, 1.0, 1024);
       G.capB[p] = grow(0, 1.0, 256);
-      G.LbM[p] = std::max<uint32_t>(4 * std::max<uint32_t>(last.lbnd[0], 2), 64);
-      G.LbB[p] = std::max<uint32_t>(4 * last.lbnd[1], 2048);
-      // (lists grow at most fourfold from one pass to the next)
-      G.EM[p] = std::max<uint64_t>((uint64_t)G.capM[p] * G.LbM[p], 4 * last.total[0]) + 4096;
-      G.EB[p] = std::max<uint64_t>((uint64_t)G.capB[p] * 1024, 4 * last.total[1]) + 65536;
+      G.LbM[p] = (uint32_t)std::min<uint64_t>(
+          n, std::max<uint64_t>((uint64_t)gf(last.lbnd[0]) * std::max<uint32_t>(last.lbnd[0], 2),
+                                64));
+      G.LbB[p] = (uint32_t)std::min<uint64_t>(
+          n, std::max<uint64_t>((uint64_t)gf(last.lbnd[1]) * last.lbnd[1], 2048));
+      G.EM[p] = std::max<uint64_t>((uint64_t)G.capM[p] * G.LbM[p],
+                                   (uint64_t)gf(last.lbnd[0]) * last.total[0]) + 4096;
+      G.EB[p] = std::max<uint64_t>((uint64_t)G.capB[p] * 1024,
+                                   (uint64_t)gf(last.lbnd[1]) * last.total[1]) + 65536;
     }
   }
   G.st_cap = std::max<uint64_t>(tr.F + tr.F / 4, 4096);
   G.ncell = tr.ncell;
   G.lbound0 = tr.lbound0;
+  G.sub = tr.dense_cells > 0;
 }
 
 mv_status graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
@@ -3253,6 +3301,7 @@ mv_status graph_create(const double *d_xyz, const uint64_t *offsets, uint32_t P,
     HIPCHK(hipStreamCreateWithFlags(&G->s2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&G->ev_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&G->ev_in, hipEventDisableTiming));
     // record once against a counting arena (its addresses are only kernel
     // arguments of a graph that never runs) to size the arena, then for real
     {
@@ -3283,6 +3332,8 @@ mv_status graph_launch(mv_graph *G, mv_tess **out, uint32_t *replayed) {
   const uint32_t P = G->P;
   auto t0 = std::chrono::steady_clock::now();
   try {
+    HIPCHK(hipEventRecord(G->ev_in, 0));
+    HIPCHK(hipStreamWaitEvent(G->s, G->ev_in, 0));
     HIPCHK(hipGraphLaunch(G->exec, G->s));
     HIPCHK(hipStreamSynchronize(G->s));
   } catch (...) {

@@ -1,7 +1,8 @@
 """Captured builds (mv_graph_create / mv_graph_launch): a replay of the
 recorded hipGraph gives the arrays of the ordinary build on the same seeds,
 bit for bit -- on the seeds it was recorded with, after the seeds moved in
-place, for many problems with masks, in 2D -- and a replay that needs more
+place, for many problems with masks, in 2D, over tight clusters (dense grid
+cells, long lists) -- and a replay that needs more
 than was recorded (clustered seeds written over uniform ones) runs the
 ordinary build instead, with the same results."""
 import numpy as np
@@ -81,6 +82,54 @@ def test_graph_many_problems_with_masks():
                 same(a, b)
             xs = [moved(x, p[3], 40 + step, 1e-4) for x, p in zip(xs, probs)]
             xt.copy_(torch.from_numpy(np.ascontiguousarray(np.concatenate(xs))))
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("mode", [0, MODE_LEARNED])
+def test_graph_replays_dense_cells_and_long_lists(mode):
+    """Tight clusters: dense grid cells (sub-columns) and rim cells whose
+    lists pass 1,024 entries.  The recorded build takes the planning build's
+    passes, so a replay on the same seeds runs the recording."""
+    import torch
+    from tests.test_dense_cells import tight_clusters
+    x = tight_clusters(24000, 61 + mode)
+    print(f"case tight clusters 24000 mode={mode}", flush=True)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    opt = BuildOptions(mode=mode)
+    ref = Voronoi.build(x, Z, O, 3, True, options=opt)
+    assert ref.stats["dense_cells"] > 0 and ref.stats["knn_big_items"] > 0
+    g = BuildGraph(xt, [(0, Z, O, True)], options=opt)
+    try:
+        for _ in range(2):
+            vs, rep = g.launch()
+            assert rep and g.last_violation() == 0
+            same(vs[0], ref)
+    finally:
+        g.close()
+
+
+def test_graph_launch_sees_writes_queued_before_it():
+    """Seeds moved by device work queued just before the launch (no
+    synchronisation in between): the replay reads the moved seeds."""
+    import torch
+    x = W.uniform(1 << 20, 71)
+    xt = torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    g = BuildGraph(xt, [(0, Z, O, True)], options=BuildOptions(mode=MODE_LEARNED))
+    try:
+        g.launch()
+        gen = torch.Generator(device="cuda:0")
+        gen.manual_seed(5)
+        d = 1e-4 * torch.randn(xt.shape, generator=gen, device="cuda:0", dtype=xt.dtype)
+        torch.cuda.synchronize()
+        # (a seed outside the box while the writes are in flight)
+        xt.add_(d)
+        xt.add_(2.0)
+        xt.remainder_(1.0)
+        vs, rep = g.launch()
+        assert rep, g.last_violation()
+        y = xt.cpu().numpy()
+        same(vs[0], Voronoi.build(y, Z, O, 3, True, options=BuildOptions(mode=MODE_LEARNED)))
     finally:
         g.close()
 
