@@ -833,13 +833,14 @@ __device__ __forceinline__ uint32_t knn_fill(
 
 // knn_fill with every wavefront of a KB_THREADS block gathering (all threads
 // call it; the count comes back through the LDS word *blk_cnt).
-// *iters: the gathers made.
+// *iters: the gathers made.  take_ball: the first ball is kept whatever its
+// count (up to cap; the list ends at the ball, certified by its radius).
 template <bool SUB>
 __device__ __forceinline__ uint32_t knn_fill_blk(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ cell_start,
     const SubCols &sc, const GridParams &g, const double *width, const double q[3], double &R,
     uint32_t Lw, double *bkey, uint64_t *bpk, uint32_t cap, uint32_t *blk_cnt, int wid,
-    int nw, uint32_t *iters) {
+    int nw, uint32_t *iters, bool take_ball = false) {
   double r_lo = 0., r_hi = 0.;
   uint32_t cnt = 0;
   for (int it = 0; it < 96; it++) {
@@ -851,7 +852,7 @@ __device__ __forceinline__ uint32_t knn_fill_blk(
     cnt = *blk_cnt;
     __syncthreads();
     bool all = (uint64_t)cnt >= g.total_stream;
-    if (cnt <= cap && (cnt >= Lw || all)) return cnt;
+    if (cnt <= cap && (cnt >= Lw || all || (take_ball && it == 0))) return cnt;
     if (cnt > cap) {
       r_hi = R;
       if (r_lo > 0.) {
@@ -1382,11 +1383,15 @@ __global__ void __launch_bounds__(KB_THREADS)
     const uint32_t pru = __builtin_amdgcn_readfirstlane(pr);
     const GridParams g = PR.grids[pru];
     const uint32_t *cell_start = cell_start_all + PR.coff[pru];
-    double R = (Rper && Rper[w] > 0.f) ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
+    // a negative radius: take the ball of that radius whole
+    const float rp = Rper ? Rper[w] : 0.f;
+    double R = rp > 0.f ? (double)rp : rp < 0.f ? -(double)rp
+                                                : knn_start_radius(cell_start, g, q, Lw);
     uint32_t iters = 0;
     const SubCols sc = sub_cols(PR, pru);
     const uint32_t cnt = knn_fill_blk<SUB>(spos, cell_start, sc, g, width, q, R, Lw, bkey, bpk,
-                                           cap, &s_gather, (int)wid, KB_THREADS / 64, &iters);
+                                           cap, &s_gather, (int)wid, KB_THREADS / 64, &iters,
+                                           rp < 0.f);
     if (gstat && tid == 0) {
       // items, ball gathers, entries gathered by the last one
       atomicAdd(gstat, 1ull);
@@ -1501,6 +1506,7 @@ __global__ void __launch_bounds__(KB_THREADS)
   }
 }
 
+// (ub, when given: the ball bounds of k_redo_count.)
 // Cells to redo get a longer list, sized from the cell's safety radius sr
 // when its list ran out and the distance kd of the first entry left out
 // (sr >= kd, else the cell would have stopped): the entries within sr are
@@ -1508,22 +1514,146 @@ __global__ void __launch_bounds__(KB_THREADS)
 // estimate (+10 %), at least 2x and at most 4x per pass (16x from 4,096
 // entries on: the few cells with such lists hold up each pass), and the next gather
 // starts at the radius expected to hold it.
+// An upper bound on the stream entries within r of q: the seeds of the grid
+// cells the ball meets (periodic images included).  One wavefront, the rows
+// of cells over the lanes; wave-uniform result.  The count stops once it
+// reaches `stop` (the result is then >= stop).
+__device__ __forceinline__ uint64_t ball_cells_count(const uint32_t *__restrict__ cell_start,
+                                                     const GridParams &g, const double *width,
+                                                     const double q[3], double r,
+                                                     uint64_t stop) {
+  const uint32_t lane = __lane_id();
+  const double rr = r + g.margin, rr2 = rr * rr;
+  uint64_t tot = 0;
+  for (int i = g.img_lo[0]; i <= g.img_hi[0]; i++)
+    for (int j = g.img_lo[1]; j <= g.img_hi[1]; j++)
+      for (int k = g.img_lo[2]; k <= g.img_hi[2]; k++) {
+        const int sv[3] = {i, j, k};
+        double qs[3];
+        int lo[3], hi[3];
+        bool miss = false;
+        for (int c = 0; c < 3; c++) {
+          qs[c] = q[c] + (double)sv[c] * width[c];
+          if (g.inv_h[c] == 0.) {
+            lo[c] = hi[c] = 0;
+            continue;
+          }
+          const double a = floor((qs[c] - rr - g.lo[c]) * g.inv_h[c]);
+          const double b = floor((qs[c] + rr - g.lo[c]) * g.inv_h[c]);
+          if (b < 0. || a > (double)(g.G[c] - 1)) miss = true;
+          lo[c] = a < 0. ? 0 : (int)fmin(a, (double)(g.G[c] - 1));
+          hi[c] = b > (double)(g.G[c] - 1) ? g.G[c] - 1 : (int)fmax(b, 0.);
+        }
+        if (miss) continue;
+        const int nb_ = hi[1] - lo[1] + 1;
+        const int nrows = (hi[0] - lo[0] + 1) * nb_;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+          const int rw = r0 + (int)lane;
+          uint32_t len = 0;
+          if (rw < nrows) {
+            const int a = lo[0] + rw / nb_, b = lo[1] + rw % nb_;
+            double d2 = 0.;
+            if (g.inv_h[0] != 0.) {
+              const double x0 = g.lo[0] + (double)a * g.h[0];
+              const double dx = fmax(0., fmax(x0 - qs[0], qs[0] - x0 - g.h[0]));
+              d2 += dx * dx;
+            }
+            if (g.inv_h[1] != 0.) {
+              const double y0 = g.lo[1] + (double)b * g.h[1];
+              const double dy = fmax(0., fmax(y0 - qs[1], qs[1] - y0 - g.h[1]));
+              d2 += dy * dy;
+            }
+            if (d2 <= rr2) {
+              int c0 = lo[2], c1 = hi[2];
+              if (g.inv_h[2] != 0.) {
+                const double hc = sqrt(rr2 - d2);
+                const double za = floor((qs[2] - hc - g.lo[2]) * g.inv_h[2]);
+                const double zb = floor((qs[2] + hc - g.lo[2]) * g.inv_h[2]);
+                c0 = max(c0, za < 0. ? 0 : (int)fmin(za, (double)g.G[2]));
+                c1 = min(c1, zb < 0. ? -1 : (int)fmin(zb, (double)g.G[2]));
+              }
+              if (c0 <= c1) {
+                const size_t row = ((size_t)a * g.G[1] + b) * g.G[2];
+                len = cell_start[row + c1 + 1] - cell_start[row + c0];
+              }
+            }
+          }
+          uint32_t t;
+          (void)wave_excl_scan(len, &t);
+          tot += t;
+          if (tot >= stop) return tot;
+        }
+      }
+  return tot;
+}
+
+// The length k_redo_size gives a redone list from its last length Lp, the
+// cell's safety radius sr when the list ran out and the distance kd of the
+// first entry left out.
+__device__ __forceinline__ double redo_length(uint32_t Lp, double sr, double kd) {
+  const double ratio = (kd > 0. && sr > kd && isfinite(sr)) ? sr / kd : 1.;
+  const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
+  // long lists (few cells, each pass of them mostly latency) grow up to 16x
+  const double lo = 2. * (double)Lp, hi = (Lp >= 4096u ? 16. : 4.) * (double)Lp;
+  return fmin(fmax(est, lo), hi);
+}
+
+// Redone lists of at least `lmin` entries: ub[w] = the seeds of the grid
+// cells within the safety radius sr[w] of the cell's generator (an upper
+// bound on the entries the cell can need, since its safety radius only
+// shrinks) when that is below the length redo_length gives, ~0u otherwise
+// (and for the other items, and balls over more than 16,384 rows of cells).
+// One wavefront per item.
+__global__ void __launch_bounds__(256)
+    k_redo_count(uint32_t nwork, const uint32_t *__restrict__ work,
+                 const uint32_t *__restrict__ Lp, const double *__restrict__ sr,
+                 const double *__restrict__ kd, const double4 *__restrict__ spos,
+                 const uint32_t *__restrict__ cell_start_all, Problems PR, uint32_t lmin,
+                 uint32_t *__restrict__ ub) {
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nwork) return;
+  const double r = sr[w];
+  const uint32_t slot = __builtin_amdgcn_readfirstlane(work[w]);
+  const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(PR, slot));
+  const GridParams g = PR.grids[pr];
+  double rows = 1.;
+  for (int c = 0; c < 2; c++)
+    if (g.inv_h[c] != 0.) rows *= fmin(2. * r * g.inv_h[c] + 2., (double)g.G[c]);
+  if (Lp[w] < lmin || !(r > 0.) || !isfinite(r) || rows > 16384.) {
+    if (__lane_id() == 0) ub[w] = ~0u;
+    return;
+  }
+  const uint64_t Ln = (uint64_t)redo_length(Lp[w], r, kd[w]);
+  const BoxParams &box = PR.boxes[pr];
+  const double width[3] = {box.width[0], box.width[1], box.width[2]};
+  const double4 qq = spos[slot];
+  const double q[3] = {qq.x, qq.y, qq.z};
+  // (a hair over r: the gather takes the ball of radius r (1 + 1e-6))
+  const uint64_t c =
+      ball_cells_count(cell_start_all + PR.coff[pr], g, width, q, r * (1. + 2e-6), Ln);
+  if (__lane_id() == 0) ub[w] = c < Ln ? (uint32_t)c : ~0u;
+}
+
 __global__ void k_redo_size(uint32_t nwork, uint32_t *__restrict__ Lio,
                             const double *__restrict__ sr, const double *__restrict__ kd,
                             float *__restrict__ Rout, uint32_t kmax,
                             unsigned int *__restrict__ cap_err,
-                            const uint32_t *__restrict__ dn) {
+                            const uint32_t *__restrict__ dn,
+                            const uint32_t *__restrict__ ub = nullptr) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwork || (dn && w >= *dn)) return;
   const uint32_t Lp = Lio[w];
   if (Lp >= kmax) atomicOr(cap_err, 1u);
   const double k = kd[w], r = sr[w];
-  const double ratio = (k > 0. && r > k && isfinite(r)) ? r / k : 1.;
-  const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
-  // long lists (few cells, each pass of them mostly latency) grow up to 16x
-  const double lo = 2. * (double)Lp, hi = (Lp >= 4096u ? 16. : 4.) * (double)Lp;
-  const double Ln = fmin(fmax(est, lo), hi);
-  const uint32_t Lnew = (uint32_t)fmin(Ln, (double)kmax);
+  const uint32_t Lnew = (uint32_t)fmin(redo_length(Lp, r, k), (double)kmax);
+  if (ub && ub[w] != ~0u && ub[w] > Lp && ub[w] < Lnew) {
+    // fewer seeds within the safety radius than that: the ball of that
+    // radius holds every entry the cell can need; the gather takes it whole
+    // (a negative radius)
+    Lio[w] = ub[w];
+    Rout[w] = -(float)(r * (1. + 1e-6));
+    return;
+  }
   Lio[w] = Lnew;
   Rout[w] = (k > 0. && isfinite(k)) ? (float)(k * cbrt((double)Lnew / (double)Lp)) : 0.f;
 }

@@ -1727,18 +1727,33 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       // 2x to 4x per pass (to 16x past 4,096 entries), from the safety
       // radius at exhaustion
       constexpr uint32_t GROW = 16;
+      // lists of 1,024 entries and more: bounded by the seeds of the grid
+      // cells within the safety radius (not in a build that plans a captured
+      // one, whose passes size lists by the estimate alone)
+      auto ball_bound = [&](uint32_t cnt, const uint32_t *wl, const uint32_t *ll,
+                            const double *rr, const double *kk) -> const uint32_t * {
+        if (!cnt || t_trace) return nullptr;
+        uint32_t *ub = need(TT.get<uint32_t>(cnt));
+        hipLaunchKernelGGL(k_redo_count, dim3((cnt + 3) / 4), dim3(256), 0, s, cnt, wl, ll, rr,
+                           kk, spos, (const uint32_t *)cell_start, PR, 1024u, ub);
+        HIPCHK(hipGetLastError());
+        return ub;
+      };
       if (nw_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nw_next)), dim3(256), 0, s, nw_next, cl2,
                            (const double *)rr_m, (const double *)kk_m, Rm_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr,
+                           ball_bound(nw_next, cw2, cl2, (const double *)rr_m, (const double *)kk_m));
       if (nb_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nb_next)), dim3(256), 0, s, nb_next, cbl2,
                            (const double *)rr_b, (const double *)kk_b, Rb_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr,
+                           ball_bound(nb_next, cb2, cbl2, (const double *)rr_b, (const double *)kk_b));
       if (nx_next)
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(nx_next)), dim3(256), 0, s, nx_next, cxl2,
                            (const double *)rr_x, (const double *)kk_x, Rx_next, kmax, cap_err,
-                           (const uint32_t *)nullptr);
+                           (const uint32_t *)nullptr,
+                           ball_bound(nx_next, cx2, cxl2, (const double *)rr_x, (const double *)kk_x));
       if (ncap)  // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(ncap)), dim3(256), 0, s,
                            (uint32_t *)(Rb_next + nb_next), ncap, 0u);
