@@ -262,6 +262,32 @@ __global__ void __launch_bounds__(256)
     if (uni) merge8(b8, nooff, std::false_type());
     zr = sqrt(b8[7]) * (1. + 1e-6);
     widen();
+  } else if (g.inv_h[2] != 0. && s1 - s0 > 8u) {
+    // a lane of a sparse cell with at least 8 other seeds: the 8th nearest
+    // of those bounds the final 8th, and the rows' scan of the dense cells
+    // next to it (rows of more than 32 seeds) is cut to |dz| below it, as
+    // above; the result is that of the full scan
+    double b8[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) b8[i] = INFINITY;
+    for (uint32_t t = s0; t < s1; t++) {
+      if (t == slot) continue;
+      const double4 p = spos[t];
+      const double o[3] = {p.x - q[0], p.y - q[1], p.z - q[2]};
+      double v = (o[0] * o[0] + o[1] * o[1]) + o[2] * o[2];
+      if (!(v < b8[7])) continue;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        if (v < b8[i]) {
+          const double tv = b8[i];
+          b8[i] = v;
+          v = tv;
+        }
+      }
+    }
+    zr = sqrt(b8[7]) * (1. + 1e-6);
+    zlo = q[2] - zr;
+    zhi = q[2] + zr;
   }
   // entries [t0, t1) with image shift (sx, sy, sz) into the sorted 8
   // (ZS = false: no shift along z)
