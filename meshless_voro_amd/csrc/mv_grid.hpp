@@ -1603,15 +1603,16 @@ __device__ __forceinline__ double redo_length(uint32_t Lp, double sr, double kd)
 // bound on the entries the cell can need, since its safety radius only
 // shrinks) when that is below the length redo_length gives, ~0u otherwise
 // (and for the other items, and balls over more than 16,384 rows of cells).
-// One wavefront per item.
+// One wavefront per item; dn: the item count on the device (nwork bounds
+// it).
 __global__ void __launch_bounds__(256)
     k_redo_count(uint32_t nwork, const uint32_t *__restrict__ work,
                  const uint32_t *__restrict__ Lp, const double *__restrict__ sr,
                  const double *__restrict__ kd, const double4 *__restrict__ spos,
                  const uint32_t *__restrict__ cell_start_all, Problems PR, uint32_t lmin,
-                 uint32_t *__restrict__ ub) {
+                 uint32_t *__restrict__ ub, const uint32_t *__restrict__ dn) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= nwork) return;
+  if (w >= nwork || (dn && w >= *dn)) return;
   const double r = sr[w];
   const uint32_t slot = __builtin_amdgcn_readfirstlane(work[w]);
   const uint32_t pr = __builtin_amdgcn_readfirstlane(problem_of(PR, slot));

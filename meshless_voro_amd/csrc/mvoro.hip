@@ -1728,14 +1728,14 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
       // radius at exhaustion
       constexpr uint32_t GROW = 16;
       // lists of 1,024 entries and more: bounded by the seeds of the grid
-      // cells within the safety radius (not in a build that plans a captured
-      // one, whose passes size lists by the estimate alone)
+      // cells within the safety radius
       auto ball_bound = [&](uint32_t cnt, const uint32_t *wl, const uint32_t *ll,
                             const double *rr, const double *kk) -> const uint32_t * {
-        if (!cnt || t_trace) return nullptr;
+        if (!cnt) return nullptr;
         uint32_t *ub = need(TT.get<uint32_t>(cnt));
         hipLaunchKernelGGL(k_redo_count, dim3((cnt + 3) / 4), dim3(256), 0, s, cnt, wl, ll, rr,
-                           kk, spos, (const uint32_t *)cell_start, PR, 1024u, ub);
+                           kk, spos, (const uint32_t *)cell_start, PR, 1024u, ub,
+                           (const uint32_t *)nullptr);
         HIPCHK(hipGetLastError());
         return ub;
       };
@@ -3009,19 +3009,34 @@ void graph_body(mv_graph &G) {
       if (capB)
         hipLaunchKernelGGL(k_graph_any_state, dim3(nblk(capB)), dim3(256), 0, s,
                            (const uint8_t *)bstate, capB, (uint8_t)CS_CAPACITY, GV_WIDE, viol);
+      // (long lists bounded by the seeds within the safety radius, as in
+      // the ordinary build)
+      auto ball_bound = [&](uint32_t cap, const uint32_t *wl, const uint32_t *ll,
+                            const double *rr, const double *kk,
+                            const uint32_t *dcnt) -> const uint32_t * {
+        uint32_t *ub = need(TT.get<uint32_t>(cap));
+        hipLaunchKernelGGL(k_redo_count, dim3((cap + 3) / 4), dim3(256), 0, s, cap, wl, ll, rr,
+                           kk, (const double4 *)spos, (const uint32_t *)cell_start, PR, 1024u,
+                           ub, dcnt);
+        return ub;
+      };
       if (capM2) {
+        const uint32_t *ubm = ball_bound(capM2, lists[im2], lens[im2], (const double *)rr_m,
+                                         (const double *)kk_m, (const uint32_t *)(cm + p + 1));
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capM2)), dim3(256), 0, s, capM2, lens[im2],
                            (const double *)rr_m, (const double *)kk_m, Rm2, kmax, cap_err,
-                           (const uint32_t *)(cm + p + 1));
+                           (const uint32_t *)(cm + p + 1), ubm);
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lens[im2],
                            (const uint32_t *)(cm + p + 1), capM2, 0u);
         hipLaunchKernelGGL(k_tail_fill_u32, dim3(nblk(capM2)), dim3(256), 0, s, lists[im2],
                            (const uint32_t *)(cm + p + 1), capM2, 0u);
       }
       if (capB2) {
+        const uint32_t *ubb = ball_bound(capB2, lists[ib2], lens[ib2], (const double *)rr_b,
+                                         (const double *)kk_b, (const uint32_t *)(cbn + p + 1));
         hipLaunchKernelGGL(k_redo_size, dim3(nblk(capB2)), dim3(256), 0, s, capB2, lens[ib2],
                            (const double *)rr_b, (const double *)kk_b, Rb2, kmax, cap_err,
-                           (const uint32_t *)(cbn + p + 1));
+                           (const uint32_t *)(cbn + p + 1), ubb);
         // moved cells keep their list; their gather starts from the density
         hipLaunchKernelGGL(k_range_fill_f32, dim3(nblk(capB2)), dim3(256), 0, s, Rb2,
                            (const uint32_t *)(cbn + p + 1), (const uint32_t *)(cb + p + 1),
