@@ -691,6 +691,9 @@ constexpr uint32_t BIG_KNN_BLOCKS = 1024;
 constexpr uint64_t BIG_KNN_BYTES = 8ull << 30;
 // the longest list the wavefront list kernels build (2,048-entry LDS tile)
 constexpr uint32_t MAIN_LIST_MAX = 1024;
+// redone lists from this length on are bounded by the seeds within the
+// cell's safety radius (k_redo_count)
+constexpr uint32_t REDO_COUNT_MIN = 1024;
 
 // The list kernel for lists of at most L entries: the smallest LDS tile
 // (512, 1024, 2048 gathered entries) that holds twice L and at least
@@ -1734,7 +1737,7 @@ void build_multi(const std::vector<mv_tess *> &Ts, const double *d_xyz, size_t n
         if (!cnt) return nullptr;
         uint32_t *ub = need(TT.get<uint32_t>(cnt));
         hipLaunchKernelGGL(k_redo_count, dim3((cnt + 3) / 4), dim3(256), 0, s, cnt, wl, ll, rr,
-                           kk, spos, (const uint32_t *)cell_start, PR, 1024u, ub,
+                           kk, spos, (const uint32_t *)cell_start, PR, REDO_COUNT_MIN, ub,
                            (const uint32_t *)nullptr);
         HIPCHK(hipGetLastError());
         return ub;
@@ -3016,7 +3019,7 @@ void graph_body(mv_graph &G) {
                             const uint32_t *dcnt) -> const uint32_t * {
         uint32_t *ub = need(TT.get<uint32_t>(cap));
         hipLaunchKernelGGL(k_redo_count, dim3((cap + 3) / 4), dim3(256), 0, s, cap, wl, ll, rr,
-                           kk, (const double4 *)spos, (const uint32_t *)cell_start, PR, 1024u,
+                           kk, (const double4 *)spos, (const uint32_t *)cell_start, PR, REDO_COUNT_MIN,
                            ub, dcnt);
         return ub;
       };
