@@ -1587,6 +1587,9 @@ __device__ __forceinline__ uint64_t ball_cells_count(const uint32_t *__restrict_
   return tot;
 }
 
+// redone lists from this length on may grow 16x per pass (4x below)
+constexpr uint32_t GROW16_MIN = 4096;
+
 // The length k_redo_size gives a redone list from its last length Lp, the
 // cell's safety radius sr when the list ran out and the distance kd of the
 // first entry left out.
@@ -1594,7 +1597,7 @@ __device__ __forceinline__ double redo_length(uint32_t Lp, double sr, double kd)
   const double ratio = (kd > 0. && sr > kd && isfinite(sr)) ? sr / kd : 1.;
   const double est = 1.1 * (double)Lp * ratio * ratio * ratio;
   // long lists (few cells, each pass of them mostly latency) grow up to 16x
-  const double lo = 2. * (double)Lp, hi = (Lp >= 4096u ? 16. : 4.) * (double)Lp;
+  const double lo = 2. * (double)Lp, hi = (Lp >= GROW16_MIN ? 16. : 4.) * (double)Lp;
   return fmin(fmax(est, lo), hi);
 }
 

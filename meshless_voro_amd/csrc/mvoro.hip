@@ -3209,7 +3209,7 @@ void plan_graph(mv_graph &G, const BuildTrace &tr) {
       // the spare pass: a few cells with lists up to the growth of the last
       // recorded pass's (at most fourfold from one pass to the next, or
       // sixteenfold from 4,096 entries)
-      auto gf = [](uint32_t lb) -> uint32_t { return lb >= 4096u ? 16u : 4u; };
+      auto gf = [](uint32_t lb) -> uint32_t { return lb >= GROW16_MIN ? 16u : 4u; };
       G.capM[p] = grow(0, 1.0, 1024);
       G.capB[p] = grow(0, 1.0, 256);
       G.LbM[p] = (uint32_t)std::min<uint64_t>(
