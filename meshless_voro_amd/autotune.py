@@ -19,7 +19,7 @@ uneven problems (half uniform, half in 8 clusters, mixed boundaries):
   length_order        learned mode: order cells by proposed list length
                       (longest first) or keep grid order (locality)
   first_list_cap      learned mode: the longest first-pass list (the edge of
-                      the k buckets the list kernels serve: 128, 256, 512)
+                      the k buckets the list kernels serve: 64 to 512)
 
 Coordinate descent from the library defaults, each candidate timed as the
 minimum over repeated device-resident builds (wall time around a
@@ -54,7 +54,7 @@ SPACE = {
     "cells_per_wave": [1, 2, 64],
     "knn_min_tile": [512, 1024, 2048],
     "length_order": [True, False],
-    "first_list_cap": [128, 256, 512],
+    "first_list_cap": [64, 96, 128, 256, 512],
 }
 LEARNED_ONLY = ("length_order", "first_list_cap")
 DEFAULT = {"occupancy16": 32, "clip_blocks_per_cu": 4, "cells_per_wave": 1,
