@@ -2,8 +2,9 @@
 neighbour gathers: builds over a few tight clusters, where the gathers read
 those sub-columns, the long lists of the clusters' rim cells included, give
 the oracle's connectivity and integrals in both boundary modes and both
-modes; a batched build over such problems equals the builds one by one; and
-a set with no dense cell builds no sub-columns."""
+modes, also in a box far from the origin with unequal sides; a batched build
+over such problems equals the builds one by one; and a set with no dense
+cell builds no sub-columns."""
 import numpy as np
 import pytest
 
@@ -43,6 +44,21 @@ def test_dense_cells_match_oracle(periodic, mode):
     # rim cells of the clusters read whole clusters (long lists)
     assert v.stats["knn_big_items"] > 0
     o = O.build(x, ZERO, ONE, 3, periodic)
+    assert_connectivity_equal(v, o)
+    assert_integrals_close(v, o)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_dense_cells_in_a_shifted_stretched_box(periodic):
+    """The sub-column boundaries and the gathers' bounds in absolute
+    coordinates far from the origin, with unequal box sides."""
+    anchor = np.array([-7.25, 3.5, 1000.0])
+    width = np.array([0.5, 3.0, 0.02])
+    x = anchor + width * tight_clusters(20000, 350 + periodic)
+    print(f"case shifted box periodic={periodic}", flush=True)
+    v = Voronoi.build(x, anchor, width, 3, periodic)
+    assert v.stats["dense_cells"] > 0
+    o = O.build(x, anchor, width, 3, periodic)
     assert_connectivity_equal(v, o)
     assert_integrals_close(v, o)
 
