@@ -1141,13 +1141,13 @@ __global__ void __launch_bounds__(256)
     const uint32_t Lw = Lper ? Lper[w] : L;
     const SubCols sc = sub_cols(PR, pr);
     // radii of redone lists come from the previous ball; first radii
-    // (proposed, or from the density of a block of grid cells) are bounded
-    // by the density inside dense cells
+    // (proposed, or from the density of a block of grid cells) inside dense
+    // cells come from the density in the seed's sub-column
     const bool given = Rper && Rper[w] > 0.f;
     double R = given ? (double)Rper[w] : knn_start_radius(cell_start, g, q, Lw);
     if (SUB && (!given || proposal)) {
       const double Rd = dense_start_radius(spos, sc, g, q, Lw);
-      if (Rd > 0.) R = given ? fmin(R, 1.5 * Rd) : Rd;
+      if (Rd > 0.) R = Rd;
     }
     // a ball with three quarters of the requested entries is kept (its list
     // ends at the ball, certified by its radius) rather than gathered again:
